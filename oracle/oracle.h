@@ -1,0 +1,131 @@
+/*
+ * CPU ORACLE for the ray-traversal + path-integration hot path.
+ *
+ * This is TEST INFRASTRUCTURE. It restates, on the CPU and in plain C++, the
+ * algorithms of the reference renderer (IlluminaPRT) that the GPU library
+ * replaces, so that tests can check the GPU results against it. Only tests/,
+ * __graft_entry__.smoke() and bench.py's CPU-baseline leg may load it, and
+ * only as the checker or as the timed CPU baseline. The product (the HIP
+ * library under illuminaprt_amd/) never calls it.
+ *
+ * Every function cites the reference file:line it restates (paths relative
+ * to the reference tree root). Functions whose results no golden vector,
+ * closed form, invariant or library routine pins are marked
+ * "parity unpinned" here and in the test that uses them.
+ */
+#ifndef ILM_ORACLE_H
+#define ILM_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- leaves (known-answer tests) ------------------------------------- */
+
+/* Shirley ray/triangle test, Core/Shape/IndexedTriangle.cpp:92-133 (closest)
+ * and :256-289 (any). Returns 1 on hit and fills t, beta (weight of p1),
+ * gamma (weight of p2). Ray.Min is ignored, as in the reference. */
+int orc_tri_intersect(const float p0[3], const float p1[3], const float p2[3],
+                      const float o[3], const float d[3], float ray_max,
+                      float* t, float* beta, float* gamma);
+
+/* Slab test, Core/Geometry/BoundingBox.cpp:157-240. Uses the exact 1/d
+ * inverse cache (Core/Geometry/Ray.cpp:27-34, Vector3.inl:321-339). */
+int orc_aabb_intersect(const float bmin[3], const float bmax[3],
+                       const float o[3], const float d[3],
+                       float ray_min, float ray_max, float* tin, float* tout);
+
+/* MT19937 with 69069 seeding, Core/Maths/Random.cpp:36-110; NextFloat
+ * :117-120. Writes n raw words (or floats) after seeding with `seed`. */
+void orc_mt_words(uint32_t seed, uint32_t* out, int n);
+void orc_mt_floats(uint32_t seed, float* out, int n);
+
+/* Philox4x64-10 (Salmon et al., SC'11) — the counter-based generator both
+ * the oracle ("repro" mode) and the GPU path use. out[4]. */
+void orc_philox4x64(const uint64_t ctr[4], const uint64_t key[2], uint64_t out[4]);
+
+/* Warps, Core/Maths/Montecarlo.h:30-37 (cosine hemisphere), :60-64 (triangle). */
+void orc_cosine_hemisphere(float u, float v, float out[3]);
+void orc_uniform_triangle(float u, float v, float* b1, float* b2);
+
+/* Camera ray, Core/Camera/PerspectiveCamera.cpp:44-51 with the frame of
+ * Camera.cpp:30-44/:72-78, Basis.cpp:28-44 and the half-pixel shift of
+ * Filter/BoxFilter.cpp:20-28. cam = {pos[3], target[3], up[3], fov_deg, aspect}. */
+void orc_camera_ray(const float cam[11], int width, int height, int px, int py,
+                    float u, float v, float o[3], float d[3]);
+
+/* ---- meshes and BVHMesh ---------------------------------------------- */
+
+typedef struct orc_mesh orc_mesh;
+
+/* Triangle mesh (Core/Shape/TriangleMesh.h:27-95): vertex positions and
+ * triangle indices; the hit index of a triangle is its position in idx. */
+orc_mesh* orc_mesh_create(const float* pos, int64_t nv, const int32_t* idx, int64_t nt);
+void orc_mesh_destroy(orc_mesh* m);
+
+/* BVHMesh::Compile, Core/Shape/BVHMesh.cpp:78-107,203-269 with
+ * TreeMesh.cpp:44-85,159-212. Returns the node count. */
+int64_t orc_bvh_build(orc_mesh* m, int max_leaf, int max_depth);
+
+/* stats: internal, leaves, max depth, refs, max leaf size, empty leaves */
+void orc_bvh_stats(const orc_mesh* m, int64_t out[6]);
+
+/* Export nodes (pre-order) for invariant tests: box[6*n] = min xyz, max xyz;
+ * child[2*n] (-1 for leaves); leaf_first/leaf_count index into refs. */
+void orc_bvh_export(const orc_mesh* m, float* box, int32_t* child,
+                    int32_t* leaf_first, int32_t* leaf_count, int32_t* refs);
+
+/* Trace n rays; rays are 8 floats each: ox oy oz tmin dx dy dz tmax.
+ * mode 0: closest hit (BVHMesh.cpp:114-119,169-198 / BasicMesh.cpp:16-45),
+ * mode 1: any hit (BVHMesh.cpp:125-130,143-165 / BasicMesh.cpp:47-64).
+ * accel 0: brute force (BasicMesh restatement), 1: BVHMesh restatement.
+ * prim[i] = hit triangle (-1 on miss); t and bary (2 floats) may be NULL.
+ * nthreads <= 0 uses all hardware threads. Returns the threads used. */
+int orc_trace(const orc_mesh* m, const float* rays, int64_t n, int mode, int accel,
+              int32_t* prim, float* t, float* bary, int nthreads);
+
+/* ---- scenes and the path integrator ---------------------------------- */
+
+typedef struct orc_scene orc_scene;
+
+/* Material record, 8 floats: kd rgb, ks rgb, phong exponent, type
+ * (0 = Lambertian (Material/LambertianBxDF.cpp), 1 = Lambertian + Phong lobe;
+ * the Phong lobe has no counterpart in the reference). */
+/* Light record, 4 floats: Le rgb, unused. The triangles whose tri_light
+ * equals the light's index form its emitter mesh. */
+orc_scene* orc_scene_create(const float* pos, const float* nrm, int64_t nv,
+                            const int32_t* idx, const int32_t* tri_mat,
+                            const int32_t* tri_light, int64_t nt,
+                            const float* mats, int nmat,
+                            const float* lights, int nlight);
+void orc_scene_destroy(orc_scene* s);
+
+typedef struct {
+    float cam[11];      /* pos, target, up, fov_deg, aspect */
+    int32_t width, height;
+    int32_t spp;        /* samples per pixel rendered by this call */
+    int32_t spp_offset; /* first sample index (progressive rendering) */
+    int32_t max_depth;  /* PathIntegrator RayDepth */
+    int32_t rr_depth;   /* RR applies when depth >= rr_depth (reference: 4) */
+    float eps;          /* reflect epsilon (reference factory: 1e-4) */
+    uint32_t seed;
+    int32_t rng_mode;   /* 0: Philox counter-based, 1: MT19937 per tile */
+    int32_t x0, y0, x1, y1; /* pixel window (x1/y1 exclusive; 0 = full) */
+} orc_render_params;
+
+/* RenderThread-style pool (Utility/Sandbox/Multithreaded.h:28-264) over
+ * 32x32 tiles, each pixel rendered as in BaseRenderer::RenderRegion
+ * (Core/Renderer/BaseRenderer.cpp:62-137) with PathIntegrator::Radiance
+ * (Core/Integrator/PathIntegrator.cpp:44-201,357-367). Adds the radiance
+ * SUM over the spp samples of each pixel into accum (width*height*3 doubles,
+ * row-major, y = camera row). ray_counts (may be NULL) receives
+ * {extension+camera rays, shadow rays}. Returns the threads used. */
+int orc_render(const orc_scene* s, const orc_render_params* p, double* accum,
+               int64_t ray_counts[2], int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
