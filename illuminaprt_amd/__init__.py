@@ -1,0 +1,326 @@
+"""MI355X ray traversal and path integration for IlluminaPRT scenes.
+
+Python bindings over the C ABI of ``libilm_gpu.so`` (declared in
+``include/ilm_gpu.h``). The library is built in-tree by
+``illuminaprt_amd.build.build()``. Every compute call runs on the GPU; when
+no HIP device is visible, ``Context`` raises ``IlmError`` (there is no CPU
+fallback).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libilm_gpu.so")
+_lib = None
+
+ILM_BUILDER_SAH = 0
+ILM_BUILDER_REFERENCE = 1
+ILM_SPAWN_SHADOW = 0
+ILM_SPAWN_COSINE = 1
+RAY_BYTES = 32
+HIT_BYTES = 16
+HIT_DTYPE = np.dtype([("t", np.float32), ("prim", np.int32), ("u", np.float32), ("v", np.float32)])
+
+
+class IlmError(RuntimeError):
+    pass
+
+
+class BvhParams(C.Structure):
+    _fields_ = [("builder", C.c_int32), ("width", C.c_int32), ("max_leaf", C.c_int32),
+                ("max_depth", C.c_int32), ("threads", C.c_int32)]
+
+
+class BvhInfo(C.Structure):
+    _fields_ = [("nodes", C.c_int64), ("node_bytes", C.c_int64), ("tri_bytes", C.c_int64),
+                ("width", C.c_int32), ("depth", C.c_int32), ("max_leaf", C.c_int32),
+                ("stack_need", C.c_int32), ("build_ms", C.c_double), ("sah_cost", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Camera(C.Structure):
+    _fields_ = [("pos", C.c_float * 3), ("target", C.c_float * 3), ("up", C.c_float * 3),
+                ("fov_deg", C.c_float), ("aspect", C.c_float)]
+
+    @classmethod
+    def make(cls, pos, target, up=(0, 1, 0), fov_deg=40.0, aspect=1.0):
+        c = cls()
+        for i in range(3):
+            c.pos[i], c.target[i], c.up[i] = pos[i], target[i], up[i]
+        c.fov_deg, c.aspect = fov_deg, aspect
+        return c
+
+    def as_list(self):
+        return [*self.pos, *self.target, *self.up, self.fov_deg, self.aspect]
+
+
+class RenderParams(C.Structure):
+    _fields_ = [("cam", Camera), ("width", C.c_int32), ("height", C.c_int32), ("spp", C.c_int32),
+                ("spp_offset", C.c_int32), ("max_depth", C.c_int32), ("rr_depth", C.c_int32),
+                ("eps", C.c_float), ("seed", C.c_uint32), ("batch_paths", C.c_int32)]
+
+
+class RenderStats(C.Structure):
+    _fields_ = [("camera_rays", C.c_int64), ("extension_rays", C.c_int64), ("shadow_rays", C.c_int64),
+                ("ms", C.c_double)]
+
+
+# name -> (restype, argtypes); every exported symbol of include/ilm_gpu.h
+_vp, _i32, _i64, _u32, _f32, _sz = C.c_void_p, C.c_int32, C.c_int64, C.c_uint32, C.c_float, C.c_size_t
+SIGNATURES = {
+    "ilm_ctx_create": (C.c_int, [C.POINTER(_vp), C.c_int]),
+    "ilm_ctx_destroy": (C.c_int, [_vp]),
+    "ilm_last_error": (C.c_char_p, []),
+    "ilm_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+    "ilm_device_info": (C.c_int, [_vp, C.c_char_p, C.c_int, C.POINTER(C.c_int)]),
+    "ilm_alloc": (C.c_int, [_vp, _sz, C.POINTER(_vp)]),
+    "ilm_free": (C.c_int, [_vp, _vp]),
+    "ilm_upload": (C.c_int, [_vp, _vp, _vp, _sz]),
+    "ilm_download": (C.c_int, [_vp, _vp, _vp, _sz]),
+    "ilm_memset": (C.c_int, [_vp, _vp, C.c_int, _sz]),
+    "ilm_sync": (C.c_int, [_vp]),
+    "ilm_timer_mark": (C.c_int, [_vp, C.POINTER(C.c_int)]),
+    "ilm_timer_elapsed": (C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(C.c_float)]),
+    "ilm_mesh_create": (C.c_int, [_vp, _vp, _i64, _vp, _i64, C.POINTER(_vp)]),
+    "ilm_mesh_destroy": (C.c_int, [_vp]),
+    "ilm_bvh_build": (C.c_int, [_vp, C.POINTER(BvhParams)]),
+    "ilm_bvh_get_info": (C.c_int, [_vp, C.POINTER(BvhInfo)]),
+    "ilm_bvh_export": (C.c_int, [_vp, _vp, _vp]),
+    "ilm_trace_closest": (C.c_int, [_vp, _vp, _i64, _vp, _vp]),
+    "ilm_trace_any": (C.c_int, [_vp, _vp, _i64, _vp, _vp]),
+    "ilm_trace_stats": (C.c_int, [_vp, _vp, _i64, C.c_int, _vp, _vp]),
+    "ilm_raygen_camera": (C.c_int, [_vp, C.POINTER(Camera), C.c_int, C.c_int, C.c_int, _u32, _u32, _vp, _vp]),
+    "ilm_raygen_camera_list": (C.c_int, [_vp, C.POINTER(Camera), C.c_int, C.c_int, _vp, _i64, _i64, _u32, _vp, _vp]),
+    "ilm_raygen_secondary": (C.c_int, [_vp, _vp, _vp, _vp, _i64, C.c_int, _vp, _f32, _u32, _vp, _vp]),
+    "ilm_compact_hits": (C.c_int, [_vp, _vp, _i64, _vp, _vp, _vp]),
+    "ilm_scenegen_icosphere_size": (C.c_int, [C.c_int, C.POINTER(_i64), C.POINTER(_i64)]),
+    "ilm_scenegen_icosphere": (C.c_int, [C.c_int, _f32, C.c_int, _u32, _vp, _vp]),
+    "ilm_scenegen_boxes": (C.c_int, [C.c_int, _f32, _f32, _f32, _u32, _vp, _vp, _i64]),
+    "ilm_scenegen_vertex_normals": (C.c_int, [_vp, _i64, _vp, _i64, _vp]),
+    "ilm_scene_create": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.c_int, _vp, C.c_int, C.POINTER(_vp)]),
+    "ilm_scene_destroy": (C.c_int, [_vp]),
+    "ilm_render": (C.c_int, [_vp, C.POINTER(RenderParams), _vp, C.POINTER(RenderStats)]),
+}
+
+
+def load():
+    """Loads libilm_gpu.so (built by illuminaprt_amd.build.build())."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise IlmError(f"{LIB_PATH} is missing: run __graft_entry__.build() (hipcc, gfx950)")
+    L = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != 0:
+        msg = load().ilm_last_error()
+        raise IlmError(f"ilm error {rc}: {msg.decode() if msg else ''}")
+
+
+def gpu_available() -> bool:
+    n = C.c_int(0)
+    check(load().ilm_device_count(C.byref(n)))
+    return n.value > 0
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+# ---------------------------------------------------------------- device memory
+class DeviceBuffer:
+    def __init__(self, ctx: "Context", nbytes: int):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        check(load().ilm_alloc(ctx.h, self.nbytes, C.byref(p)))
+        self.ptr = p.value
+
+    def upload(self, arr: np.ndarray, offset: int = 0):
+        arr = np.ascontiguousarray(arr)
+        assert offset + arr.nbytes <= self.nbytes
+        check(load().ilm_upload(self.ctx.h, self.ptr + offset, _ptr(arr), arr.nbytes))
+        return self
+
+    def download(self, dtype, count=None, offset: int = 0):
+        dtype = np.dtype(dtype)
+        if count is None:
+            count = (self.nbytes - offset) // dtype.itemsize
+        out = np.empty(count, dtype)
+        assert offset + out.nbytes <= self.nbytes
+        check(load().ilm_download(self.ctx.h, _ptr(out), self.ptr + offset, out.nbytes))
+        return out
+
+    def zero(self):
+        check(load().ilm_memset(self.ctx.h, self.ptr, 0, self.nbytes))
+        return self
+
+    def free(self):
+        if self.ptr:
+            check(load().ilm_free(self.ctx.h, self.ptr))
+            self.ptr = None
+
+
+class Context:
+    """A device context (one stream, owned allocations and scratch)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        check(load().ilm_ctx_create(C.byref(h), device))
+        self.h = h.value
+        name = C.create_string_buffer(64)
+        cu = C.c_int()
+        check(load().ilm_device_info(self.h, name, 64, C.byref(cu)))
+        self.name, self.cu_count = name.value.decode(), cu.value
+
+    def close(self):
+        if self.h:
+            check(load().ilm_ctx_destroy(self.h))
+            self.h = None
+
+    def buffer(self, nbytes) -> DeviceBuffer:
+        return DeviceBuffer(self, nbytes)
+
+    def to_device(self, arr: np.ndarray) -> DeviceBuffer:
+        arr = np.ascontiguousarray(arr)
+        return DeviceBuffer(self, max(arr.nbytes, 16)).upload(arr)
+
+    def sync(self):
+        check(load().ilm_sync(self.h))
+
+    def mark(self) -> int:
+        s = C.c_int()
+        check(load().ilm_timer_mark(self.h, C.byref(s)))
+        return s.value
+
+    def elapsed_ms(self, s0: int, s1: int) -> float:
+        ms = C.c_float()
+        check(load().ilm_timer_elapsed(self.h, s0, s1, C.byref(ms)))
+        return ms.value
+
+    # ---- ray sets
+    def raygen_camera(self, cam: Camera, width, height, out: DeviceBuffer, jitter=False, seed=0, sample=0):
+        check(load().ilm_raygen_camera(self.h, C.byref(cam), width, height, int(jitter), seed, sample, out.ptr, None))
+
+    def raygen_camera_list(self, cam: Camera, width, height, pixels: DeviceBuffer, npix, n, out: DeviceBuffer,
+                           seed=0):
+        check(load().ilm_raygen_camera_list(self.h, C.byref(cam), width, height, pixels.ptr, npix, n, seed,
+                                            out.ptr, None))
+
+    def compact_hits(self, hits: DeviceBuffer, n, idx_out: DeviceBuffer) -> int:
+        cnt = self.buffer(8)
+        check(load().ilm_compact_hits(self.h, hits.ptr, n, idx_out.ptr, cnt.ptr, None))
+        c = int(cnt.download(np.int64, 1)[0])
+        cnt.free()
+        return c
+
+
+class Mesh:
+    """A triangle mesh with a GPU BVH (replaces the reference's BVHMesh)."""
+
+    def __init__(self, ctx: Context | None, pos: np.ndarray, idx: np.ndarray):
+        self.ctx = ctx
+        self.pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+        self.idx = np.ascontiguousarray(idx, np.int32).reshape(-1, 3)
+        h = C.c_void_p()
+        check(load().ilm_mesh_create(ctx.h if ctx else None, _ptr(self.pos), len(self.pos), _ptr(self.idx),
+                                     len(self.idx), C.byref(h)))
+        self.h = h.value
+
+    @property
+    def ntri(self):
+        return len(self.idx)
+
+    def build(self, builder=ILM_BUILDER_SAH, width=4, max_leaf=0, max_depth=0, threads=0) -> dict:
+        p = BvhParams(builder, width, max_leaf, max_depth, threads)
+        check(load().ilm_bvh_build(self.h, C.byref(p)))
+        return self.info()
+
+    def info(self) -> dict:
+        i = BvhInfo()
+        check(load().ilm_bvh_get_info(self.h, C.byref(i)))
+        return i.as_dict()
+
+    def export(self):
+        info = self.info()
+        nodes = np.zeros(info["node_bytes"], np.uint8)
+        order = np.zeros(self.ntri, np.int32)
+        check(load().ilm_bvh_export(self.h, _ptr(nodes), _ptr(order)))
+        return nodes, order
+
+    def trace_closest(self, rays: DeviceBuffer, n: int, hits: DeviceBuffer):
+        check(load().ilm_trace_closest(self.h, rays.ptr, n, hits.ptr, None))
+
+    def trace_any(self, rays: DeviceBuffer, n: int, occ: DeviceBuffer):
+        check(load().ilm_trace_any(self.h, rays.ptr, n, occ.ptr, None))
+
+    def trace_stats(self, rays: DeviceBuffer, n: int, any_hit: bool, visits: DeviceBuffer):
+        check(load().ilm_trace_stats(self.h, rays.ptr, n, int(any_hit), visits.ptr, None))
+
+    def spawn(self, rays: DeviceBuffer, hits: DeviceBuffer, n: int, kind: int, out: DeviceBuffer, light=None,
+              eps=1e-4, seed=0, src: DeviceBuffer | None = None):
+        lp = None
+        if light is not None:
+            light = np.ascontiguousarray(light, np.float32).reshape(9)
+            lp = _ptr(light)
+        check(load().ilm_raygen_secondary(self.h, rays.ptr, hits.ptr, src.ptr if src else None, n, kind, lp,
+                                          eps, seed, out.ptr, None))
+
+    def close(self):
+        if self.h:
+            check(load().ilm_mesh_destroy(self.h))
+            self.h = None
+
+
+# ---------------------------------------------------------------- host-side helpers
+def rays_to_device(ctx: Context, rays: np.ndarray) -> DeviceBuffer:
+    rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+    return ctx.to_device(rays)
+
+
+def icosphere(level: int, amplitude=0.0, octaves=0, seed=0):
+    L = load()
+    nv, nt = C.c_int64(), C.c_int64()
+    check(L.ilm_scenegen_icosphere_size(level, C.byref(nv), C.byref(nt)))
+    pos = np.zeros((nv.value, 3), np.float32)
+    idx = np.zeros((nt.value, 3), np.int32)
+    check(L.ilm_scenegen_icosphere(level, amplitude, octaves, seed, _ptr(pos), _ptr(idx)))
+    return pos, idx
+
+
+def boxes(n: int, emin=0.01, emax=0.1, radius=3.0, seed=11, vertex_offset=0):
+    pos = np.zeros((24 * n, 3), np.float32)
+    idx = np.zeros((12 * n, 3), np.int32)
+    check(load().ilm_scenegen_boxes(n, emin, emax, radius, seed, _ptr(pos), _ptr(idx), vertex_offset))
+    return pos, idx
+
+
+def vertex_normals(pos, idx):
+    pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+    idx = np.ascontiguousarray(idx, np.int32).reshape(-1, 3)
+    nrm = np.zeros_like(pos)
+    check(load().ilm_scenegen_vertex_normals(_ptr(pos), len(pos), _ptr(idx), len(idx), _ptr(nrm)))
+    return nrm
+
+
+def decode_bvh4(nodes_bytes: np.ndarray):
+    """Splits an exported BVH4 node array into boxes (n,4,6) and child words (n,4)."""
+    raw = nodes_bytes.view(np.uint8).reshape(-1, 112)
+    f = raw[:, :96].copy().view(np.float32).reshape(-1, 6, 4)   # lox,hix,loy,hiy,loz,hiz
+    child = raw[:, 96:].copy().view(np.uint32).reshape(-1, 4)
+    boxes = np.stack([f[:, 0], f[:, 2], f[:, 4], f[:, 1], f[:, 3], f[:, 5]], axis=-1)  # (n,4,6) lo xyz, hi xyz
+    return boxes, child

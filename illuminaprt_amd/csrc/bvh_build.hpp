@@ -1,0 +1,50 @@
+// Host BVH construction and wide-node flattening.
+#pragma once
+
+#include <stdint.h>
+
+#include <vector>
+
+namespace ilm {
+
+struct Bvh2Node {
+    float lo[3], hi[3];
+    int32_t left = -1, right = -1;  // children (internal)
+    int32_t first = 0, count = 0;   // primitive range in Bvh2::order (leaf if count > 0 or left < 0)
+    bool leaf() const { return left < 0; }
+};
+
+struct Bvh2 {
+    std::vector<Bvh2Node> nodes;   // nodes[0] is the root
+    std::vector<int32_t> order;    // primitive index of each leaf slot
+};
+
+// Binned SAH over centroids (32 bins, 3 axes), leaf size <= max_leaf.
+Bvh2 build_sah(const float* pos, const int32_t* idx, int64_t nt, int max_leaf, int threads);
+
+// The reference BVHMesh topology (Shape/BVHMesh.cpp:78-107,203-269 with
+// Shape/TreeMesh.cpp:44-85,159-212): 128-bin count heuristic over the node
+// box, round-robin axes after the largest-extent first axis, midpoint
+// distribution, leaf when n <= max_leaf, depth == max_depth or the node
+// radius <= root radius / 1000.
+Bvh2 build_reference(const float* pos, const int32_t* idx, int64_t nt, int max_leaf, int max_depth);
+
+struct WideBvh {
+    int width = 4;
+    std::vector<uint8_t> nodes;    // WideNode<width> array, root at 0
+    int64_t nnodes = 0;
+    std::vector<int32_t> order;    // original triangle index per triangle slot
+    int depth = 0, max_leaf = 0, stack_need = 0;
+    double sah_cost = 0;
+};
+
+// Collapses a binary tree into a wide tree: each wide node repeatedly opens
+// its largest-area internal child until it holds `width` children.
+WideBvh collapse(const Bvh2& b, int width);
+
+// Triangle array in leaf order for the device: 3 float4 per triangle,
+// (v0, original index as int bits), (v1, 0), (v2, 0).
+void pack_triangles(const float* pos, const int32_t* idx, const std::vector<int32_t>& order,
+                    std::vector<float>& out);
+
+}  // namespace ilm

@@ -1,0 +1,248 @@
+// Math shared by the traversal and shading kernels (gfx950). A few helpers
+// (node layout, camera frame) are also used by host code that builds the
+// inputs of those kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cfloat>
+#include <cmath>
+
+#define ILM_HD __host__ __device__ __forceinline__
+
+namespace ilm {
+
+struct f3 {
+    float x, y, z;
+};
+ILM_HD f3 mk(float x, float y, float z) { return f3{x, y, z}; }
+ILM_HD f3 operator+(f3 a, f3 b) { return f3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+ILM_HD f3 operator-(f3 a, f3 b) { return f3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+ILM_HD f3 operator-(f3 a) { return f3{-a.x, -a.y, -a.z}; }
+ILM_HD f3 operator*(f3 a, float s) { return f3{a.x * s, a.y * s, a.z * s}; }
+ILM_HD f3 operator*(float s, f3 a) { return f3{s * a.x, s * a.y, s * a.z}; }
+ILM_HD f3 operator/(f3 a, float s) { return f3{a.x / s, a.y / s, a.z / s}; }
+ILM_HD f3 mul(f3 a, f3 b) { return f3{a.x * b.x, a.y * b.y, a.z * b.z}; }
+ILM_HD float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+ILM_HD f3 cross(f3 a, f3 b) { return f3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+ILM_HD float length(f3 a) { return sqrtf(dot(a, a)); }
+ILM_HD f3 normalize(f3 a) { float l = length(a); return f3{a.x / l, a.y / l, a.z / l}; }
+ILM_HD float comp(f3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+
+// ------------------------------------------------------------------ BVH layout
+// Wide node, W children, SoA boxes then child words: 7 x 16 B for W = 4.
+// child word: bit 31 clear -> internal node index; bit 31 set -> leaf with
+// first triangle (bits 5..30) and triangle count (bits 0..4). A leaf with
+// count 0 is an empty slot (its box is a far-away point).
+template <int W>
+struct alignas(16) WideNode {
+    float lox[W], hix[W], loy[W], hiy[W], loz[W], hiz[W];
+    uint32_t child[W];
+};
+static_assert(sizeof(WideNode<4>) == 112, "BVH4 node is 112 bytes");
+static_assert(sizeof(WideNode<8>) == 224, "BVH8 node is 224 bytes");
+static_assert(sizeof(WideNode<2>) == 64, "BVH2 node is 64 bytes (padded)");
+
+constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr uint32_t kNoNode = 0x7FFFFFFFu;  // stack sentinel
+constexpr int kLeafCountBits = 5;
+constexpr int kMaxLeafTris = 31;
+ILM_HD bool is_leaf(uint32_t c) { return (c & kLeafBit) != 0; }
+ILM_HD uint32_t leaf_first(uint32_t c) { return (c & ~kLeafBit) >> kLeafCountBits; }
+ILM_HD uint32_t leaf_count(uint32_t c) { return c & ((1u << kLeafCountBits) - 1u); }
+ILM_HD uint32_t make_leaf(uint32_t first, uint32_t count) { return kLeafBit | (first << kLeafCountBits) | count; }
+
+// ------------------------------------------------------------------ t window
+// The reference accepts a triangle hit when 1e-5 <= t <= Max - 1e-5, both
+// compared in double (Shape/IndexedTriangle.cpp:114-115). For float t these
+// are t >= kTLo and t <= round_down((double)Max - 1e-5).
+constexpr float kTLo = 1.00000008e-05f;  // smallest float >= 1e-5 (double)
+
+ILM_HD float round_down_f(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __double2float_rd(x);
+#else
+    float f = (float)x;
+    if ((double)f > x) f = nextafterf(f, -INFINITY);
+    return f;
+#endif
+}
+ILM_HD float tmax_window(float tmax) { return round_down_f((double)tmax - 1e-5); }
+
+// ------------------------------------------------------------------ rays
+struct RayState {
+    f3 o, d;
+    f3 inv;   // 1/d with zero components replaced by +-1e-18
+    f3 oinv;  // o * inv
+    int kx, ky, kz;
+    float Sx, Sy, Sz;
+    float tlo, thi;  // accepted t window (thi shrinks with closest hits)
+};
+
+ILM_HD float safe_rcp(float d) {
+    const float tiny = 1e-18f;
+    return 1.0f / (fabsf(d) > tiny ? d : copysignf(tiny, d));
+}
+
+ILM_HD void ray_setup(RayState& r, f3 o, f3 d, float tmax) {
+    r.o = o;
+    r.d = d;
+    r.inv = mk(safe_rcp(d.x), safe_rcp(d.y), safe_rcp(d.z));
+    r.oinv = mk(o.x * r.inv.x, o.y * r.inv.y, o.z * r.inv.z);
+    // Watertight test setup (Woop, Benthin, Wald, JCGT 2013): kz = dominant axis.
+    float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    int kz = (ax > ay) ? ((ax > az) ? 0 : 2) : ((ay > az) ? 1 : 2);
+    int kx = kz + 1 == 3 ? 0 : kz + 1;
+    int ky = kx + 1 == 3 ? 0 : kx + 1;
+    float dkz = comp(d, kz);
+    if (dkz < 0.0f) { int t = kx; kx = ky; ky = t; }
+    r.kx = kx; r.ky = ky; r.kz = kz;
+    r.Sx = comp(d, kx) / dkz;
+    r.Sy = comp(d, ky) / dkz;
+    r.Sz = 1.0f / dkz;
+    r.tlo = kTLo;
+    r.thi = tmax_window(tmax);
+}
+
+// ------------------------------------------------------------------ box test
+// Slab test of child i; returns entry distance, or +inf on a miss. The exit
+// distance is relaxed by 2^-21 relative so that rounding of the fma form
+// never culls a box that the ray touches.
+template <int W>
+ILM_HD float slab(const WideNode<W>& n, int i, const RayState& r) {
+    float tx0 = fmaf(n.lox[i], r.inv.x, -r.oinv.x), tx1 = fmaf(n.hix[i], r.inv.x, -r.oinv.x);
+    float ty0 = fmaf(n.loy[i], r.inv.y, -r.oinv.y), ty1 = fmaf(n.hiy[i], r.inv.y, -r.oinv.y);
+    float tz0 = fmaf(n.loz[i], r.inv.z, -r.oinv.z), tz1 = fmaf(n.hiz[i], r.inv.z, -r.oinv.z);
+    float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), r.tlo));
+    float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), r.thi));
+    return (tn <= tf * 1.00000048f) ? tn : INFINITY;
+}
+
+// ------------------------------------------------------------------ triangle test
+// Watertight ray/triangle test (Woop, Benthin, Wald 2013). The edge functions
+// are evaluated without fused multiply-add so that the two triangles sharing
+// an edge compute exactly negated values; zero edge functions are recomputed
+// in double. Accepts tlo <= t <= thi. u, v = barycentric weights of v1, v2.
+ILM_HD float mul_rn(float a, float b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __fmul_rn(a, b);
+#else
+    return a * b;
+#endif
+}
+ILM_HD float sub_rn(float a, float b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __fsub_rn(a, b);
+#else
+    return a - b;
+#endif
+}
+
+ILM_HD bool tri_watertight(const RayState& r, f3 v0, f3 v1, f3 v2, float& t_out, float& u_out, float& v_out) {
+    f3 A = v0 - r.o, B = v1 - r.o, C = v2 - r.o;
+    float Akz = comp(A, r.kz), Bkz = comp(B, r.kz), Ckz = comp(C, r.kz);
+    float Ax = fmaf(-r.Sx, Akz, comp(A, r.kx)), Ay = fmaf(-r.Sy, Akz, comp(A, r.ky));
+    float Bx = fmaf(-r.Sx, Bkz, comp(B, r.kx)), By = fmaf(-r.Sy, Bkz, comp(B, r.ky));
+    float Cx = fmaf(-r.Sx, Ckz, comp(C, r.kx)), Cy = fmaf(-r.Sy, Ckz, comp(C, r.ky));
+    float U = sub_rn(mul_rn(Cx, By), mul_rn(Cy, Bx));
+    float V = sub_rn(mul_rn(Ax, Cy), mul_rn(Ay, Cx));
+    float Wf = sub_rn(mul_rn(Bx, Ay), mul_rn(By, Ax));
+    if (U == 0.0f || V == 0.0f || Wf == 0.0f) {
+        double CxBy = (double)Cx * (double)By, CyBx = (double)Cy * (double)Bx;
+        double AxCy = (double)Ax * (double)Cy, AyCx = (double)Ay * (double)Cx;
+        double BxAy = (double)Bx * (double)Ay, ByAx = (double)By * (double)Ax;
+        U = (float)(CxBy - CyBx);
+        V = (float)(AxCy - AyCx);
+        Wf = (float)(BxAy - ByAx);
+    }
+    if ((U < 0.0f || V < 0.0f || Wf < 0.0f) && (U > 0.0f || V > 0.0f || Wf > 0.0f)) return false;
+    float det = U + V + Wf;
+    if (det == 0.0f) return false;
+    float T = U * (r.Sz * Akz) + V * (r.Sz * Bkz) + Wf * (r.Sz * Ckz);
+    float rcp = 1.0f / det;
+    float t = T * rcp;
+    if (!(t >= r.tlo && t <= r.thi)) return false;
+    t_out = t;
+    u_out = V * rcp;
+    v_out = Wf * rcp;
+    return true;
+}
+
+// ------------------------------------------------------------------ RNG
+// Philox4x64-10 (Salmon, Moraes, Dror, Shaw, SC'11). Counter-based: the
+// sample stream of a path is a pure function of (seed, pixel, sample,
+// bounce, stream), so renders are reproducible and need no shared state.
+ILM_HD void mulhilo64(uint64_t a, uint64_t b, uint64_t& hi, uint64_t& lo) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    lo = a * b;
+    hi = __umul64hi(a, b);
+#else
+    unsigned __int128 p = (unsigned __int128)a * b;
+    hi = (uint64_t)(p >> 64);
+    lo = (uint64_t)p;
+#endif
+}
+
+ILM_HD void philox4x64(uint64_t c0, uint64_t c1, uint64_t c2, uint64_t c3, uint64_t k0, uint64_t k1, uint64_t out[4]) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r) { k0 += 0x9E3779B97F4A7C15ull; k1 += 0xBB67AE8584CAA73Bull; }
+        uint64_t hi0, lo0, hi1, lo1;
+        mulhilo64(0xD2E7470EE14C6C93ull, c0, hi0, lo0);
+        mulhilo64(0xCA5A826395121157ull, c2, hi1, lo1);
+        uint64_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+// u32 -> [0, 1): the reference's Random::NextFloat mapping (Maths/Random.cpp:117-120).
+ILM_HD float u32_to_unit(uint32_t u) {
+    return ((float)((double)u * 2.3283064370807974e-10)) * (1.0f - FLT_EPSILON);
+}
+
+// 8 uniform floats for (seed, pixel, sample, bounce, stream).
+struct Block8 { float v[8]; };
+ILM_HD Block8 sample_block(uint32_t seed, uint64_t pixel, uint32_t sample, uint32_t bounce, uint32_t stream) {
+    uint64_t o[4];
+    philox4x64(pixel, ((uint64_t)bounce << 32) | sample, stream, 0, seed, 0, o);
+    Block8 b;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        b.v[2 * i] = u32_to_unit((uint32_t)o[i]);
+        b.v[2 * i + 1] = u32_to_unit((uint32_t)(o[i] >> 32));
+    }
+    return b;
+}
+
+// ------------------------------------------------------------------ frames and warps
+struct Basis { f3 U, V, W; };
+// OrthonormalBasis::InitFromW (Geometry/Basis.cpp:85-99).
+ILM_HD Basis basis_from_w(f3 w) {
+    Basis b;
+    b.W = normalize(w);
+    if (fabsf(b.W.x) > fabsf(b.W.y)) {
+        float inv = 1.f / sqrtf(b.W.x * b.W.x + b.W.z * b.W.z);
+        b.V = mk(-b.W.z * inv, 0.f, b.W.x * inv);
+    } else {
+        float inv = 1.f / sqrtf(b.W.y * b.W.y + b.W.z * b.W.z);
+        b.V = mk(0.f, b.W.z * inv, -b.W.y * inv);
+    }
+    b.U = cross(b.W, b.V);
+    return b;
+}
+ILM_HD f3 to_local(const Basis& b, f3 v) { return mk(dot(b.U, v), dot(b.V, v), dot(b.W, v)); }
+ILM_HD f3 to_world(const Basis& b, f3 v) { return v.x * b.U + v.y * b.V + v.z * b.W; }
+
+constexpr float kPi = 3.14159265358979f;
+constexpr float kInvPi = 0.318309886183791f;
+
+// Montecarlo::CosineSampleHemisphere (Maths/Montecarlo.h:30-37).
+ILM_HD f3 cosine_hemisphere(float u, float v) {
+    float a = 2.0f * kPi * v;
+    float b = sqrtf(u);
+    return mk(b * cosf(a), b * sinf(a), sqrtf(1.0f - u));
+}
+
+}  // namespace ilm

@@ -1,0 +1,64 @@
+// Internal state behind the C ABI: error reporting, contexts, meshes.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <mutex>
+#include <set>
+#include <string>
+#include <vector>
+
+#include "../../include/ilm_gpu.h"
+
+namespace ilm {
+
+// Records the calling thread's last error and returns the code.
+int fail(int code, const std::string& msg);
+int fail_hip(hipError_t e, const char* what);
+
+#define ILM_HIP(call)                                   \
+    do {                                                \
+        hipError_t e_ = (call);                         \
+        if (e_ != hipSuccess) return ::ilm::fail_hip(e_, #call); \
+    } while (0)
+
+}  // namespace ilm
+
+struct ilm_ctx {
+    int device = 0;
+    int cu_count = 0;
+    std::string name;
+    hipStream_t stream = nullptr;
+    std::set<void*> allocs;
+    std::vector<hipEvent_t> events;  // timer slots
+    int next_event = 0;
+    // per-context scratch for traversal: ray-fetch counters and the
+    // traversal-stack spill area, grown on demand
+    unsigned long long* counters = nullptr;  // [0] next ray, [1] finished waves, [2] error flags
+    uint32_t* spill = nullptr;
+    size_t spill_words = 0;
+    std::mutex mu;
+};
+
+struct ilm_mesh {
+    ilm_ctx* ctx = nullptr;
+    std::vector<float> pos;
+    std::vector<int32_t> idx;
+    int64_t nv = 0, nt = 0;
+    // device data
+    void* d_nodes = nullptr;
+    float* d_tris = nullptr;  // 12 floats per triangle, leaf order
+    int32_t* d_slot_of = nullptr;  // original triangle index -> slot (built on demand)
+    int width = 0;
+    ilm_bvh_info info{};
+    std::vector<uint8_t> h_nodes;
+    std::vector<int32_t> h_order;
+    bool built = false;
+};
+
+namespace ilm {
+// Ensures ctx->spill holds at least `words` 32-bit words.
+int ensure_spill(ilm_ctx* ctx, size_t words);
+hipStream_t pick_stream(ilm_ctx* ctx, void* stream);
+}  // namespace ilm

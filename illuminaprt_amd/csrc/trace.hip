@@ -1,0 +1,534 @@
+// Ray traversal kernels for CDNA4 (gfx950) and the tracing entry points of
+// the C ABI.
+//
+// Design: persistent workgroups sized to the chip's occupancy; each wave
+// refills idle lanes with new rays from a global counter (one atomic per
+// wave per refill, prefix via ballot/mbcnt), so incoherent rays of very
+// different traversal length keep SIMD lanes busy. Per lane: a 16-entry
+// traversal stack in LDS (lane-strided, conflict-free) that spills its
+// oldest entries to a global area only when deeper than 16. Nodes are BVH4,
+// 112 bytes, loaded as seven 16-byte vectors; children are sorted by entry
+// distance with a 5-exchange network and the nearest is visited first.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+
+#include "geom.hpp"
+#include "internal.hpp"
+
+namespace ilm {
+
+constexpr int kBlock = 256;
+constexpr int kLds = 16;
+constexpr int kRefillMin = 16;
+
+struct TraceArgs {
+    const float4* __restrict__ nodes;  // 7 float4 per BVH4 node
+    const float4* __restrict__ tris;   // 3 float4 per triangle
+    const ilm_ray* __restrict__ rays;
+    int64_t n;
+    ilm_hit* __restrict__ hits;
+    uint8_t* __restrict__ occ;
+    int32_t* __restrict__ stats;
+    uint32_t* __restrict__ spill;
+    int32_t spill_depth;
+    int64_t grid_threads;
+    unsigned long long* counters;
+    uint32_t total_waves;
+};
+
+__device__ __forceinline__ float slab1(float lox, float hix, float loy, float hiy, float loz, float hiz,
+                                       const RayState& r) {
+    float tx0 = fmaf(lox, r.inv.x, -r.oinv.x), tx1 = fmaf(hix, r.inv.x, -r.oinv.x);
+    float ty0 = fmaf(loy, r.inv.y, -r.oinv.y), ty1 = fmaf(hiy, r.inv.y, -r.oinv.y);
+    float tz0 = fmaf(loz, r.inv.z, -r.oinv.z), tz1 = fmaf(hiz, r.inv.z, -r.oinv.z);
+    float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), r.tlo));
+    float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), r.thi));
+    return (tn <= tf * 1.00000048f) ? tn : INFINITY;
+}
+
+#define ILM_CSWAP(a, b)                                   \
+    if (d##b < d##a) {                                    \
+        float td = d##a; d##a = d##b; d##b = td;          \
+        uint32_t tc = c##a; c##a = c##b; c##b = tc;       \
+    }
+
+template <bool ANY, bool STATS>
+__global__ void __launch_bounds__(kBlock) k_trace4(TraceArgs a) {
+    __shared__ uint32_t lds[kLds * kBlock];
+    const int tid = threadIdx.x;
+    const int64_t gtid = (int64_t)blockIdx.x * kBlock + tid;
+
+    bool active = false;
+    bool exhausted = false;  // uniform across the wave
+    int64_t ray_id = 0;
+    RayState r;
+    uint32_t cur = kNoNode;
+    int sp = 0, lcnt = 0;
+    float best_t = 0.f, best_u = 0.f, best_v = 0.f;
+    int32_t best_prim = -1;
+    int32_t n_nodes = 0, n_tris = 0;
+
+    auto push = [&](uint32_t x) {
+        if (lcnt == kLds) {
+            int pos = sp - kLds;
+            if (pos < a.spill_depth) {
+                a.spill[(int64_t)pos * a.grid_threads + gtid] = lds[(sp % kLds) * kBlock + tid];
+            } else {
+                atomicOr(&a.counters[2], 1ull);  // stack bound exceeded (never expected)
+                return;
+            }
+        } else {
+            ++lcnt;
+        }
+        lds[(sp % kLds) * kBlock + tid] = x;
+        ++sp;
+    };
+    auto pop = [&]() -> uint32_t {
+        if (sp == 0) return kNoNode;
+        --sp;
+        if (lcnt > 0) {
+            --lcnt;
+            return lds[(sp % kLds) * kBlock + tid];
+        }
+        return a.spill[(int64_t)sp * a.grid_threads + gtid];
+    };
+    auto finish = [&]() {
+        if (ANY) {
+            a.occ[ray_id] = best_prim >= 0 ? 1 : 0;
+        } else {
+            float4 h = make_float4(best_t, __int_as_float(best_prim), best_u, best_v);
+            reinterpret_cast<float4*>(a.hits)[ray_id] = h;
+        }
+        if (STATS) {
+            a.stats[2 * ray_id] = n_nodes;
+            a.stats[2 * ray_id + 1] = n_tris;
+        }
+        active = false;
+    };
+
+    for (;;) {
+        unsigned long long idle = __ballot(!active);
+        if (!exhausted && idle != 0ull && (__popcll(idle) >= kRefillMin || idle == ~0ull)) {
+            unsigned long long base = 0;
+            if (__lane_id() == 0) base = atomicAdd(&a.counters[0], (unsigned long long)__popcll(idle));
+            base = __shfl(base, 0);
+            if (base + (unsigned long long)__popcll(idle) >= (unsigned long long)a.n) exhausted = true;
+            if (!active) {
+                int64_t my = (int64_t)base + __popcll(idle & __lanemask_lt());
+                if (my < a.n) {
+                    ray_id = my;
+                    const float4* rp = reinterpret_cast<const float4*>(a.rays + my);
+                    float4 o4 = rp[0], d4 = rp[1];
+                    ray_setup(r, mk(o4.x, o4.y, o4.z), mk(d4.x, d4.y, d4.z), d4.w);
+                    cur = 0;
+                    sp = 0;
+                    lcnt = 0;
+                    best_prim = -1;
+                    best_t = best_u = best_v = 0.f;
+                    n_nodes = n_tris = 0;
+                    active = true;
+                }
+            }
+        }
+        if (!__any(active)) break;
+
+        // inner nodes
+        while (active && !is_leaf(cur)) {
+            const float4* np = a.nodes + (int64_t)cur * 7;
+            float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
+            uint4 ch = reinterpret_cast<const uint4*>(np)[6];
+            if (STATS) ++n_nodes;
+            float d0 = slab1(lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, r);
+            float d1 = slab1(lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, r);
+            float d2 = slab1(lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, r);
+            float d3 = slab1(lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, r);
+            uint32_t c0 = ch.x, c1 = ch.y, c2 = ch.z, c3 = ch.w;
+            ILM_CSWAP(0, 1) ILM_CSWAP(2, 3) ILM_CSWAP(0, 2) ILM_CSWAP(1, 3) ILM_CSWAP(1, 2)
+            if (d3 < INFINITY) push(c3);
+            if (d2 < INFINITY) push(c2);
+            if (d1 < INFINITY) push(c1);
+            if (d0 < INFINITY) {
+                cur = c0;
+            } else {
+                cur = pop();
+                if (cur == kNoNode) finish();
+            }
+        }
+        // leaves
+        while (active && is_leaf(cur)) {
+            uint32_t first = leaf_first(cur), cnt = leaf_count(cur);
+            bool done = false;
+            for (uint32_t k = 0; k < cnt; ++k) {
+                const float4* tp = a.tris + 3 * (int64_t)(first + k);
+                float4 A = tp[0], B = tp[1], C = tp[2];
+                if (STATS) ++n_tris;
+                float t, u, v;
+                if (tri_watertight(r, mk(A.x, A.y, A.z), mk(B.x, B.y, B.z), mk(C.x, C.y, C.z), t, u, v)) {
+                    best_t = t;
+                    best_u = u;
+                    best_v = v;
+                    best_prim = __float_as_int(A.w);
+                    r.thi = tmax_window(t);
+                    if (ANY) { done = true; break; }
+                }
+            }
+            if (done) { finish(); break; }
+            cur = pop();
+            if (cur == kNoNode) finish();
+        }
+    }
+
+    // the last wave of the grid resets the fetch counters for the next launch
+    if (__lane_id() == 0) {
+        unsigned long long prev = atomicAdd(&a.counters[1], 1ull);
+        if (prev == (unsigned long long)a.total_waves - 1ull) {
+            atomicExch(&a.counters[0], 0ull);
+            atomicExch(&a.counters[1], 0ull);
+        }
+    }
+}
+
+template <bool ANY, bool STATS>
+static int launch_trace(ilm_mesh* m, const ilm_ray* rays, int64_t n, ilm_hit* hits, uint8_t* occ, int32_t* stats,
+                        void* stream) {
+    if (!m) return fail(ILM_ERR_INVALID, "null mesh");
+    if (!m->built) return fail(ILM_ERR_STATE, "BVH not built");
+    if (!m->ctx) return fail(ILM_ERR_STATE, "mesh was created without a device context");
+    if (n < 0 || (n > 0 && !rays)) return fail(ILM_ERR_INVALID, "bad ray buffer");
+    if (n == 0) return ILM_OK;
+    if (!ANY && !hits) return fail(ILM_ERR_INVALID, "null hits");
+    if (ANY && !occ) return fail(ILM_ERR_INVALID, "null occlusion output");
+    if (STATS && !stats) return fail(ILM_ERR_INVALID, "null stats");
+    ilm_ctx* ctx = m->ctx;
+    ILM_HIP(hipSetDevice(ctx->device));
+    int per_cu = 0;
+    ILM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace4<ANY, STATS>, kBlock, 0));
+    per_cu = std::max(per_cu, 1);
+    int64_t blocks = (int64_t)per_cu * ctx->cu_count;
+    blocks = std::min<int64_t>(blocks, (n + kBlock - 1) / kBlock);
+    blocks = std::max<int64_t>(blocks, 1);
+    TraceArgs a;
+    a.nodes = reinterpret_cast<const float4*>(m->d_nodes);
+    a.tris = reinterpret_cast<const float4*>(m->d_tris);
+    a.rays = rays;
+    a.n = n;
+    a.hits = hits;
+    a.occ = occ;
+    a.stats = stats;
+    a.grid_threads = blocks * kBlock;
+    a.spill_depth = std::max(1, m->info.stack_need + 1);
+    int rc = ensure_spill(ctx, (size_t)a.spill_depth * (size_t)a.grid_threads);
+    if (rc) return rc;
+    a.spill = ctx->spill;
+    a.counters = ctx->counters;
+    a.total_waves = (uint32_t)(blocks * (kBlock / 64));
+    hipLaunchKernelGGL((k_trace4<ANY, STATS>), dim3((unsigned)blocks), dim3(kBlock), 0, pick_stream(ctx, stream), a);
+    ILM_HIP(hipGetLastError());
+    return ILM_OK;
+}
+
+// ---------------------------------------------------------------- ray generation
+
+struct CamConsts {
+    f3 centre, corner, across, up;
+};
+
+// PerspectiveCamera frame (Camera/PerspectiveCamera.cpp:15-25, Camera.cpp:30-44,
+// Geometry/Basis.cpp:28-44) with the factory's distance 1 (CameraFactories.h:46-49).
+static CamConsts camera_consts(const ilm_camera& c) {
+    f3 pos = mk(c.pos[0], c.pos[1], c.pos[2]), tgt = mk(c.target[0], c.target[1], c.target[2]);
+    f3 upv = mk(c.up[0], c.up[1], c.up[2]);
+    f3 W = normalize(normalize(tgt - pos));
+    f3 V = normalize(upv);
+    V = V - dot(V, W) * W;
+    if (dot(V, V) <= FLT_EPSILON) {
+        V = mk(W.z, W.x, W.y);
+        V = normalize(cross(V, W));
+        V = normalize(V - dot(V, W) * W);
+    }
+    f3 U = cross(V, W);
+    float deg2rad = (float)atan(1.0) / 45.0f;
+    float half = c.fov_deg * deg2rad * 0.5f;
+    float v1 = (float)tan((double)half), v0 = -v1;
+    float u0 = v0 * c.aspect, u1 = v1 * c.aspect;
+    CamConsts k;
+    k.centre = pos;
+    k.corner = pos + u0 * U + v0 * V + 1.0f * W;
+    k.across = (u1 - u0) * U;
+    k.up = (v1 - v0) * V;
+    return k;
+}
+
+__device__ __forceinline__ void camera_ray(const CamConsts& c, int W, int H, int x, int y, float u, float v,
+                                           ilm_ray& out) {
+    float rcpW = 1.f / (float)W, rcpH = 1.f / (float)H;
+    float nx = (float)x * rcpW, ny = (float)y * rcpH;
+    float xi1 = (u - 0.5f) * rcpW, xi2 = (v - 0.5f) * rcpH;
+    f3 t = c.corner + c.across * (nx + xi1) + c.up * (ny + xi2);
+    f3 d = normalize(t - c.centre);
+    out.ox = c.centre.x; out.oy = c.centre.y; out.oz = c.centre.z; out.tmin = 0.0f;
+    out.dx = d.x; out.dy = d.y; out.dz = d.z; out.tmax = FLT_MAX;
+}
+
+__global__ void k_camera(CamConsts c, int W, int H, int jitter, uint32_t seed, uint32_t sample, ilm_ray* out) {
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (int64_t)W * H) return;
+    int x = (int)(p % W), y = (int)(p / W);
+    float u = 0.5f, v = 0.5f;
+    if (jitter) {
+        Block8 b = sample_block(seed, (uint64_t)p, sample, 0, 0);
+        u = b.v[0];
+        v = b.v[1];
+    }
+    ilm_ray r;
+    camera_ray(c, W, H, x, y, u, v, r);
+    out[p] = r;
+}
+
+// out[i] = camera ray through pixel pixels[i % npix]; the first npix rays use the
+// pixel's own sample position, later ones counter-based jitter (sample i / npix).
+__global__ void k_camera_list(CamConsts c, int W, int H, const int32_t* pixels, int64_t npix, int64_t n,
+                              uint32_t seed, ilm_ray* out) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int64_t p = pixels[i % npix];
+    uint32_t s = (uint32_t)(i / npix);
+    float u = 0.5f, v = 0.5f;
+    if (s > 0) {
+        Block8 b = sample_block(seed, (uint64_t)p, s, 0, 0);
+        u = b.v[0];
+        v = b.v[1];
+    }
+    ilm_ray r;
+    camera_ray(c, W, H, (int)(p % W), (int)(p / W), u, v, r);
+    out[i] = r;
+}
+
+struct SpawnArgs {
+    const ilm_ray* rays;
+    const ilm_hit* hits;
+    const int32_t* src;
+    int64_t n;
+    const float4* tris;
+    const int32_t* slot_of;  // original triangle index -> triangle slot
+    int kind;
+    f3 lc, lu, lv;
+    float eps;
+    uint32_t seed;
+    ilm_ray* out;
+};
+
+__global__ void k_spawn(SpawnArgs a) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    int64_t s = a.src ? (int64_t)a.src[i] : i;
+    ilm_hit h = a.hits[s];
+    ilm_ray r = a.rays[s];
+    ilm_ray o;
+    if (h.prim < 0) {  // no hit: a ray that cannot hit anything
+        o.ox = r.ox; o.oy = r.oy; o.oz = r.oz; o.tmin = 0.0f;
+        o.dx = r.dx; o.dy = r.dy; o.dz = r.dz; o.tmax = -1.0f;
+        a.out[i] = o;
+        return;
+    }
+    f3 ro = mk(r.ox, r.oy, r.oz), rd = mk(r.dx, r.dy, r.dz);
+    f3 P = ro + h.t * rd;  // Ray::PointAlongRay (Geometry/Ray.inl:32-34)
+    Block8 b = sample_block(a.seed, (uint64_t)i, 0, 1, (uint32_t)a.kind);
+    if (a.kind == ILM_SPAWN_SHADOW) {
+        f3 Q = a.lc + b.v[0] * a.lu + b.v[1] * a.lv;
+        f3 d = Q - P;
+        o.ox = P.x; o.oy = P.y; o.oz = P.z; o.tmin = 1e-4f;
+        o.dx = d.x; o.dy = d.y; o.dz = d.z; o.tmax = 1.0f - 1e-4f;
+    } else {
+        int32_t slot = a.slot_of[h.prim];
+        const float4* tp = a.tris + 3 * (int64_t)slot;
+        float4 A = tp[0], B = tp[1], C = tp[2];
+        f3 ng = normalize(cross(mk(B.x - A.x, B.y - A.y, B.z - A.z), mk(C.x - A.x, C.y - A.y, C.z - A.z)));
+        if (dot(ng, rd) > 0.0f) ng = -ng;  // face the incoming ray
+        f3 org = P + ng * a.eps;
+        Basis bs = basis_from_w(ng);
+        f3 d = to_world(bs, cosine_hemisphere(b.v[0], b.v[1]));
+        o.ox = org.x; o.oy = org.y; o.oz = org.z; o.tmin = a.eps;
+        o.dx = d.x; o.dy = d.y; o.dz = d.z; o.tmax = FLT_MAX;
+    }
+    a.out[i] = o;
+}
+
+// ---------------------------------------------------------------- compaction
+// Deterministic three-pass compaction of valid hits: per-block counts with
+// wave ballots, one scan block, then ordered writes.
+constexpr int kCompactItems = 4;  // items per thread per block pass
+
+__global__ void k_compact_count(const ilm_hit* hits, int64_t n, int32_t* block_counts) {
+    __shared__ int wave_sums[kBlock / 64];
+    int64_t base = (int64_t)blockIdx.x * kBlock * kCompactItems;
+    int cnt = 0;
+    for (int k = 0; k < kCompactItems; ++k) {
+        int64_t i = base + (int64_t)k * kBlock + threadIdx.x;
+        bool ok = i < n && hits[i].prim >= 0;
+        cnt += __popcll(__ballot(ok));
+    }
+    if (__lane_id() == 0) wave_sums[threadIdx.x / 64] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int s = 0;
+        for (int w = 0; w < kBlock / 64; ++w) s += wave_sums[w];
+        block_counts[blockIdx.x] = s;
+    }
+}
+
+__global__ void k_compact_scan(int32_t* block_counts, int64_t nblocks, int64_t* total) {
+    // single block: each thread scans a contiguous chunk, then the chunk sums
+    __shared__ int64_t part[kBlock];
+    int64_t per = (nblocks + kBlock - 1) / kBlock;
+    int64_t s0 = threadIdx.x * per, s1 = std::min<int64_t>(nblocks, s0 + per);
+    int64_t sum = 0;
+    for (int64_t i = s0; i < s1; ++i) sum += block_counts[i];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t run = 0;
+        for (int t = 0; t < kBlock; ++t) { int64_t v = part[t]; part[t] = run; run += v; }
+        *total = run;
+    }
+    __syncthreads();
+    int64_t run = part[threadIdx.x];
+    for (int64_t i = s0; i < s1; ++i) { int32_t v = block_counts[i]; block_counts[i] = (int32_t)run; run += v; }
+}
+
+__global__ void k_compact_write(const ilm_hit* hits, int64_t n, const int32_t* block_offsets, int32_t* out) {
+    __shared__ int cnt[kCompactItems][kBlock / 64];
+    const int wave = threadIdx.x / 64;
+    int64_t base = (int64_t)blockIdx.x * kBlock * kCompactItems;
+    unsigned long long masks[kCompactItems];
+    bool oks[kCompactItems];
+    for (int k = 0; k < kCompactItems; ++k) {
+        int64_t i = base + (int64_t)k * kBlock + threadIdx.x;
+        oks[k] = i < n && hits[i].prim >= 0;
+        masks[k] = __ballot(oks[k]);
+        if (__lane_id() == 0) cnt[k][wave] = __popcll(masks[k]);
+    }
+    __syncthreads();
+    // slots follow the index order: item k, then wave, then lane
+    int off = block_offsets[blockIdx.x];
+    for (int k = 0; k < kCompactItems; ++k) {
+        int b = off;
+        for (int w = 0; w < wave; ++w) b += cnt[k][w];
+        if (oks[k]) out[b + __popcll(masks[k] & __lanemask_lt())] = (int32_t)(base + (int64_t)k * kBlock + threadIdx.x);
+        for (int w = 0; w < kBlock / 64; ++w) off += cnt[k][w];
+    }
+}
+
+}  // namespace ilm
+
+using namespace ilm;
+
+extern "C" {
+
+int ilm_trace_closest(ilm_mesh* mesh, const ilm_ray* rays, int64_t n, ilm_hit* hits, void* stream) {
+    return launch_trace<false, false>(mesh, rays, n, hits, nullptr, nullptr, stream);
+}
+
+int ilm_trace_any(ilm_mesh* mesh, const ilm_ray* rays, int64_t n, uint8_t* occluded, void* stream) {
+    return launch_trace<true, false>(mesh, rays, n, nullptr, occluded, nullptr, stream);
+}
+
+int ilm_trace_stats(ilm_mesh* mesh, const ilm_ray* rays, int64_t n, int any_hit, int32_t* visits, void* stream) {
+    if (!mesh) return fail(ILM_ERR_INVALID, "null mesh");
+    if (any_hit) {
+        uint8_t* occ = nullptr;
+        int rc = ilm_alloc(mesh->ctx, (size_t)std::max<int64_t>(n, 1), (void**)&occ);
+        if (rc) return rc;
+        rc = launch_trace<true, true>(mesh, rays, n, nullptr, occ, visits, stream);
+        int rc2 = ilm_free(mesh->ctx, occ);
+        return rc ? rc : rc2;
+    }
+    ilm_hit* h = nullptr;
+    int rc = ilm_alloc(mesh->ctx, (size_t)std::max<int64_t>(n, 1) * sizeof(ilm_hit), (void**)&h);
+    if (rc) return rc;
+    rc = launch_trace<false, true>(mesh, rays, n, h, nullptr, visits, stream);
+    int rc2 = ilm_free(mesh->ctx, h);
+    return rc ? rc : rc2;
+}
+
+int ilm_raygen_camera(ilm_ctx* ctx, const ilm_camera* cam, int width, int height, int jitter, uint32_t seed,
+                      uint32_t sample_index, ilm_ray* rays, void* stream) {
+    if (!ctx || !cam || !rays || width <= 0 || height <= 0) return fail(ILM_ERR_INVALID, "bad camera arguments");
+    CamConsts c = camera_consts(*cam);
+    int64_t n = (int64_t)width * height;
+    hipLaunchKernelGGL(k_camera, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, pick_stream(ctx, stream), c, width,
+                       height, jitter, seed, sample_index, rays);
+    ILM_HIP(hipGetLastError());
+    return ILM_OK;
+}
+
+int ilm_raygen_camera_list(ilm_ctx* ctx, const ilm_camera* cam, int width, int height, const int32_t* pixels,
+                           int64_t npix, int64_t n, uint32_t seed, ilm_ray* rays, void* stream) {
+    if (!ctx || !cam || !rays || !pixels || npix <= 0 || n < 0) return fail(ILM_ERR_INVALID, "bad arguments");
+    if (n == 0) return ILM_OK;
+    CamConsts c = camera_consts(*cam);
+    hipLaunchKernelGGL(k_camera_list, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, pick_stream(ctx, stream), c,
+                       width, height, pixels, npix, n, seed, rays);
+    ILM_HIP(hipGetLastError());
+    return ILM_OK;
+}
+
+int ilm_raygen_secondary(ilm_mesh* mesh, const ilm_ray* rays_in, const ilm_hit* hits, const int32_t* src, int64_t n,
+                         int kind, const float* light, float eps, uint32_t seed, ilm_ray* rays_out, void* stream) {
+    if (!mesh || !rays_in || !hits || !rays_out || n < 0) return fail(ILM_ERR_INVALID, "bad arguments");
+    if (!mesh->built) return fail(ILM_ERR_STATE, "BVH not built");
+    if (!mesh->ctx) return fail(ILM_ERR_STATE, "mesh was created without a device context");
+    if (kind != ILM_SPAWN_SHADOW && kind != ILM_SPAWN_COSINE) return fail(ILM_ERR_INVALID, "unknown spawn kind");
+    if (kind == ILM_SPAWN_SHADOW && !light) return fail(ILM_ERR_INVALID, "shadow rays need a light");
+    if (n == 0) return ILM_OK;
+    ilm_ctx* ctx = mesh->ctx;
+    if (!mesh->d_slot_of) {
+        std::vector<int32_t> slot(mesh->nt);
+        for (int64_t k = 0; k < mesh->nt; ++k) slot[mesh->h_order[k]] = (int32_t)k;
+        int rc = ilm_alloc(ctx, slot.size() * sizeof(int32_t), (void**)&mesh->d_slot_of);
+        if (rc) return rc;
+        rc = ilm_upload(ctx, mesh->d_slot_of, slot.data(), slot.size() * sizeof(int32_t));
+        if (rc) return rc;
+    }
+    SpawnArgs a;
+    a.rays = rays_in;
+    a.hits = hits;
+    a.src = src;
+    a.n = n;
+    a.tris = reinterpret_cast<const float4*>(mesh->d_tris);
+    a.slot_of = mesh->d_slot_of;
+    a.kind = kind;
+    a.lc = light ? mk(light[0], light[1], light[2]) : mk(0, 0, 0);
+    a.lu = light ? mk(light[3], light[4], light[5]) : mk(0, 0, 0);
+    a.lv = light ? mk(light[6], light[7], light[8]) : mk(0, 0, 0);
+    a.eps = eps;
+    a.seed = seed;
+    a.out = rays_out;
+    hipLaunchKernelGGL(k_spawn, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, pick_stream(ctx, stream), a);
+    ILM_HIP(hipGetLastError());
+    return ILM_OK;
+}
+
+int ilm_compact_hits(ilm_ctx* ctx, const ilm_hit* hits, int64_t n, int32_t* idx_out, int64_t* count_dev,
+                     void* stream) {
+    if (!ctx || !hits || !idx_out || !count_dev || n < 0) return fail(ILM_ERR_INVALID, "bad arguments");
+    if (n >= (int64_t(1) << 31)) return fail(ILM_ERR_INVALID, "too many hits for int32 indices");
+    int64_t per_block = (int64_t)kBlock * kCompactItems;
+    int64_t nb = std::max<int64_t>(1, (n + per_block - 1) / per_block);
+    int32_t* bc = nullptr;
+    int rc = ilm_alloc(ctx, (size_t)nb * sizeof(int32_t), (void**)&bc);
+    if (rc) return rc;
+    hipStream_t s = pick_stream(ctx, stream);
+    hipLaunchKernelGGL(k_compact_count, dim3((unsigned)nb), dim3(kBlock), 0, s, hits, n, bc);
+    hipLaunchKernelGGL(k_compact_scan, dim3(1), dim3(kBlock), 0, s, bc, nb, count_dev);
+    hipLaunchKernelGGL(k_compact_write, dim3((unsigned)nb), dim3(kBlock), 0, s, hits, n, bc, idx_out);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    int rc2 = ilm_free(ctx, bc);
+    if (e != hipSuccess) return fail_hip(e, "compaction");
+    return rc2;
+}
+
+}  // extern "C"

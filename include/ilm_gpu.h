@@ -1,0 +1,272 @@
+/*
+ * ilm_gpu.h — C ABI of the MI355X ray-traversal and path-integration library.
+ *
+ * The library replaces the CPU hot path of the IlluminaPRT renderer: the mesh
+ * accelerator behind IShape::Intersects, the linear space behind
+ * ISpace::Intersects, and the path integrator behind
+ * IIntegrator::Radiance / IRenderer::RenderRegion. Each entry point below
+ * names the reference interface it replaces (paths relative to the reference
+ * tree root). All signatures use plain pointers and sizes.
+ *
+ * Conventions
+ *  - Every call returns ILM_OK (0) or a negative ILM_ERR_* code; nothing
+ *    throws across the ABI (the reference throws heap-allocated exceptions:
+ *    System/FactoryManager.inl:69,128-129). ilm_last_error() returns the
+ *    message of the calling thread's last failure.
+ *  - "dev" pointers are device (HBM) addresses, e.g. from ilm_alloc.
+ *    "host" pointers are ordinary host memory owned by the caller.
+ *  - A context owns its device allocations, its stream and its scratch
+ *    buffers. Use one context per host thread, or synchronise externally
+ *    (the reference calls Intersects concurrently from tile threads:
+ *    Utility/Sandbox/Multithreaded.h:207-263; batch those calls instead).
+ *  - stream == NULL means the context's own stream. Calls are asynchronous
+ *    on that stream unless documented otherwise.
+ */
+#ifndef ILM_GPU_H
+#define ILM_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ILM_OK 0
+#define ILM_ERR_INVALID (-1)  /* bad argument */
+#define ILM_ERR_HIP (-2)      /* HIP runtime failure (message has the HIP error) */
+#define ILM_ERR_NOMEM (-3)    /* allocation failed */
+#define ILM_ERR_STATE (-4)    /* object not ready (e.g. BVH not built) */
+#define ILM_ERR_NODEV (-5)    /* no HIP device */
+
+typedef struct ilm_ctx ilm_ctx;
+typedef struct ilm_mesh ilm_mesh;
+typedef struct ilm_scene ilm_scene;
+
+/* A ray: 32 bytes, two 16-byte halves. Replaces Geometry/Ray.h:17-54.
+ * tmin is kept for the caller but, as in the reference triangle test
+ * (Shape/IndexedTriangle.cpp:114-115), hits are accepted for
+ * 1e-5 <= t <= tmax - 1e-5 (absolute, t in units of |d|). */
+typedef struct {
+    float ox, oy, oz, tmin;
+    float dx, dy, dz, tmax;
+} ilm_ray;
+
+/* A closest-hit record: 16 bytes. prim = triangle index in the mesh as
+ * uploaded (the reference's hit index is the triangle's position in
+ * ITriangleMesh::TriangleList, IndexedTriangle.cpp:225 SetShape), -1 on a
+ * miss. u, v = barycentric weights of vertices 1 and 2 (the reference's
+ * beta, gamma). */
+typedef struct {
+    float t;
+    int32_t prim;
+    float u, v;
+} ilm_hit;
+
+/* ---------------------------------------------------------------- context */
+
+/* Creates a context on HIP device `device`. No reference counterpart (the
+ * reference runs in the host process; EngineKernel, System/EngineKernel.h:34-57,
+ * is the closest owner of global state). */
+int ilm_ctx_create(ilm_ctx** out, int device);
+int ilm_ctx_destroy(ilm_ctx* ctx);
+const char* ilm_last_error(void);
+/* Number of visible HIP devices (0 when none). */
+int ilm_device_count(int* count);
+/* Device name and compute-unit count. */
+int ilm_device_info(ilm_ctx* ctx, char* name, int name_len, int* cu_count);
+
+/* Device buffers owned by the context (freed by ilm_ctx_destroy). */
+int ilm_alloc(ilm_ctx* ctx, size_t bytes, void** dev);
+int ilm_free(ilm_ctx* ctx, void* dev);
+int ilm_upload(ilm_ctx* ctx, void* dev, const void* host, size_t bytes);     /* synchronous */
+int ilm_download(ilm_ctx* ctx, void* host, const void* dev, size_t bytes);   /* synchronous */
+int ilm_memset(ilm_ctx* ctx, void* dev, int value, size_t bytes);            /* synchronous */
+/* Waits for the context stream; reports ILM_ERR_STATE if a traversal hit
+ * its stack bound since context creation (never expected: the bound is the
+ * tree's exact worst case). */
+int ilm_sync(ilm_ctx* ctx);
+/* Event timing on the context stream: ilm_timer_mark records an event and
+ * returns its slot; ilm_timer_elapsed gives milliseconds between two slots
+ * (synchronises). Used by benchmarks to time kernels on the stream they
+ * are launched on. */
+int ilm_timer_mark(ilm_ctx* ctx, int* slot);
+int ilm_timer_elapsed(ilm_ctx* ctx, int slot0, int slot1, float* ms);
+
+/* ---------------------------------------------------------------- meshes */
+
+/* Creates a triangle mesh from host arrays: pos = nv*3 floats, idx = nt*3
+ * vertex indices. Replaces ITriangleMesh::AddVertex/AddIndexedTriangle
+ * (Shape/TriangleMesh.h:41-94). The triangle order defines hit indices.
+ * ctx may be NULL: the mesh can then be built and exported on the host but
+ * not traced (tracing calls return ILM_ERR_STATE). */
+int ilm_mesh_create(ilm_ctx* ctx, const float* pos, int64_t nv,
+                    const int32_t* idx, int64_t nt, ilm_mesh** out);
+int ilm_mesh_destroy(ilm_mesh* mesh);
+
+#define ILM_BUILDER_SAH 0        /* binned SAH over centroids (default) */
+#define ILM_BUILDER_REFERENCE 1  /* the reference BVHMesh topology */
+
+typedef struct {
+    int32_t builder;     /* ILM_BUILDER_* */
+    int32_t width;       /* 4 (default when 0); 2 and 8 also accepted */
+    int32_t max_leaf;    /* SAH: max triangles per leaf (default 4);
+                            REFERENCE: MaximumLeafObjects (default 10) */
+    int32_t max_depth;   /* REFERENCE: MaximumTreeDepth (default 35) */
+    int32_t threads;     /* host build threads (0 = all) */
+} ilm_bvh_params;
+
+typedef struct {
+    int64_t nodes;        /* wide nodes */
+    int64_t node_bytes;   /* bytes of the node array in HBM */
+    int64_t tri_bytes;    /* bytes of the triangle array in HBM */
+    int32_t width;
+    int32_t depth;        /* wide-tree depth */
+    int32_t max_leaf;     /* largest leaf (triangles) */
+    int32_t stack_need;   /* worst-case traversal stack entries */
+    double build_ms;      /* host build + flatten time */
+    double sah_cost;      /* SAH cost of the wide tree (Ci = 1, Ct = 1) */
+} ilm_bvh_info;
+
+/* Builds the hierarchy on the host, flattens it to a wide BVH and uploads
+ * node and triangle arrays. Replaces BVHMesh::Compile
+ * (Shape/BVHMesh.cpp:78-107,203-269) and ITreeMesh's split helpers
+ * (Shape/TreeMesh.cpp:44-212). Synchronous. */
+int ilm_bvh_build(ilm_mesh* mesh, const ilm_bvh_params* params);
+int ilm_bvh_get_info(ilm_mesh* mesh, ilm_bvh_info* info);
+
+/* Host copy of the flattened tree for validation: nodes as in HBM
+ * (node_bytes), and the original triangle index of each slot of the
+ * reordered triangle array (ntri int32). Either pointer may be NULL. */
+int ilm_bvh_export(ilm_mesh* mesh, void* nodes, int32_t* tri_order);
+
+/* ---------------------------------------------------------------- tracing */
+
+/* Closest hit for n rays. Replaces IShape::Intersects(const Ray&,
+ * DifferentialSurface&) of BVHMesh (Shape/BVHMesh.cpp:114-119,169-198) and
+ * ISpace::Intersects(const Ray&, Intersection&) (Space/BasicSpace.cpp:58-73),
+ * batched. rays and hits are dev pointers. */
+int ilm_trace_closest(ilm_mesh* mesh, const ilm_ray* rays, int64_t n, ilm_hit* hits, void* stream);
+
+/* Any hit (occlusion) for n rays: occluded[i] = 1 if some triangle is hit
+ * in [1e-5, tmax - 1e-5]. Replaces IShape::Intersects(const Ray&)
+ * (Shape/BVHMesh.cpp:125-130,143-165) as used by VisibilityQuery::IsOccluded
+ * (Scene/Visibility.cpp:33-37). */
+int ilm_trace_any(ilm_mesh* mesh, const ilm_ray* rays, int64_t n, uint8_t* occluded, void* stream);
+
+/* Same traversal with per-ray counters: visits[2*i] = wide nodes visited,
+ * visits[2*i+1] = triangles tested. Used to state algorithmic bytes. */
+int ilm_trace_stats(ilm_mesh* mesh, const ilm_ray* rays, int64_t n, int any_hit,
+                    int32_t* visits, void* stream);
+
+/* ---------------------------------------------------------------- ray sets */
+
+/* Camera description: PerspectiveCamera with the factory's frame
+ * (Camera/CameraFactories.h:40-78, Camera/Camera.cpp:30-44). */
+typedef struct {
+    float pos[3], target[3], up[3];
+    float fov_deg, aspect;
+} ilm_camera;
+
+/* Camera rays for a width x height image, one per pixel, ordered row-major
+ * (y = camera row as in BaseRenderer::RenderRegion, BaseRenderer.cpp:92-118).
+ * jitter = 0: the pixel's own sample position (0.5, 0.5); jitter = 1:
+ * counter-based sample (seed, pixel, sample_index). Replaces
+ * PerspectiveCamera::GetRay (Camera/PerspectiveCamera.cpp:53-60). */
+int ilm_raygen_camera(ilm_ctx* ctx, const ilm_camera* cam, int width, int height,
+                      int jitter, uint32_t seed, uint32_t sample_index, ilm_ray* rays, void* stream);
+
+/* out[i] = camera ray through pixel pixels[i % npix] (pixel = y*width + x):
+ * the first npix rays use the pixel's own sample position (0.5, 0.5),
+ * later ones a counter-based jittered sample (seed, pixel, i / npix). Used
+ * to draw a primary ray set made only of rays that hit geometry. */
+int ilm_raygen_camera_list(ilm_ctx* ctx, const ilm_camera* cam, int width, int height,
+                           const int32_t* pixels, int64_t npix, int64_t n, uint32_t seed,
+                           ilm_ray* rays, void* stream);
+
+#define ILM_SPAWN_SHADOW 0   /* segment to a point on a parallelogram light */
+#define ILM_SPAWN_COSINE 1   /* cosine-distributed bounce about the geometric normal */
+
+/* Secondary rays from closest hits: rays_out[i] starts at the hit of
+ * rays_in[src[i]] (src may be NULL: identity). Shadow rays follow
+ * DiffuseAreaLight::SampleRadiance + VisibilityQuery::SetSegment
+ * (Light/DiffuseAreaLight.cpp:52-69, Scene/Visibility.cpp:50-53): origin =
+ * hit point, direction = light point - origin, tmin 1e-4, tmax 1 - 1e-4;
+ * light = {corner[3], edge_u[3], edge_v[3]}. Cosine rays follow
+ * PathIntegrator::Radiance's bounce (Integrator/PathIntegrator.cpp:138-187):
+ * origin offset by eps along the geometric normal (faced toward the
+ * incoming ray), tmin eps. */
+int ilm_raygen_secondary(ilm_mesh* mesh, const ilm_ray* rays_in, const ilm_hit* hits,
+                         const int32_t* src, int64_t n, int kind, const float* light,
+                         float eps, uint32_t seed, ilm_ray* rays_out, void* stream);
+
+/* Stream compaction of rays whose hit is valid: writes indices of hits with
+ * prim >= 0 to idx_out and their count to *count_dev (dev int64). Built on
+ * wavefront ballot + prefix counts. */
+int ilm_compact_hits(ilm_ctx* ctx, const ilm_hit* hits, int64_t n, int32_t* idx_out,
+                     int64_t* count_dev, void* stream);
+
+/* ---------------------------------------------------------------- scene generators */
+
+/* Deterministic procedural scenes (host arrays; sizes from the *_size
+ * queries). No reference counterpart: the reference reads .ilm scripts
+ * (Scene/EnvironmentLoader.cpp:136-931); these generators build the
+ * benchmark scenes procedurally. */
+int ilm_scenegen_icosphere_size(int level, int64_t* nv, int64_t* nt);
+/* Icosphere of the given subdivision level, displaced radially by
+ * octaves of seeded value noise: r = 1 + amplitude * noise(p). */
+int ilm_scenegen_icosphere(int level, float amplitude, int octaves, uint32_t seed,
+                           float* pos, int32_t* idx);
+/* n boxes (24 vertices, 12 triangles each, outward winding), edge lengths
+ * uniform in [emin, emax], uniform random rotation, centres uniform in a
+ * ball of radius `radius`. Appends: vertex/triangle offsets are given. */
+int ilm_scenegen_boxes(int n, float emin, float emax, float radius, uint32_t seed,
+                       float* pos, int32_t* idx, int64_t vertex_offset);
+/* Area-weighted vertex normals (ITriangleMesh::UpdateNormals,
+ * Shape/TriangleMesh.cpp:96-133). */
+int ilm_scenegen_vertex_normals(const float* pos, int64_t nv, const int32_t* idx, int64_t nt,
+                                float* nrm);
+
+/* ---------------------------------------------------------------- path integrator */
+
+/* Material: 8 floats — kd rgb, ks rgb, phong exponent, type (0 = Lambertian
+ * MatteMaterial, Material/Matte.h:16-65; 1 = Lambertian + normalised Phong
+ * lobe, which has no reference counterpart). Light: 4 floats — Le rgb, 0;
+ * its emitter is every triangle whose tri_light equals its index
+ * (DiffuseAreaLight over a triangle mesh, Light/DiffuseAreaLight.cpp:17-149). */
+int ilm_scene_create(ilm_ctx* ctx, ilm_mesh* mesh, const float* nrm,
+                     const int32_t* tri_mat, const int32_t* tri_light,
+                     const float* mats, int nmat, const float* lights, int nlight,
+                     ilm_scene** out);
+int ilm_scene_destroy(ilm_scene* scene);
+
+typedef struct {
+    ilm_camera cam;
+    int32_t width, height;
+    int32_t spp;          /* samples per pixel in this call */
+    int32_t spp_offset;   /* first sample index (progressive rendering) */
+    int32_t max_depth;    /* PathTracing RayDepth (IntegratorFactories.h:43-52) */
+    int32_t rr_depth;     /* roulette when depth >= rr_depth (reference: 4) */
+    float eps;            /* PathTracing Epsilon (reference: 1e-4) */
+    uint32_t seed;
+    int32_t batch_paths;  /* paths per wavefront batch (0 = auto) */
+} ilm_render_params;
+
+typedef struct {
+    int64_t camera_rays, extension_rays, shadow_rays;
+    double ms;            /* GPU time of the call (events on the stream) */
+} ilm_render_stats;
+
+/* Renders spp samples per pixel with the wavefront path integrator and
+ * ADDS the per-pixel radiance sums into accum (dev, width*height*3 doubles).
+ * Replaces PathIntegrator::Radiance (Integrator/PathIntegrator.cpp:44-367)
+ * driven by BaseRenderer::RenderRegion (Renderer/BaseRenderer.cpp:62-137)
+ * over the RenderThread tile pool (Sandbox/Multithreaded.h:28-264).
+ * Synchronous; stats may be NULL. */
+int ilm_render(ilm_scene* scene, const ilm_render_params* params, double* accum,
+               ilm_render_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

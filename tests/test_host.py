@@ -1,0 +1,146 @@
+"""CPU-only checks of the library: the C ABI loads and exports every declared
+symbol, the host BVH builders satisfy their invariants (and the reference
+topology builder reproduces the oracle's BVHMesh leaves), the scene
+generators are well formed, and GPU entry points fail loudly without a GPU."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from tests import meshgen
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    txt = open(os.path.join(ROOT, "include", "ilm_gpu.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(ilm_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol(ilm):
+    import ctypes
+    lib = ctypes.CDLL(ilm.LIB_PATH)
+    syms = declared_symbols()
+    assert len(syms) > 20
+    missing = [s for s in syms if not hasattr(lib, s)]
+    assert not missing, missing
+    # the Python bindings cover the whole ABI
+    assert set(syms) == set(ilm.SIGNATURES), set(syms) ^ set(ilm.SIGNATURES)
+
+
+def test_no_gpu_fails_loudly(ilm):
+    if ilm.gpu_available():
+        pytest.skip("a GPU is visible")
+    with pytest.raises(ilm.IlmError, match="no HIP device"):
+        ilm.Context(0)
+
+
+def test_icosphere_generator(ilm):
+    pos, idx = ilm.icosphere(4, 0.1, 4, 7)
+    assert pos.shape == (2562, 3) and idx.shape == (5120, 3)
+    r = np.linalg.norm(pos, axis=1)
+    assert r.min() > 0.9 - 1e-6 and r.max() < 1.1 + 1e-6 and r.std() > 0.005
+    tri = pos[idx]
+    n = np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0])
+    assert (np.einsum("ij,ij->i", n, tri.mean(axis=1)) > 0).all()  # outward winding
+    # closed 2-manifold: every undirected edge used exactly twice
+    e = np.sort(np.concatenate([idx[:, [0, 1]], idx[:, [1, 2]], idx[:, [2, 0]]]), axis=1)
+    _, cnt = np.unique(e, axis=0, return_counts=True)
+    assert (cnt == 2).all()
+    pos2, _ = ilm.icosphere(4, 0.1, 4, 7)
+    assert np.array_equal(pos, pos2)  # deterministic
+
+
+def test_box_generator(ilm):
+    pos, idx = ilm.boxes(50, 0.01, 0.1, 3.0, 11)
+    assert pos.shape == (1200, 3) and idx.shape == (600, 3)
+    for b in range(50):
+        p = pos[24 * b: 24 * b + 24]
+        c = p.mean(axis=0)
+        assert np.linalg.norm(c) <= 3.0 + 0.1
+        tri = pos[idx[12 * b: 12 * b + 12]]
+        n = np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0])
+        assert (np.einsum("ij,ij->i", n, tri.mean(axis=1) - c) > 0).all()  # outward
+        ext = np.linalg.norm(n, axis=1)
+        assert (ext > 0).all()
+
+
+def _bvh_invariants(ilm, pos, idx, nodes, order):
+    boxes, child = ilm.decode_bvh4(nodes)
+    nt = len(idx)
+    assert sorted(order.tolist()) == list(range(nt))
+    tri = pos[idx[order]]                       # triangles in slot order
+    tlo, thi = tri.min(axis=1), tri.max(axis=1)
+    seen = np.zeros(nt, np.int64)
+    internal_refs = np.zeros(len(boxes), np.int64)
+    for ni in range(len(boxes)):
+        for k in range(4):
+            c = int(child[ni, k])
+            b = boxes[ni, k]
+            if c & 0x80000000:
+                first, cnt = (c & 0x7FFFFFFF) >> 5, c & 31
+                if cnt == 0:
+                    continue
+                s = slice(first, first + cnt)
+                seen[s] += 1
+                assert (b[:3] <= tlo[s]).all() and (b[3:] >= thi[s]).all()
+            else:
+                assert 0 < c < len(boxes)
+                internal_refs[c] += 1
+                cb = boxes[c]
+                valid = (child[c] & 0x80000000 == 0) | ((child[c] & 31) > 0)
+                assert (b[:3] <= cb[valid, :3]).all() and (b[3:] >= cb[valid, 3:]).all()
+    assert (seen == 1).all()
+    assert internal_refs[0] == 0 and (internal_refs[1:] == 1).all()   # a tree
+
+
+@pytest.mark.parametrize("builder", [0, 1])
+def test_wide_bvh_invariants(ilm, builder):
+    sp, si = meshgen.icosphere(4)
+    tp, ti = meshgen.random_triangles(3000, seed=3)
+    pos, idx = meshgen.merge([(sp, si), (tp, ti)])
+    m = ilm.Mesh(None, pos, idx)       # host-only mesh: build and export, no tracing
+    info = m.build(builder=builder)
+    nodes, order = m.export()
+    _bvh_invariants(ilm, pos, idx, nodes, order)
+    assert info["nodes"] == len(nodes) // 112
+    if builder == 0:
+        assert info["max_leaf"] <= 4   # the reference topology's depth/radius stops allow bigger leaves
+    m.close()
+
+
+def test_reference_builder_matches_oracle_bvhmesh(ilm, orc):
+    # The ILM_BUILDER_REFERENCE topology restates BVHMesh::Compile; its leaves
+    # (as triangle sets) must equal those of the oracle's BVHMesh restatement.
+    sp, si = meshgen.icosphere(3)
+    tp, ti = meshgen.random_triangles(1500, seed=9)
+    pos, idx = meshgen.merge([(sp, si), (tp, ti)])
+    om = orc.Mesh(pos, idx)
+    om.build(10, 35)
+    _, ochild, olf, olc, orefs = om.export()
+    oleaves = sorted(tuple(sorted(orefs[olf[i]:olf[i] + olc[i]].tolist()))
+                     for i in range(len(ochild)) if ochild[i, 0] < 0 and olc[i] > 0)
+    m = ilm.Mesh(None, pos, idx)
+    m.build(builder=1)
+    nodes, order = m.export()
+    _, child = ilm.decode_bvh4(nodes)
+    leaves = []
+    for c in child.reshape(-1):
+        c = int(c)
+        if c & 0x80000000 and (c & 31):
+            first = (c & 0x7FFFFFFF) >> 5
+            leaves.append(tuple(sorted(order[first:first + (c & 31)].tolist())))
+    assert sorted(leaves) == oleaves
+    m.close()
+
+
+def test_mesh_without_context_cannot_trace(ilm):
+    pos, idx = meshgen.icosphere(1)
+    m = ilm.Mesh(None, pos, idx)
+    m.build()
+    import ctypes
+    rc = ilm.load().ilm_trace_closest(m.h, ctypes.c_void_p(16), 1, ctypes.c_void_p(16), None)
+    assert rc == -4 and b"device context" in ilm.load().ilm_last_error()
+    m.close()
