@@ -88,6 +88,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--only", choices=["incoherent", "primary", "shadow"], default=None,
+                    help="time one ray set only (profiling runs)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run the plain copy kernel over MIB mebibytes (bandwidth / counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -99,6 +103,15 @@ def main():
 
     ilm.load()
     ctx = ilm.Context(0)
+    if args.copy_probe:
+        nbytes = args.copy_probe << 20
+        src, dst = ctx.buffer(nbytes), ctx.buffer(nbytes)
+        for _ in range(args.warmup):
+            ctx.copy(dst, src, nbytes)
+        _, per = time_launches(ctx, lambda: ctx.copy(dst, src, nbytes), args.steps)
+        ms = float(np.mean(per))
+        print(json.dumps({"copy_bytes": nbytes, "kernel_ms": ms, "gbs_read_plus_write": 2 * nbytes / ms / 1e6}))
+        return
     t0 = time.perf_counter()
     pos, idx = workloads.intersection_scene()
     mesh = ilm.Mesh(ctx, pos, idx)
@@ -115,6 +128,8 @@ def main():
         "primary": (lambda: mesh.trace_closest(rs.primary, n, hits), rs.primary, False),
         "shadow": (lambda: mesh.trace_any(rs.shadow, n, occ), rs.shadow, True),
     }
+    if args.only:
+        sets = {args.only: sets[args.only]}
     res = {}
     for name, (launch, buf, anyhit) in sets.items():
         mesh.trace_stats(buf, n, anyhit, visits)
@@ -138,6 +153,9 @@ def main():
         }
         log(f"{name}: {res[name]}")
     ctx.sync()
+    if "incoherent" not in res:
+        print(json.dumps(res), flush=True)
+        return
 
     inc = res["incoherent"]
     achieved = inc["bytes_per_ray"] * n / (inc["kernel_ms"] * 1e-3) / 1e9

@@ -85,6 +85,7 @@ SIGNATURES = {
     "ilm_download": (C.c_int, [_vp, _vp, _vp, _sz]),
     "ilm_memset": (C.c_int, [_vp, _vp, C.c_int, _sz]),
     "ilm_sync": (C.c_int, [_vp]),
+    "ilm_device_copy": (C.c_int, [_vp, _vp, _vp, _sz, _vp]),
     "ilm_timer_mark": (C.c_int, [_vp, C.POINTER(C.c_int)]),
     "ilm_timer_elapsed": (C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(C.c_float)]),
     "ilm_mesh_create": (C.c_int, [_vp, _vp, _i64, _vp, _i64, C.POINTER(_vp)]),
@@ -201,6 +202,9 @@ class Context:
 
     def sync(self):
         check(load().ilm_sync(self.h))
+
+    def copy(self, dst: DeviceBuffer, src: DeviceBuffer, nbytes: int):
+        check(load().ilm_device_copy(self.h, dst.ptr, src.ptr, nbytes, None))
 
     def mark(self) -> int:
         s = C.c_int()

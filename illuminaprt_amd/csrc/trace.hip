@@ -421,6 +421,14 @@ __global__ void k_compact_write(const ilm_hit* hits, int64_t n, const int32_t* b
     }
 }
 
+// Plain streaming copy (16 bytes per lane per iteration, grid-stride): the
+// bandwidth probe used to re-measure HBM throughput and to calibrate the
+// byte counters of the profiler.
+__global__ void k_copy(const float4* __restrict__ src, float4* __restrict__ dst, int64_t n16) {
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+}
+
 }  // namespace ilm
 
 using namespace ilm;
@@ -507,6 +515,17 @@ int ilm_raygen_secondary(ilm_mesh* mesh, const ilm_ray* rays_in, const ilm_hit* 
     a.seed = seed;
     a.out = rays_out;
     hipLaunchKernelGGL(k_spawn, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, pick_stream(ctx, stream), a);
+    ILM_HIP(hipGetLastError());
+    return ILM_OK;
+}
+
+int ilm_device_copy(ilm_ctx* ctx, void* dst, const void* src, size_t bytes, void* stream) {
+    if (!ctx || !dst || !src || (bytes % 16) != 0) return fail(ILM_ERR_INVALID, "copy needs 16-byte multiples");
+    if (bytes == 0) return ILM_OK;
+    int64_t n16 = (int64_t)(bytes / 16);
+    int64_t blocks = std::min<int64_t>((n16 + 255) / 256, (int64_t)ctx->cu_count * 32);
+    hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(256), 0, pick_stream(ctx, stream),
+                       reinterpret_cast<const float4*>(src), reinterpret_cast<float4*>(dst), n16);
     ILM_HIP(hipGetLastError());
     return ILM_OK;
 }

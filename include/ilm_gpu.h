@@ -86,6 +86,10 @@ int ilm_memset(ilm_ctx* ctx, void* dev, int value, size_t bytes);            /* 
  * its stack bound since context creation (never expected: the bound is the
  * tree's exact worst case). */
 int ilm_sync(ilm_ctx* ctx);
+/* Device-to-device streaming copy kernel (bytes a multiple of 16). No
+ * reference counterpart: it re-measures HBM bandwidth and calibrates the
+ * profiler's byte counters against a transfer of known size. */
+int ilm_device_copy(ilm_ctx* ctx, void* dst, const void* src, size_t bytes, void* stream);
 /* Event timing on the context stream: ilm_timer_mark records an event and
  * returns its slot; ilm_timer_elapsed gives milliseconds between two slots
  * (synchronises). Used by benchmarks to time kernels on the stream they
