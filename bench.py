@@ -81,6 +81,41 @@ def cpu_baseline(pos, idx, rays_host, budget_s=15.0):
     }
 
 
+def path_tracing_room(ctx, ilm, workloads, args):
+    """Box room, 256 x 256, 16 spp, depth 5: GPU frame time and Mrays/s over
+    camera + extension + shadow rays, beside the oracle's frame."""
+    sa = workloads.box_room()
+    pos, nrm, idx, tm, tl, mats, lights = sa
+    mesh = ilm.Mesh(ctx, pos, idx)
+    mesh.build()
+    sc = ilm.Scene(ctx, mesh, nrm, tm, tl, mats, lights)
+    cam = workloads.room_camera()
+    acc = ctx.buffer(256 * 256 * 3 * 8).zero()
+    for i in range(args.warmup):
+        sc.render(cam, 256, 256, 16, acc, seed=100 + i)
+    frames = []
+    for i in range(max(3, args.steps // 4)):
+        frames.append(sc.render(cam, 256, 256, 16, acc, seed=1000 + i))
+    ms = float(np.mean([f["ms"] for f in frames]))
+    rays = frames[0]["camera_rays"] + frames[0]["extension_rays"] + frames[0]["shadow_rays"]
+    out = {"workload": "box room 34+2 tris, 256x256, 16 spp, depth 5", "frame_ms": round(ms, 3),
+           "mrays_s": round(rays / ms / 1e3, 1), "rays_per_frame": int(rays)}
+    if not args.no_cpu:
+        import oracle
+        s = oracle.Scene(*sa)
+        t0 = time.perf_counter()
+        s.render(cam.as_list(), 256, 256, 16, seed=1234)
+        dt = time.perf_counter() - t0
+        out["cpu_frame_ms"] = round(dt * 1e3, 1)
+        out["cpu_mrays_s"] = round(float(s.last_rays.sum()) / dt / 1e6, 2)
+        out["cpu_cores"] = int(s.last_threads)
+    log(f"path tracing room: {out}")
+    acc.free()
+    sc.close()
+    mesh.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -164,6 +199,7 @@ def main():
         rays_host = rs.incoherent.download(np.float32, 8 * n).reshape(n, 8)
         cpu = cpu_baseline(pos, idx, rays_host, args.cpu_budget)
         log(f"cpu baseline: {cpu}")
+    room = None if args.only else path_tracing_room(ctx, ilm, workloads, args)
     out = {
         "metric": METRIC,
         "value": round(inc["wall_mrays_s"], 3),
@@ -199,6 +235,7 @@ def main():
             "bytes_per_ray": round(inc["bytes_per_ray"], 1),
         },
         "cpu_baseline": cpu,
+        "path_tracing_room": room,
     }
     print(json.dumps(out), flush=True)
     for b in (hits, occ, visits):

@@ -290,6 +290,35 @@ class Mesh:
             self.h = None
 
 
+class Scene:
+    """Shading data over a Mesh, rendered by the wavefront path integrator
+    (replaces the reference's PathIntegrator + BaseRenderer::RenderRegion)."""
+
+    def __init__(self, ctx: Context, mesh: Mesh, nrm, tri_mat, tri_light, mats, lights):
+        self.ctx, self.mesh = ctx, mesh
+        nrm = np.ascontiguousarray(nrm, np.float32).reshape(-1, 3)
+        tri_mat = np.ascontiguousarray(tri_mat, np.int32)
+        tri_light = np.ascontiguousarray(tri_light, np.int32)
+        mats = np.ascontiguousarray(mats, np.float32).reshape(-1, 8)
+        lights = np.ascontiguousarray(lights, np.float32).reshape(-1, 4)
+        h = C.c_void_p()
+        check(load().ilm_scene_create(ctx.h, mesh.h, _ptr(nrm), _ptr(tri_mat), _ptr(tri_light), _ptr(mats),
+                                      len(mats), _ptr(lights), len(lights), C.byref(h)))
+        self.h = h.value
+
+    def render(self, cam: Camera, width, height, spp, accum: DeviceBuffer, max_depth=5, rr_depth=4, eps=1e-4,
+               seed=1234, spp_offset=0, batch_paths=0) -> dict:
+        p = RenderParams(cam, width, height, spp, spp_offset, max_depth, rr_depth, eps, seed, batch_paths)
+        st = RenderStats()
+        check(load().ilm_render(self.h, C.byref(p), accum.ptr, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in st._fields_}
+
+    def close(self):
+        if self.h:
+            check(load().ilm_scene_destroy(self.h))
+            self.h = None
+
+
 # ---------------------------------------------------------------- host-side helpers
 def rays_to_device(ctx: Context, rays: np.ndarray) -> DeviceBuffer:
     rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)

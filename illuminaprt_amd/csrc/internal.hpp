@@ -61,4 +61,10 @@ namespace ilm {
 // Ensures ctx->spill holds at least `words` 32-bit words.
 int ensure_spill(ilm_ctx* ctx, size_t words);
 hipStream_t pick_stream(ilm_ctx* ctx, void* stream);
+// PerspectiveCamera frame: out = centre, corner, across, up (12 floats).
+void camera_frame(const ilm_camera& c, float out[12]);
+// Closest hit (occ == null) or any hit over a ray queue whose length lives in
+// device memory (*n_dev, at most `capacity`), without a host round trip.
+int trace_queue(ilm_mesh* m, const ilm_ray* rays, const unsigned long long* n_dev, int64_t capacity, ilm_hit* hits,
+                uint8_t* occ, hipStream_t stream);
 }  // namespace ilm

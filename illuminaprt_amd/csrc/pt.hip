@@ -1,19 +1,585 @@
-// Wavefront path integrator (filled in next; entry points report that the
-// integrator is unavailable until then).
+// Wavefront path integrator for CDNA4 (gfx950).
+//
+// Restates PathIntegrator::Radiance (Integrator/PathIntegrator.cpp:44-201)
+// driven by BaseRenderer::RenderRegion (Renderer/BaseRenderer.cpp:62-137) as
+// a wavefront: all paths of a batch advance one bounce at a time through
+//   generate -> extend (closest hit) -> shade -> shadow (any hit) -> accumulate
+// Queue lengths live in device memory, so a bounce is a fixed sequence of
+// launches with no host round trip. Queues are appended with one atomic per
+// wave (ballot + popcount + lane prefix). Random numbers are counter-based
+// (Philox4x64-10 keyed by seed; counter = pixel, sample, bounce, stream), so a
+// path's samples do not depend on scheduling and renders are reproducible.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "geom.hpp"
 #include "internal.hpp"
+
+namespace ilm {
+
+constexpr int kPtBlock = 256;
+
+struct Material {
+    f3 kd, ks;
+    float exponent;
+    int type;  // 0 Lambertian, 1 Lambertian + Phong lobe
+};
+
+__device__ __forceinline__ Material load_material(const float* m, int i) {
+    const float* q = m + 8 * i;
+    Material r;
+    r.kd = mk(q[0], q[1], q[2]);
+    r.ks = mk(q[3], q[4], q[5]);
+    r.exponent = q[6];
+    r.type = (int)q[7];
+    return r;
+}
+
+// Phong lobe (no reference counterpart): normalised modified Phong about the
+// mirror direction of wOut in the local frame.
+__device__ __forceinline__ f3 phong_f(const Material& m, f3 wo, f3 wi) {
+    f3 r = mk(-wo.x, -wo.y, wo.z);
+    float c = dot(r, wi);
+    if (c <= 0.f) return mk(0, 0, 0);
+    return m.ks * ((m.exponent + 2.f) * (0.5f * kInvPi) * powf(c, m.exponent));
+}
+__device__ __forceinline__ float phong_pdf(const Material& m, f3 wo, f3 wi) {
+    f3 r = mk(-wo.x, -wo.y, wo.z);
+    float c = dot(r, wi);
+    if (c <= 0.f) return 0.f;
+    return (m.exponent + 1.f) * (0.5f * kInvPi) * powf(c, m.exponent);
+}
+// BSDF::F (Material/BSDF.cpp:215-237): reflection only when wIn.z > 0.
+__device__ __forceinline__ f3 bsdf_f(const Material& m, f3 wo, f3 wi) {
+    if (!(wi.z > 0.f)) return mk(0, 0, 0);
+    f3 f = m.kd * kInvPi;  // Lambertian::F (Material/LambertianBxDF.cpp:41-44)
+    if (m.type == 1) f = f + phong_f(m, wo, wi);
+    return f;
+}
+// BSDF::SampleF (Material/BSDF.cpp:141-211) over {Lambertian, [Phong]}.
+__device__ __forceinline__ f3 bsdf_sample(const Material& m, f3 wo, f3& wi, float u, float v, float w, float& pdf) {
+    int count = (m.type == 1) ? 2 : 1;
+    int which = (int)(w * count) % count;
+    if (which == 0) {
+        // Lambertian::SampleF (Material/LambertianBxDF.cpp:27-37)
+        wi = cosine_hemisphere(u, v);
+        if (wo.z < 0.f) wi.z = -wi.z;
+        pdf = fabsf(wi.z) * kInvPi;
+    } else {
+        f3 r = mk(-wo.x, -wo.y, wo.z);
+        float ct = powf(u, 1.f / (m.exponent + 1.f));
+        float st = sqrtf(fmaxf(0.f, 1.f - ct * ct));
+        float ph = 2.0f * kPi * v;
+        wi = to_world(basis_from_w(r), mk(st * cosf(ph), st * sinf(ph), ct));
+        pdf = phong_pdf(m, wo, wi);
+    }
+    if (pdf == 0.0f) return mk(0, 0, 0);
+    if (count > 1) {
+        if (which == 0) pdf += phong_pdf(m, wo, wi);
+        else pdf += fabsf(wi.z) * kInvPi;
+        pdf /= count;
+    }
+    return bsdf_f(m, wo, wi);
+}
+
+__device__ __forceinline__ float luminance(f3 s) { return 0.212671f * s.x + 0.715160f * s.y + 0.072169f * s.z; }
+__device__ __forceinline__ bool is_black(f3 s) { return s.x == 0.f && s.y == 0.f && s.z == 0.f; }
+
+// One atomic per wave: lanes with `want` get consecutive slots.
+__device__ __forceinline__ int64_t wave_append(unsigned long long* counter, bool want) {
+    unsigned long long m = __ballot(want);
+    if (m == 0ull) return -1;
+    int leader = __ffsll((long long)m) - 1;
+    unsigned long long base = 0;
+    if ((int)__lane_id() == leader) base = atomicAdd(counter, (unsigned long long)__popcll(m));
+    base = __shfl(base, leader);
+    return want ? (int64_t)(base + __popcll(m & __lanemask_lt())) : -1;
+}
+
+struct CamK {
+    f3 centre, corner, across, up;
+};
+
+struct PtArgs {
+    // scene
+    const float4* shade;     // 6 float4 per triangle: (v0, mat), (v1, light), (v2, 0), n0, n1, n2
+    const float* mats;
+    const float4* ltris;     // emitter triangles, 3 float4 each
+    const float4* lights;    // (Le rgb, area)
+    const int2* lrange;      // (first emitter triangle, count)
+    int nlights;
+    // camera and pixels
+    CamK cam;
+    int W, H;
+    int64_t pix0;
+    int spp, spp_offset;
+    uint32_t seed;
+    int max_depth, rr_depth;
+    float eps;
+    // path state
+    int64_t npaths;
+    float4* beta;            // per path
+    float* L;                // 3 floats per path
+    // queues (ping-pong) and shadow queue
+    ilm_ray* rays[2];
+    int32_t* qpath[2];
+    ilm_hit* hits;
+    ilm_ray* srays;
+    float4* scontrib;        // rgb contribution, path index in w
+    uint8_t* socc;
+    unsigned long long* counts;  // [0], [1] ray queues, [2] shadow queue, [3] extension total, [4] shadow total
+    double* accum;
+};
+
+__device__ __forceinline__ void path_pixel(const PtArgs& a, int64_t p, int64_t& pixel, uint32_t& sample) {
+    pixel = a.pix0 + p / a.spp;
+    sample = (uint32_t)(a.spp_offset + (int)(p % a.spp));
+}
+
+__global__ void k_generate(PtArgs a) {
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p == 0) {
+        a.counts[0] = (unsigned long long)a.npaths;
+        a.counts[1] = 0ull;
+        a.counts[2] = 0ull;
+    }
+    if (p >= a.npaths) return;
+    int64_t pixel;
+    uint32_t sample;
+    path_pixel(a, p, pixel, sample);
+    Block8 b = sample_block(a.seed, (uint64_t)pixel, sample, 0, 0);
+    int x = (int)(pixel % a.W), y = (int)(pixel / a.W);
+    // PerspectiveCamera::GetRay (Camera/PerspectiveCamera.cpp:53-60) as fed by
+    // RenderRegion with the box filter's -0.5 shift (BaseRenderer.cpp:106-118)
+    float rcpW = 1.f / (float)a.W, rcpH = 1.f / (float)a.H;
+    float nx = (float)x * rcpW, ny = (float)y * rcpH;
+    float xi1 = (b.v[0] - 0.5f) * rcpW, xi2 = (b.v[1] - 0.5f) * rcpH;
+    f3 t = a.cam.corner + a.cam.across * (nx + xi1) + a.cam.up * (ny + xi2);
+    f3 d = normalize(t - a.cam.centre);
+    ilm_ray r;
+    r.ox = a.cam.centre.x; r.oy = a.cam.centre.y; r.oz = a.cam.centre.z; r.tmin = 0.f;
+    r.dx = d.x; r.dy = d.y; r.dz = d.z; r.tmax = FLT_MAX;
+    a.rays[0][p] = r;
+    a.qpath[0][p] = (int32_t)p;
+    a.beta[p] = make_float4(1.f, 1.f, 1.f, 0.f);
+    a.L[3 * p] = 0.f;
+    a.L[3 * p + 1] = 0.f;
+    a.L[3 * p + 2] = 0.f;
+}
+
+// One bounce of PathIntegrator::Radiance for every path in queue `cur`.
+__global__ void __launch_bounds__(kPtBlock) k_shade(PtArgs a, int bounce, int cur) {
+    const int nxt = cur ^ 1;
+    const int64_t count = (int64_t)a.counts[cur];
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const bool last = bounce + 1 >= a.max_depth;
+    // iterate in whole-wave steps so that every lane of a wave reaches the
+    // queue appends together
+    for (int64_t base = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) & ~63ll; base < count; base += stride) {
+        const int64_t k = base + (int64_t)__lane_id();
+        bool live = k < count;
+        int32_t p = 0;
+        ilm_hit h;
+        ilm_ray ray;
+        if (live) {
+            p = a.qpath[cur][k];
+            h = a.hits[k];
+            ray = a.rays[cur][k];
+            live = h.prim >= 0;  // a miss adds nothing for area lights (PathIntegrator.cpp:192-199)
+        }
+        int mat_id = -1, light_id = -1;
+        f3 P = mk(0, 0, 0), Ng = mk(0, 0, 1), Ns = mk(0, 0, 1), wOut = mk(0, 0, 1);
+        f3 beta = mk(0, 0, 0);
+        int64_t pixel = 0;
+        uint32_t sample = 0;
+        if (live) {
+            const float4* sr = a.shade + 6 * (int64_t)h.prim;
+            float4 A = sr[0], B = sr[1], C = sr[2], N0 = sr[3], N1 = sr[4], N2 = sr[5];
+            mat_id = __float_as_int(A.w);
+            light_id = __float_as_int(B.w);
+            f3 rd = mk(ray.dx, ray.dy, ray.dz);
+            P = mk(ray.ox, ray.oy, ray.oz) + h.t * rd;  // GeometricPrimitive.cpp:151
+            float alpha = 1.f - h.u - h.v;
+            Ns = normalize(mk(N0.x * alpha + N1.x * h.u + N2.x * h.v, N0.y * alpha + N1.y * h.u + N2.y * h.v,
+                              N0.z * alpha + N1.z * h.u + N2.z * h.v));
+            Ng = normalize(cross(mk(B.x - A.x, B.y - A.y, B.z - A.z), mk(C.x - A.x, C.y - A.y, C.z - A.z)));
+            wOut = -normalize(rd);
+            float4 b4 = a.beta[p];
+            beta = mk(b4.x, b4.y, b4.z);
+            path_pixel(a, p, pixel, sample);
+            if (mat_id < 0) live = false;  // no material: the path ends (PathIntegrator.cpp:110-111)
+        }
+        Material mat;
+        if (live) mat = load_material(a.mats, mat_id);
+        // emission at depth 0 (no specular BxDFs), DiffuseAreaLight::Radiance (DiffuseAreaLight.cpp:47-50)
+        if (live && bounce == 0 && light_id >= 0 && dot(Ns, wOut) > 0.f) {
+            float4 le = a.lights[light_id];
+            atomicAdd(&a.L[3 * p], beta.x * le.x);
+            atomicAdd(&a.L[3 * p + 1], beta.y * le.y);
+            atomicAdd(&a.L[3 * p + 2], beta.z * le.z);
+        }
+        // next-event estimation over every light except the one hit
+        // (SampleAllLights / EstimateDirectLighting, Integrator.cpp:114-185)
+        Basis gb = basis_from_w(Ng);
+        for (int l = 0; l < a.nlights; ++l) {
+            bool want = false;
+            ilm_ray sray;
+            f3 contrib = mk(0, 0, 0);
+            if (live && l != light_id) {
+                Block8 lb = sample_block(a.seed, (uint64_t)pixel, sample, (uint32_t)bounce, 1u + (uint32_t)l);
+                int2 rg = a.lrange[l];
+                float4 le = a.lights[l];
+                int kk = min((int)((float)rg.y * lb.v[2]), rg.y - 1);
+                const float4* lt = a.ltris + 3 * (int64_t)(rg.x + kk);
+                float4 p1 = lt[0], p2 = lt[1], p3 = lt[2];
+                float su = sqrtf(lb.v[0]);
+                float b1 = 1.f - su, b2 = lb.v[1] * su;  // Montecarlo::UniformSampleTriangle
+                float b3 = 1.f - b1 - b2;
+                f3 Q = mk(b1 * p1.x + b2 * p2.x + b3 * p3.x, b1 * p1.y + b2 * p2.y + b3 * p3.y,
+                          b1 * p1.z + b2 * p2.z + b3 * p3.z);
+                f3 nl = normalize(cross(mk(p3.x - p1.x, p3.y - p1.y, p3.z - p1.z),
+                                        mk(p2.x - p1.x, p2.y - p1.y, p2.z - p1.z)));
+                if (dot(Q - P, nl) > 0.f) nl = -nl;  // ITriangleMesh::SamplePoint (TriangleMesh.cpp:150-161)
+                f3 wIn = P - Q;
+                float d2 = dot(wIn, wIn);
+                wIn = normalize(wIn);
+                float c = fmaxf(0.f, dot(wIn, nl));
+                float pdf = 1.0f / le.w;
+                f3 Ls = mk(le.x, le.y, le.z) * c / (pdf * d2);
+                if (!is_black(Ls)) {
+                    wIn = -wIn;
+                    f3 f = bsdf_f(mat, to_local(gb, wOut), to_local(gb, wIn));
+                    contrib = mul(beta, mul(Ls * fmaxf(0.f, dot(wIn, Ns)), f));
+                    if (!is_black(contrib)) {
+                        want = true;
+                        f3 sd = Q - P;  // VisibilityQuery::SetSegment (Visibility.cpp:50-53)
+                        sray.ox = P.x; sray.oy = P.y; sray.oz = P.z; sray.tmin = 1e-4f;
+                        sray.dx = sd.x; sray.dy = sd.y; sray.dz = sd.z; sray.tmax = 1.0f - 1e-4f;
+                    }
+                }
+            }
+            int64_t slot = wave_append(&a.counts[2], want);
+            if (want) {
+                a.srays[slot] = sray;
+                a.scontrib[slot] = make_float4(contrib.x, contrib.y, contrib.z, __int_as_float(p));
+            }
+        }
+        // BSDF sampling in the shading frame (PathIntegrator.cpp:138-177)
+        bool cont = false;
+        ilm_ray nray;
+        if (live) {
+            Block8 b = sample_block(a.seed, (uint64_t)pixel, sample, (uint32_t)bounce, 0);
+            Basis sb = basis_from_w(Ns);
+            f3 wiL;
+            float pdf = 0.f;
+            f3 f = bsdf_sample(mat, to_local(sb, wOut), wiL, b.v[2], b.v[3], b.v[4], pdf);
+            if (!is_black(f) && pdf != 0.0f) {
+                f3 wIn = to_world(sb, wiL);
+                beta = mul(beta, (f * fabsf(dot(wIn, Ns))) / pdf);
+                cont = true;
+                if (bounce >= a.rr_depth) {
+                    float q = fminf(0.5f, luminance(beta));
+                    if (b.v[5] > q) cont = false;
+                    else beta = beta / q;
+                }
+                if (last) cont = false;  // the last vertex's continuation ray is never used
+                if (cont) {
+                    f3 org = P + Ng * a.eps;  // offset along the unflipped geometric normal
+                    nray.ox = org.x; nray.oy = org.y; nray.oz = org.z; nray.tmin = a.eps;
+                    nray.dx = wIn.x; nray.dy = wIn.y; nray.dz = wIn.z; nray.tmax = FLT_MAX;
+                }
+            }
+        }
+        int64_t slot = wave_append(&a.counts[nxt], cont);
+        if (cont) {
+            a.rays[nxt][slot] = nray;
+            a.qpath[nxt][slot] = p;
+            a.beta[p] = make_float4(beta.x, beta.y, beta.z, 0.f);
+        }
+    }
+}
+
+__global__ void k_shadow_accum(PtArgs a) {
+    const int64_t count = (int64_t)a.counts[2];
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < count; k += stride) {
+        if (a.socc[k]) continue;
+        float4 c = a.scontrib[k];
+        int64_t p = __float_as_int(c.w);
+        atomicAdd(&a.L[3 * p], c.x);
+        atomicAdd(&a.L[3 * p + 1], c.y);
+        atomicAdd(&a.L[3 * p + 2], c.z);
+    }
+}
+
+// Bookkeeping between bounces (a one-thread kernel, so the bounce loop needs
+// no host synchronisation and no memset nodes).
+__global__ void k_end_bounce(unsigned long long* counts, int cur) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        counts[3] += counts[cur];
+        counts[4] += counts[2];
+        counts[cur] = 0ull;
+        counts[2] = 0ull;
+    }
+}
+
+// Sum of the spp samples of each pixel into the FP64 accumulator.
+__global__ void k_accumulate(PtArgs a, int64_t npix) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npix) return;
+    double s0 = 0, s1 = 0, s2 = 0;
+    for (int s = 0; s < a.spp; ++s) {
+        int64_t p = i * a.spp + s;
+        s0 += (double)a.L[3 * p];
+        s1 += (double)a.L[3 * p + 1];
+        s2 += (double)a.L[3 * p + 2];
+    }
+    double* o = a.accum + 3 * (a.pix0 + i);
+    o[0] += s0;
+    o[1] += s1;
+    o[2] += s2;
+}
+
+}  // namespace ilm
+
+// ------------------------------------------------------------------ scene
+struct ilm_scene {
+    ilm_ctx* ctx = nullptr;
+    ilm_mesh* mesh = nullptr;
+    float4* d_shade = nullptr;
+    float* d_mats = nullptr;
+    float4* d_ltris = nullptr;
+    float4* d_lights = nullptr;
+    int2* d_lrange = nullptr;
+    int nlights = 0, nmat = 0;
+    // path-state buffers (grown on demand)
+    int64_t cap = 0;
+    std::vector<void*> bufs;
+    ilm::PtArgs a{};
+};
+
+namespace {
+
+void scene_free_state(ilm_scene* s) {
+    for (void* p : s->bufs) ilm_free(s->ctx, p);
+    s->bufs.clear();
+    s->cap = 0;
+}
+
+int scene_alloc(ilm_scene* s, size_t bytes, void** out) {
+    int rc = ilm_alloc(s->ctx, bytes, out);
+    if (rc == ILM_OK) s->bufs.push_back(*out);
+    return rc;
+}
+
+int ensure_state(ilm_scene* s, int64_t cap) {
+    if (s->cap >= cap) return ILM_OK;
+    scene_free_state(s);
+    int nl = std::max(1, s->nlights);
+    ilm::PtArgs& a = s->a;
+    int rc = 0;
+    rc |= scene_alloc(s, cap * sizeof(float4), (void**)&a.beta);
+    rc |= scene_alloc(s, cap * 3 * sizeof(float), (void**)&a.L);
+    rc |= scene_alloc(s, cap * sizeof(ilm_ray), (void**)&a.rays[0]);
+    rc |= scene_alloc(s, cap * sizeof(ilm_ray), (void**)&a.rays[1]);
+    rc |= scene_alloc(s, cap * sizeof(int32_t), (void**)&a.qpath[0]);
+    rc |= scene_alloc(s, cap * sizeof(int32_t), (void**)&a.qpath[1]);
+    rc |= scene_alloc(s, cap * sizeof(ilm_hit), (void**)&a.hits);
+    rc |= scene_alloc(s, cap * nl * sizeof(ilm_ray), (void**)&a.srays);
+    rc |= scene_alloc(s, cap * nl * sizeof(float4), (void**)&a.scontrib);
+    rc |= scene_alloc(s, cap * nl, (void**)&a.socc);
+    rc |= scene_alloc(s, 8 * sizeof(unsigned long long), (void**)&a.counts);
+    if (rc) {
+        scene_free_state(s);
+        return ILM_ERR_NOMEM;
+    }
+    s->cap = cap;
+    return ILM_OK;
+}
+
+}  // namespace
+
+using ilm::fail;
 
 extern "C" {
 
-int ilm_scene_create(ilm_ctx*, ilm_mesh*, const float*, const int32_t*, const int32_t*, const float*, int,
-                     const float*, int, ilm_scene** out) {
-    if (out) *out = nullptr;
-    return ilm::fail(ILM_ERR_STATE, "path integrator not available in this build");
+int ilm_scene_create(ilm_ctx* ctx, ilm_mesh* mesh, const float* nrm, const int32_t* tri_mat,
+                     const int32_t* tri_light, const float* mats, int nmat, const float* lights, int nlight,
+                     ilm_scene** out) {
+    if (!out) return fail(ILM_ERR_INVALID, "null out");
+    *out = nullptr;
+    if (!ctx || !mesh || !nrm || !tri_mat || !tri_light || (nmat > 0 && !mats) || (nlight > 0 && !lights))
+        return fail(ILM_ERR_INVALID, "null argument");
+    if (mesh->ctx != ctx) return fail(ILM_ERR_INVALID, "mesh belongs to another context");
+    if (!mesh->built) return fail(ILM_ERR_STATE, "build the mesh's BVH first");
+    if (nmat <= 0) return fail(ILM_ERR_INVALID, "a scene needs at least one material");
+    const int64_t nt = mesh->nt;
+    std::vector<float> shade(24 * nt);
+    std::vector<std::vector<int64_t>> ltri(nlight);
+    for (int64_t i = 0; i < nt; ++i) {
+        int m = tri_mat[i], l = tri_light[i];
+        if (m >= nmat || l >= nlight) return fail(ILM_ERR_INVALID, "material or light index out of range");
+        float* o = shade.data() + 24 * i;
+        for (int v = 0; v < 3; ++v) {
+            int64_t vi = mesh->idx[3 * i + v];
+            for (int k = 0; k < 3; ++k) {
+                o[4 * v + k] = mesh->pos[3 * vi + k];
+                o[12 + 4 * v + k] = nrm[3 * vi + k];
+            }
+            o[12 + 4 * v + 3] = 0.f;
+        }
+        int32_t mm = m < 0 ? -1 : m, ll = l < 0 ? -1 : l;
+        memcpy(&o[3], &mm, 4);
+        memcpy(&o[7], &ll, 4);
+        o[11] = 0.f;
+        if (l >= 0) ltri[l].push_back(i);
+    }
+    std::vector<float> lt, lrec(4 * std::max(1, nlight));
+    std::vector<int32_t> lrange(2 * std::max(1, nlight));
+    for (int l = 0; l < nlight; ++l) {
+        if (ltri[l].empty()) return fail(ILM_ERR_INVALID, "light " + std::to_string(l) + " has no triangles");
+        lrange[2 * l] = (int32_t)(lt.size() / 12);
+        lrange[2 * l + 1] = (int32_t)ltri[l].size();
+        double area = 0;
+        for (int64_t t : ltri[l]) {
+            float p[3][3];
+            for (int v = 0; v < 3; ++v) {
+                int64_t vi = mesh->idx[3 * t + v];
+                for (int k = 0; k < 3; ++k) p[v][k] = mesh->pos[3 * vi + k];
+                lt.insert(lt.end(), {p[v][0], p[v][1], p[v][2], 0.f});
+            }
+            ilm::f3 e1 = ilm::mk(p[1][0] - p[0][0], p[1][1] - p[0][1], p[1][2] - p[0][2]);
+            ilm::f3 e2 = ilm::mk(p[2][0] - p[0][0], p[2][1] - p[0][1], p[2][2] - p[0][2]);
+            area += (double)(ilm::length(ilm::cross(e1, e2)) * 0.5f);  // IndexedTriangle::GetArea
+        }
+        lrec[4 * l] = lights[4 * l];
+        lrec[4 * l + 1] = lights[4 * l + 1];
+        lrec[4 * l + 2] = lights[4 * l + 2];
+        lrec[4 * l + 3] = (float)area;
+    }
+    if (lt.empty()) lt.assign(12, 0.f);
+    ilm_scene* s = new ilm_scene();
+    s->ctx = ctx;
+    s->mesh = mesh;
+    s->nlights = nlight;
+    s->nmat = nmat;
+    int rc = 0;
+    rc = rc ? rc : scene_alloc(s, shade.size() * 4, (void**)&s->d_shade);
+    rc = rc ? rc : scene_alloc(s, (size_t)nmat * 8 * 4, (void**)&s->d_mats);
+    rc = rc ? rc : scene_alloc(s, lt.size() * 4, (void**)&s->d_ltris);
+    rc = rc ? rc : scene_alloc(s, lrec.size() * 4, (void**)&s->d_lights);
+    rc = rc ? rc : scene_alloc(s, lrange.size() * 4, (void**)&s->d_lrange);
+    rc = rc ? rc : ilm_upload(ctx, s->d_shade, shade.data(), shade.size() * 4);
+    rc = rc ? rc : ilm_upload(ctx, s->d_mats, mats, (size_t)nmat * 8 * 4);
+    rc = rc ? rc : ilm_upload(ctx, s->d_ltris, lt.data(), lt.size() * 4);
+    rc = rc ? rc : ilm_upload(ctx, s->d_lights, lrec.data(), lrec.size() * 4);
+    rc = rc ? rc : ilm_upload(ctx, s->d_lrange, lrange.data(), lrange.size() * 4);
+    if (rc) {
+        std::string msg = ilm_last_error();
+        for (void* p : s->bufs) ilm_free(ctx, p);
+        delete s;
+        return fail(rc, msg);
+    }
+    // scene buffers are not path state: keep them out of the state list
+    s->bufs.clear();
+    *out = s;
+    return ILM_OK;
 }
 
-int ilm_scene_destroy(ilm_scene*) { return ILM_OK; }
+int ilm_scene_destroy(ilm_scene* s) {
+    if (!s) return ILM_OK;
+    scene_free_state(s);
+    for (void* p : {(void*)s->d_shade, (void*)s->d_mats, (void*)s->d_ltris, (void*)s->d_lights, (void*)s->d_lrange})
+        if (p) ilm_free(s->ctx, p);
+    delete s;
+    return ILM_OK;
+}
 
-int ilm_render(ilm_scene*, const ilm_render_params*, double*, ilm_render_stats*) {
-    return ilm::fail(ILM_ERR_STATE, "path integrator not available in this build");
+int ilm_render(ilm_scene* s, const ilm_render_params* prm, double* accum, ilm_render_stats* stats) {
+    if (!s || !prm || !accum) return fail(ILM_ERR_INVALID, "null argument");
+    const ilm_render_params& p = *prm;
+    if (p.width <= 0 || p.height <= 0 || p.spp <= 0 || p.max_depth <= 0)
+        return fail(ILM_ERR_INVALID, "width, height, spp and max_depth must be positive");
+    ilm_ctx* ctx = s->ctx;
+    ILM_HIP(hipSetDevice(ctx->device));
+    const int64_t npix_total = (int64_t)p.width * p.height;
+    int64_t want = p.batch_paths > 0 ? p.batch_paths : (int64_t(1) << 22);
+    int64_t ppb = std::max<int64_t>(1, want / p.spp);          // pixels per batch
+    ppb = std::min(ppb, npix_total);
+    const int64_t cap = ppb * p.spp;
+    if (cap > (int64_t(1) << 31) - 1) return fail(ILM_ERR_INVALID, "batch too large");
+    int rc = ensure_state(s, cap);
+    if (rc) return fail(rc, "path state allocation");
+    ilm::PtArgs a = s->a;
+    a.shade = s->d_shade;
+    a.mats = s->d_mats;
+    a.ltris = s->d_ltris;
+    a.lights = s->d_lights;
+    a.lrange = s->d_lrange;
+    a.nlights = s->nlights;
+    {
+        float fr[12];
+        ilm::camera_frame(p.cam, fr);  // same frame as ilm_raygen_camera
+        a.cam.centre = ilm::mk(fr[0], fr[1], fr[2]);
+        a.cam.corner = ilm::mk(fr[3], fr[4], fr[5]);
+        a.cam.across = ilm::mk(fr[6], fr[7], fr[8]);
+        a.cam.up = ilm::mk(fr[9], fr[10], fr[11]);
+    }
+    a.W = p.width;
+    a.H = p.height;
+    a.spp = p.spp;
+    a.spp_offset = p.spp_offset;
+    a.seed = p.seed;
+    a.max_depth = p.max_depth;
+    a.rr_depth = p.rr_depth;
+    a.eps = p.eps;
+    a.accum = accum;
+    hipStream_t st = ctx->stream;
+    int ev0 = 0, ev1 = 0;
+    rc = ilm_timer_mark(ctx, &ev0);
+    if (rc) return rc;
+    ILM_HIP(hipMemsetAsync(a.counts, 0, 8 * sizeof(unsigned long long), st));
+    const int grid = ctx->cu_count * 8;
+    for (int64_t pix0 = 0; pix0 < npix_total; pix0 += ppb) {
+        int64_t npix = std::min(ppb, npix_total - pix0);
+        a.pix0 = pix0;
+        a.npaths = npix * p.spp;
+        hipLaunchKernelGGL(ilm::k_generate, dim3((unsigned)((a.npaths + 255) / 256)), dim3(256), 0, st, a);
+        int cur = 0;
+        for (int b = 0; b < p.max_depth; ++b) {
+            rc = ilm::trace_queue(s->mesh, a.rays[cur], a.counts + cur, cap, a.hits, nullptr, st);
+            if (rc) return rc;
+            hipLaunchKernelGGL(ilm::k_shade, dim3(grid), dim3(ilm::kPtBlock), 0, st, a, b, cur);
+            if (s->nlights > 0) {
+                rc = ilm::trace_queue(s->mesh, a.srays, a.counts + 2, cap * s->nlights, nullptr, a.socc, st);
+                if (rc) return rc;
+                hipLaunchKernelGGL(ilm::k_shadow_accum, dim3(grid), dim3(256), 0, st, a);
+            }
+            hipLaunchKernelGGL(ilm::k_end_bounce, dim3(1), dim3(1), 0, st, a.counts, cur);
+            cur ^= 1;
+        }
+        hipLaunchKernelGGL(ilm::k_accumulate, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, st, a, npix);
+        ILM_HIP(hipGetLastError());
+    }
+    rc = ilm_timer_mark(ctx, &ev1);
+    if (rc) return rc;
+    float ms = 0.f;
+    rc = ilm_timer_elapsed(ctx, ev0, ev1, &ms);
+    if (rc) return rc;
+    unsigned long long cnt[8];
+    ILM_HIP(hipMemcpy(cnt, a.counts, sizeof(cnt), hipMemcpyDeviceToHost));
+    if (stats) {
+        stats->camera_rays = npix_total * p.spp;
+        stats->extension_rays = (int64_t)cnt[3] - stats->camera_rays;
+        stats->shadow_rays = (int64_t)cnt[4];
+        stats->ms = ms;
+    }
+    return ilm_sync(ctx);
 }
 
 }  // extern "C"

@@ -27,7 +27,8 @@ struct TraceArgs {
     const float4* __restrict__ nodes;  // 7 float4 per BVH4 node
     const float4* __restrict__ tris;   // 3 float4 per triangle
     const ilm_ray* __restrict__ rays;
-    int64_t n;
+    int64_t n;                        // ray count, or capacity when n_dev is set
+    const unsigned long long* n_dev;  // ray count in device memory (wavefront queues), or null
     ilm_hit* __restrict__ hits;
     uint8_t* __restrict__ occ;
     int32_t* __restrict__ stats;
@@ -59,6 +60,7 @@ __global__ void __launch_bounds__(kBlock) k_trace4(TraceArgs a) {
     __shared__ uint32_t lds[kLds * kBlock];
     const int tid = threadIdx.x;
     const int64_t gtid = (int64_t)blockIdx.x * kBlock + tid;
+    const int64_t n = a.n_dev ? (int64_t)min((unsigned long long)a.n, *a.n_dev) : a.n;
 
     bool active = false;
     bool exhausted = false;  // uniform across the wave
@@ -114,10 +116,10 @@ __global__ void __launch_bounds__(kBlock) k_trace4(TraceArgs a) {
             unsigned long long base = 0;
             if (__lane_id() == 0) base = atomicAdd(&a.counters[0], (unsigned long long)__popcll(idle));
             base = __shfl(base, 0);
-            if (base + (unsigned long long)__popcll(idle) >= (unsigned long long)a.n) exhausted = true;
+            if (base + (unsigned long long)__popcll(idle) >= (unsigned long long)n) exhausted = true;
             if (!active) {
                 int64_t my = (int64_t)base + __popcll(idle & __lanemask_lt());
-                if (my < a.n) {
+                if (my < n) {
                     ray_id = my;
                     const float4* rp = reinterpret_cast<const float4*>(a.rays + my);
                     float4 o4 = rp[0], d4 = rp[1];
@@ -192,7 +194,7 @@ __global__ void __launch_bounds__(kBlock) k_trace4(TraceArgs a) {
 
 template <bool ANY, bool STATS>
 static int launch_trace(ilm_mesh* m, const ilm_ray* rays, int64_t n, ilm_hit* hits, uint8_t* occ, int32_t* stats,
-                        void* stream) {
+                        void* stream, const unsigned long long* n_dev = nullptr) {
     if (!m) return fail(ILM_ERR_INVALID, "null mesh");
     if (!m->built) return fail(ILM_ERR_STATE, "BVH not built");
     if (!m->ctx) return fail(ILM_ERR_STATE, "mesh was created without a device context");
@@ -214,6 +216,7 @@ static int launch_trace(ilm_mesh* m, const ilm_ray* rays, int64_t n, ilm_hit* hi
     a.tris = reinterpret_cast<const float4*>(m->d_tris);
     a.rays = rays;
     a.n = n;
+    a.n_dev = n_dev;
     a.hits = hits;
     a.occ = occ;
     a.stats = stats;
@@ -259,6 +262,16 @@ static CamConsts camera_consts(const ilm_camera& c) {
     k.across = (u1 - u0) * U;
     k.up = (v1 - v0) * V;
     return k;
+}
+
+void camera_frame(const ilm_camera& c, float out[12]) {
+    CamConsts k = camera_consts(c);
+    const f3 v[4] = {k.centre, k.corner, k.across, k.up};
+    for (int i = 0; i < 4; ++i) {
+        out[3 * i] = v[i].x;
+        out[3 * i + 1] = v[i].y;
+        out[3 * i + 2] = v[i].z;
+    }
 }
 
 __device__ __forceinline__ void camera_ray(const CamConsts& c, int W, int H, int x, int y, float u, float v,
@@ -429,6 +442,14 @@ __global__ void k_copy(const float4* __restrict__ src, float4* __restrict__ dst,
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
 }
 
+}  // namespace ilm
+
+namespace ilm {
+int trace_queue(ilm_mesh* m, const ilm_ray* rays, const unsigned long long* n_dev, int64_t capacity, ilm_hit* hits,
+                uint8_t* occ, hipStream_t stream) {
+    if (occ) return launch_trace<true, false>(m, rays, capacity, nullptr, occ, nullptr, stream, n_dev);
+    return launch_trace<false, false>(m, rays, capacity, hits, nullptr, nullptr, stream, n_dev);
+}
 }  // namespace ilm
 
 using namespace ilm;

@@ -80,3 +80,86 @@ def intersection_rays(ctx: Context, mesh: Mesh, width=2048, height=2048, seed=12
     rs.replaced = int(len(miss))
     rs.centre_hits = nhit
     return rs
+
+
+# ---------------------------------------------------------------- box room
+# Plain path-tracing workload: procedural box room in the unit cube, open at
+# z = 0: floor, ceiling, back, left and right walls (10 triangles) plus two
+# axis-aligned boxes (24 triangles), Lambertian; one ceiling quad emitter
+# (2 triangles, Le = (17, 12, 4), facing down). Geometric normals face the
+# room interior / box exterior; vertex normals equal the face normals.
+# Pinhole camera at (0.5, 0.5, -1.4) looking at (0.5, 0.5, 0.5), fov 40 deg.
+ROOM_LE = (17.0, 12.0, 4.0)
+ROOM_MATS = np.array([
+    [0.73, 0.73, 0.73, 0, 0, 0, 1, 0],    # 0 white
+    [0.63, 0.065, 0.05, 0, 0, 0, 1, 0],   # 1 red (left wall)
+    [0.14, 0.45, 0.09, 0, 0, 0, 1, 0],    # 2 green (right wall)
+    [0.0, 0.0, 0.0, 0, 0, 0, 1, 0],       # 3 emitter surface (black)
+], np.float32)
+
+
+def _quad(p0, p1, p2, p3, want_normal):
+    p = np.array([p0, p1, p2, p3], np.float64)
+    n = np.cross(p[1] - p[0], p[2] - p[0])
+    n /= np.linalg.norm(n)
+    if np.dot(n, want_normal) < 0:
+        p = p[[0, 3, 2, 1]]
+        n = -n
+    return p, n
+
+
+def _box_faces(lo, hi):
+    lo, hi = np.asarray(lo, np.float64), np.asarray(hi, np.float64)
+    c = (lo + hi) / 2
+    faces = []
+    for axis in range(3):
+        for side in (0, 1):
+            v = hi[axis] if side else lo[axis]
+            a1, a2 = (axis + 1) % 3, (axis + 2) % 3
+            corners = []
+            for (s1, s2) in ((0, 0), (1, 0), (1, 1), (0, 1)):
+                q = np.zeros(3)
+                q[axis] = v
+                q[a1] = hi[a1] if s1 else lo[a1]
+                q[a2] = hi[a2] if s2 else lo[a2]
+                corners.append(q)
+            want = np.zeros(3)
+            want[axis] = 1.0 if side else -1.0
+            faces.append(_quad(*corners, want))
+    return faces
+
+
+def box_room():
+    """Returns pos, nrm, idx, tri_mat, tri_light, mats, lights (36 triangles)."""
+    quads = []  # (points, normal, material, light)
+    quads.append((*_quad([0, 0, 0], [1, 0, 0], [1, 0, 1], [0, 0, 1], [0, 1, 0]), 0, -1))    # floor
+    quads.append((*_quad([0, 1, 0], [1, 1, 0], [1, 1, 1], [0, 1, 1], [0, -1, 0]), 0, -1))   # ceiling
+    quads.append((*_quad([0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1], [0, 0, -1]), 0, -1))   # back
+    quads.append((*_quad([0, 0, 0], [0, 1, 0], [0, 1, 1], [0, 0, 1], [1, 0, 0]), 1, -1))    # left
+    quads.append((*_quad([1, 0, 0], [1, 1, 0], [1, 1, 1], [1, 0, 1], [-1, 0, 0]), 2, -1))   # right
+    for lo, hi in (([0.13, 0.0, 0.15], [0.43, 0.30, 0.45]), ([0.55, 0.0, 0.45], [0.85, 0.60, 0.75])):
+        for p, n in _box_faces(lo, hi):
+            quads.append((p, n, 0, -1))
+    y = 0.999
+    quads.append((*_quad([0.4, y, 0.4], [0.6, y, 0.4], [0.6, y, 0.6], [0.4, y, 0.6], [0, -1, 0]), 3, 0))
+    pos, nrm, idx, tm, tl = [], [], [], [], []
+    for p, n, m, l in quads:
+        b = len(pos)
+        pos += list(p)
+        nrm += [n] * 4
+        idx += [[b, b + 1, b + 2], [b, b + 2, b + 3]]
+        tm += [m, m]
+        tl += [l, l]
+    lights = np.array([[*ROOM_LE, 0.0]], np.float32)
+    return (np.array(pos, np.float32), np.array(nrm, np.float32), np.array(idx, np.int32),
+            np.array(tm, np.int32), np.array(tl, np.int32), ROOM_MATS.copy(), lights)
+
+
+def room_camera():
+    return Camera.make((0.5, 0.5, -1.4), (0.5, 0.5, 0.5), (0.0, 1.0, 0.0), 40.0, 1.0)
+
+
+def rel_rmse(img, ref):
+    """sqrt(mean over pixels and channels of (I - R)^2 / (R^2 + 0.01))."""
+    img, ref = np.asarray(img, np.float64), np.asarray(ref, np.float64)
+    return float(np.sqrt(np.mean((img - ref) ** 2 / (ref ** 2 + 0.01))))
