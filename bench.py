@@ -98,7 +98,7 @@ def path_tracing_room(ctx, ilm, workloads, args):
         frames.append(sc.render(cam, 256, 256, 16, acc, seed=1000 + i))
     ms = float(np.mean([f["ms"] for f in frames]))
     rays = frames[0]["camera_rays"] + frames[0]["extension_rays"] + frames[0]["shadow_rays"]
-    out = {"workload": "box room 34+2 tris, 256x256, 16 spp, depth 5", "frame_ms": round(ms, 3),
+    out = {"workload": "box room 40+2 tris, 256x256, 16 spp, depth 5", "frame_ms": round(ms, 3),
            "mrays_s": round(rays / ms / 1e3, 1), "rays_per_frame": int(rays)}
     if not args.no_cpu:
         import oracle

@@ -66,6 +66,11 @@ class RenderParams(C.Structure):
                 ("eps", C.c_float), ("seed", C.c_uint32), ("batch_paths", C.c_int32)]
 
 
+class NrcParams(C.Structure):
+    _fields_ = [("bounds_lo", C.c_float * 3), ("bounds_hi", C.c_float * 3), ("lr", C.c_float),
+                ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float), ("seed", C.c_uint32)]
+
+
 class RenderStats(C.Structure):
     _fields_ = [("camera_rays", C.c_int64), ("extension_rays", C.c_int64), ("shadow_rays", C.c_int64),
                 ("ms", C.c_double)]
@@ -107,6 +112,14 @@ SIGNATURES = {
     "ilm_scene_create": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.c_int, _vp, C.c_int, C.POINTER(_vp)]),
     "ilm_scene_destroy": (C.c_int, [_vp]),
     "ilm_render": (C.c_int, [_vp, C.POINTER(RenderParams), _vp, C.POINTER(RenderStats)]),
+    "ilm_nrc_create": (C.c_int, [_vp, C.POINTER(NrcParams), C.POINTER(_vp)]),
+    "ilm_nrc_destroy": (C.c_int, [_vp]),
+    "ilm_nrc_param_count": (C.c_int, [C.POINTER(C.c_int)]),
+    "ilm_nrc_set_weights": (C.c_int, [_vp, _vp]),
+    "ilm_nrc_get_weights": (C.c_int, [_vp, _vp]),
+    "ilm_nrc_get_gradients": (C.c_int, [_vp, _vp]),
+    "ilm_nrc_infer": (C.c_int, [_vp, _vp, _i64, _vp, _vp]),
+    "ilm_nrc_train_step": (C.c_int, [_vp, _vp, _vp, _i64, _vp, _vp]),
 }
 
 
@@ -316,6 +329,51 @@ class Scene:
     def close(self):
         if self.h:
             check(load().ilm_scene_destroy(self.h))
+            self.h = None
+
+
+class Nrc:
+    """Neural radiance cache (fused bf16 MFMA MLP, trained online)."""
+
+    QUERY_FLOATS = 16
+
+    def __init__(self, ctx: Context, bounds_lo, bounds_hi, lr=1e-3, beta1=0.9, beta2=0.99, eps=1e-8, seed=1):
+        self.ctx = ctx
+        p = NrcParams()
+        for i in range(3):
+            p.bounds_lo[i], p.bounds_hi[i] = bounds_lo[i], bounds_hi[i]
+        p.lr, p.beta1, p.beta2, p.eps, p.seed = lr, beta1, beta2, eps, seed
+        h = C.c_void_p()
+        check(load().ilm_nrc_create(ctx.h, C.byref(p), C.byref(h)))
+        self.h = h.value
+        n = C.c_int()
+        check(load().ilm_nrc_param_count(C.byref(n)))
+        self.nparams = n.value
+
+    def weights(self):
+        w = np.zeros(self.nparams, np.float32)
+        check(load().ilm_nrc_get_weights(self.h, _ptr(w)))
+        return w
+
+    def gradients(self):
+        g = np.zeros(self.nparams, np.float32)
+        check(load().ilm_nrc_get_gradients(self.h, _ptr(g)))
+        return g
+
+    def set_weights(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        assert w.size == self.nparams
+        check(load().ilm_nrc_set_weights(self.h, _ptr(w)))
+
+    def infer(self, queries: DeviceBuffer, n: int, out: DeviceBuffer):
+        check(load().ilm_nrc_infer(self.h, queries.ptr, n, out.ptr, None))
+
+    def train_step(self, queries: DeviceBuffer, targets: DeviceBuffer, n: int, loss: DeviceBuffer | None = None):
+        check(load().ilm_nrc_train_step(self.h, queries.ptr, targets.ptr, n, loss.ptr if loss else None, None))
+
+    def close(self):
+        if self.h:
+            check(load().ilm_nrc_destroy(self.h))
             self.h = None
 
 

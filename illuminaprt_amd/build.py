@@ -19,7 +19,7 @@ OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libilm_gpu.so")
 ARCH = "gfx950"
 
-SOURCES = ["api.cpp", "bvh_build.cpp", "scenegen.cpp", "trace.hip", "pt.hip"]
+SOURCES = ["api.cpp", "bvh_build.cpp", "scenegen.cpp", "trace.hip", "pt.hip", "nrc.hip"]
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 

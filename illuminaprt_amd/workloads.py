@@ -84,11 +84,16 @@ def intersection_rays(ctx: Context, mesh: Mesh, width=2048, height=2048, seed=12
 
 # ---------------------------------------------------------------- box room
 # Plain path-tracing workload: procedural box room in the unit cube, open at
-# z = 0: floor, ceiling, back, left and right walls (10 triangles) plus two
-# axis-aligned boxes (24 triangles), Lambertian; one ceiling quad emitter
-# (2 triangles, Le = (17, 12, 4), facing down). Geometric normals face the
-# room interior / box exterior; vertex normals equal the face normals.
-# Pinhole camera at (0.5, 0.5, -1.4) looking at (0.5, 0.5, 0.5), fov 40 deg.
+# z = 0: floor, back, left and right walls (2 triangles each), the ceiling as
+# four quads around a 0.2 x 0.2 opening (8 triangles), two axis-aligned boxes
+# (24 triangles), Lambertian; one emitter quad (2 triangles, Le = (17, 12, 4),
+# facing down) filling the opening flush with the ceiling. The emitter is
+# flush because the reference's mesh-light sampling turns the emitter normal
+# toward the shaded point (Shape/TriangleMesh.cpp:150-161): an emitter hung
+# below the ceiling would light the ceiling around it at grazing angles with
+# an unbounded 1/d^4-like estimator. Geometric normals face the room interior
+# / box exterior; vertex normals equal the face normals. Pinhole camera at
+# (0.5, 0.5, -1.4) looking at (0.5, 0.5, 0.5), fov 40 deg.
 ROOM_LE = (17.0, 12.0, 4.0)
 ROOM_MATS = np.array([
     [0.73, 0.73, 0.73, 0, 0, 0, 1, 0],    # 0 white
@@ -130,18 +135,18 @@ def _box_faces(lo, hi):
 
 
 def box_room():
-    """Returns pos, nrm, idx, tri_mat, tri_light, mats, lights (36 triangles)."""
+    """Returns pos, nrm, idx, tri_mat, tri_light, mats, lights (42 triangles)."""
     quads = []  # (points, normal, material, light)
     quads.append((*_quad([0, 0, 0], [1, 0, 0], [1, 0, 1], [0, 0, 1], [0, 1, 0]), 0, -1))    # floor
-    quads.append((*_quad([0, 1, 0], [1, 1, 0], [1, 1, 1], [0, 1, 1], [0, -1, 0]), 0, -1))   # ceiling
+    for (x0, x1, z0, z1) in ((0, 1, 0, 0.4), (0, 1, 0.6, 1), (0, 0.4, 0.4, 0.6), (0.6, 1, 0.4, 0.6)):  # ceiling
+        quads.append((*_quad([x0, 1, z0], [x1, 1, z0], [x1, 1, z1], [x0, 1, z1], [0, -1, 0]), 0, -1))
     quads.append((*_quad([0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1], [0, 0, -1]), 0, -1))   # back
     quads.append((*_quad([0, 0, 0], [0, 1, 0], [0, 1, 1], [0, 0, 1], [1, 0, 0]), 1, -1))    # left
     quads.append((*_quad([1, 0, 0], [1, 1, 0], [1, 1, 1], [1, 0, 1], [-1, 0, 0]), 2, -1))   # right
     for lo, hi in (([0.13, 0.0, 0.15], [0.43, 0.30, 0.45]), ([0.55, 0.0, 0.45], [0.85, 0.60, 0.75])):
         for p, n in _box_faces(lo, hi):
             quads.append((p, n, 0, -1))
-    y = 0.999
-    quads.append((*_quad([0.4, y, 0.4], [0.6, y, 0.4], [0.6, y, 0.6], [0.4, y, 0.6], [0, -1, 0]), 3, 0))
+    quads.append((*_quad([0.4, 1, 0.4], [0.6, 1, 0.4], [0.6, 1, 0.6], [0.4, 1, 0.6], [0, -1, 0]), 3, 0))
     pos, nrm, idx, tm, tl = [], [], [], [], []
     for p, n, m, l in quads:
         b = len(pos)

@@ -270,6 +270,50 @@ typedef struct {
 int ilm_render(ilm_scene* scene, const ilm_render_params* params, double* accum,
                ilm_render_stats* stats);
 
+/* ---------------------------------------------------------------- neural radiance cache */
+
+/* A small MLP (96 -> 5 x 64 ReLU -> 16, bf16 weights and activations, FP32
+ * accumulation and FP32 master weights, Adam) trained online on radiance
+ * produced by the integrator's own paths. No reference counterpart: it
+ * stands in for the irradiance cache (Integrator/ICIntegrator.cpp:29-668)
+ * at the far end of each path.
+ * A query is 16 floats: position xyz, direction (theta/pi, phi/2pi), normal
+ * (theta/pi, phi/2pi), roughness, diffuse albedo rgb, specular albedo rgb,
+ * two unused. The position is normalised by the bounds given at creation. */
+typedef struct ilm_nrc ilm_nrc;
+
+typedef struct {
+    float bounds_lo[3], bounds_hi[3];
+    float lr;      /* Adam learning rate (default 1e-3 when 0) */
+    float beta1;   /* default 0.9 */
+    float beta2;   /* default 0.99 */
+    float eps;     /* default 1e-8 */
+    uint32_t seed; /* weight initialisation */
+} ilm_nrc_params;
+
+/* Checks the MFMA operand layout once per process (fails with ILM_ERR_STATE
+ * if it differs from the one the kernels assume), then allocates and
+ * initialises the network. */
+int ilm_nrc_create(ilm_ctx* ctx, const ilm_nrc_params* params, ilm_nrc** out);
+int ilm_nrc_destroy(ilm_nrc* nrc);
+/* Number of FP32 master weights (row-major per layer, padding included). */
+int ilm_nrc_param_count(int* count);
+/* Host copies of the FP32 master weights (param_count floats). Setting
+ * them resets the Adam moments. */
+int ilm_nrc_set_weights(ilm_nrc* nrc, const float* weights);
+int ilm_nrc_get_weights(ilm_nrc* nrc, float* weights);
+/* Summed loss gradient of the last training step (param_count floats),
+ * kept so the backward kernels can be checked against a CPU oracle. */
+int ilm_nrc_get_gradients(ilm_nrc* nrc, float* grads);
+/* Radiance predictions for n queries (dev): out = n x 4 floats (rgb, 0). */
+int ilm_nrc_infer(ilm_nrc* nrc, const float* queries, int64_t n, float* out, void* stream);
+/* One Adam step on n (query, target) pairs (dev). targets = n x 4 floats
+ * (rgb, weight). Loss: mean over samples and channels of
+ * weight * (y - t)^2 / (y^2 + 0.01), the denominator held constant in the
+ * gradient. loss (dev, n floats) receives per-sample losses, or NULL. */
+int ilm_nrc_train_step(ilm_nrc* nrc, const float* queries, const float* targets, int64_t n,
+                       float* loss, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,122 @@
+"""GPU parity of the neural radiance cache kernels (bf16 MFMA forward,
+backward, weight gradients and Adam) against the numpy oracle, through the
+C ABI. Tolerances: the oracle rounds to bf16 at the same points as the GPU;
+what remains is FP32 accumulation order (about 1e-3 relative per layer)."""
+import numpy as np
+import pytest
+
+from oracle import nrc as onrc
+
+pytestmark = pytest.mark.gpu
+
+LO, HI = [-1.0, -1.0, -1.0], [1.0, 1.0, 1.0]
+
+
+def _queries(n, seed):
+    rng = np.random.default_rng(seed)
+    q = np.zeros((n, 16), np.float32)
+    q[:, 0:3] = rng.uniform(-1, 1, (n, 3))
+    q[:, 3:7] = rng.uniform(0, 1, (n, 4))
+    q[:, 7] = rng.uniform(0.1, 1, n)
+    q[:, 8:14] = rng.uniform(0, 1, (n, 6))
+    return q
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def test_infer_matches_oracle(gpu_ctx, ilm):
+    net = ilm.Nrc(gpu_ctx, LO, HI, seed=3)
+    w = onrc.init_weights(5)
+    net.set_weights(w)
+    assert np.array_equal(net.weights(), w)
+    n = 5003                                   # not a multiple of the 32-sample tile
+    q = _queries(n, 1)
+    qb = gpu_ctx.to_device(q)
+    ob = gpu_ctx.buffer(n * 16)
+    net.infer(qb, n, ob)
+    gpu_ctx.sync()
+    y = ob.download(np.float32, 4 * n).reshape(n, 4)[:, :3]
+    x = onrc.encode(q, LO, HI)
+    _, yb = onrc.forward(w, x, use_bf16=True)
+    _, yf = onrc.forward(w, x, use_bf16=False)
+    e_b, e_f = _rel(y, yb[:, :3]), _rel(y, yf[:, :3])
+    print(f"[nrc infer] rel err vs bf16 oracle {e_b:.2e}, vs fp64 oracle {e_f:.2e}")
+    assert e_b < 5e-3 and e_f < 3e-2
+    net.close()
+
+
+def test_gradients_and_adam_match_oracle(gpu_ctx, ilm):
+    net = ilm.Nrc(gpu_ctx, LO, HI, lr=1e-3)
+    w = onrc.init_weights(7)
+    net.set_weights(w)
+    n = 3001
+    q = _queries(n, 2)
+    rng = np.random.default_rng(4)
+    t = np.zeros((n, 4), np.float32)
+    t[:, :3] = rng.uniform(0, 2, (n, 3))
+    t[:, 3] = 1.0
+    qb, tb = gpu_ctx.to_device(q), gpu_ctx.to_device(t)
+    lb = gpu_ctx.buffer(n * 4)
+    net.train_step(qb, tb, n, lb)
+    gpu_ctx.sync()
+    g = net.gradients().astype(np.float64)
+    w1 = net.weights().astype(np.float64)
+    x = onrc.encode(q, LO, HI)
+    loss, go = onrc.loss_grad(w, x, t, use_bf16=True)
+    gl = lb.download(np.float32, n)
+    print(f"[nrc grad] loss gpu {gl.sum() / (3 * n):.6f} oracle {loss:.6f}")
+    assert abs(gl.sum() / (3 * n) - loss) < 1e-3 * loss
+    off = 0
+    for k, (o, i) in enumerate(onrc.SHAPES):
+        s = slice(off, off + o * i)
+        e = _rel(g[s], go[s])
+        print(f"[nrc grad] layer {k} rel err {e:.2e} |g| {np.linalg.norm(go[s]):.3e}")
+        assert e < 2e-2
+        off += o * i
+    # padding stays exactly zero
+    ws = onrc.split(g)
+    assert (ws[0][:, 83:] == 0).all() and (ws[5][3:, :] == 0).all()
+    # Adam step 1 from the GPU's own gradient
+    wa, _, _ = onrc.adam(w, g, 0, 0, 1, lr=1e-3, b1=0.9, b2=0.99, eps=1e-8)
+    assert np.abs(w1 - wa).max() < 1e-6
+    net.close()
+
+
+def test_training_fits_a_smooth_field(gpu_ctx, ilm):
+    """Online training on a fixed synthetic radiance field lowers the loss and
+    generalises to held-out queries."""
+    def field(q):
+        p = q[:, 0:3]
+        r = 0.6 + 0.3 * np.sin(2 * p[:, 0]) * np.cos(3 * p[:, 1]) + 0.2 * q[:, 8]
+        gch = 0.4 + 0.3 * np.cos(2 * p[:, 2] + p[:, 0]) * q[:, 6]
+        b = 0.2 + 0.15 * (p[:, 1] > 0) + 0.1 * q[:, 9]
+        t = np.stack([r, gch, b, np.ones(len(q))], axis=1).astype(np.float32)
+        return t
+
+    net = ilm.Nrc(gpu_ctx, LO, HI, lr=1e-2, seed=11)
+    n = 16384
+    losses = []
+    lb = gpu_ctx.buffer(n * 4)
+    for step in range(300):
+        q = _queries(n, 100 + step)
+        qb, tb = gpu_ctx.to_device(q), gpu_ctx.to_device(field(q))
+        net.train_step(qb, tb, n, lb)
+        if step % 50 == 0 or step == 299:
+            gpu_ctx.sync()
+            losses.append(float(lb.download(np.float32, n).mean() / 3))
+        qb.free()
+        tb.free()
+    q = _queries(4096, 99999)
+    qb = gpu_ctx.to_device(q)
+    ob = gpu_ctx.buffer(4096 * 16)
+    net.infer(qb, 4096, ob)
+    gpu_ctx.sync()
+    y = ob.download(np.float32, 4 * 4096).reshape(-1, 4)[:, :3]
+    t = field(q)[:, :3]
+    err = float(np.sqrt(np.mean((y - t) ** 2)) / np.sqrt(np.mean(t ** 2)))
+    print(f"[nrc fit] losses {losses} held-out rel RMS {err:.4f}")
+    assert losses[-1] < losses[0] / 10
+    assert err < 0.1
+    net.close()
