@@ -71,6 +71,11 @@ class NrcParams(C.Structure):
                 ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float), ("seed", C.c_uint32)]
 
 
+class NrcRenderParams(C.Structure):
+    _fields_ = [("prefix_depth", C.c_int32), ("suffix_depth", C.c_int32), ("train_fraction", C.c_float),
+                ("train_batch", C.c_int32), ("train_steps", C.c_int32), ("accumulate", C.c_int32)]
+
+
 class RenderStats(C.Structure):
     _fields_ = [("camera_rays", C.c_int64), ("extension_rays", C.c_int64), ("shadow_rays", C.c_int64),
                 ("ms", C.c_double)]
@@ -120,6 +125,8 @@ SIGNATURES = {
     "ilm_nrc_get_gradients": (C.c_int, [_vp, _vp]),
     "ilm_nrc_infer": (C.c_int, [_vp, _vp, _i64, _vp, _vp]),
     "ilm_nrc_train_step": (C.c_int, [_vp, _vp, _vp, _i64, _vp, _vp]),
+    "ilm_render_nrc": (C.c_int, [_vp, _vp, C.POINTER(RenderParams), C.POINTER(NrcRenderParams), _vp,
+                                 C.POINTER(RenderStats)]),
 }
 
 
@@ -324,6 +331,18 @@ class Scene:
         p = RenderParams(cam, width, height, spp, spp_offset, max_depth, rr_depth, eps, seed, batch_paths)
         st = RenderStats()
         check(load().ilm_render(self.h, C.byref(p), accum.ptr, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in st._fields_}
+
+    def render_nrc(self, nrc: "Nrc", cam: Camera, width, height, spp, accum: DeviceBuffer | None, seed=1234,
+                   spp_offset=0, rr_depth=4, eps=1e-4, prefix_depth=2, suffix_depth=8, train_fraction=1 / 16,
+                   train_batch=65536, train_steps=1, batch_paths=0) -> dict:
+        p = RenderParams(cam, width, height, spp, spp_offset, prefix_depth + suffix_depth, rr_depth, eps, seed,
+                         batch_paths)
+        q = NrcRenderParams(prefix_depth, suffix_depth, train_fraction, train_batch, train_steps,
+                            1 if accum is not None else 0)
+        st = RenderStats()
+        check(load().ilm_render_nrc(self.h, nrc.h, C.byref(p), C.byref(q), accum.ptr if accum else None,
+                                    C.byref(st)))
         return {k: getattr(st, k) for k, _ in st._fields_}
 
     def close(self):

@@ -1,8 +1,10 @@
-"""Builds libilm_gpu.so in-tree with hipcc for gfx950.
+"""Builds libilm_gpu.so in-tree with hipcc for gfx950, then the C++ plug-in.
 
 Every source is compiled as HIP (device + host) into illuminaprt_amd/_build/
 and linked into illuminaprt_amd/libilm_gpu.so. The packed-FP32 feature is
-disabled for all device code.
+disabled for all device code. The plug-in layer (plugin/ilm_plugin.cpp, host
+C++ over the C ABI) is built with g++ into libilm_plugin.so, next to the
+sandbox program that loads it (ilm_plugin_sandbox).
 """
 from __future__ import annotations
 
@@ -17,6 +19,9 @@ ROOT = os.path.dirname(HERE)
 SRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libilm_gpu.so")
+PLUGIN = os.path.join(HERE, "libilm_plugin.so")
+SANDBOX = os.path.join(HERE, "ilm_plugin_sandbox")
+PLUGIN_SRC = os.path.join(HERE, "plugin")
 ARCH = "gfx950"
 
 SOURCES = ["api.cpp", "bvh_build.cpp", "scenegen.cpp", "trace.hip", "pt.hip", "nrc.hip"]
@@ -57,7 +62,32 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_plugin(verbose)
     return LIB
+
+
+def _run(cmd: list[str], what: str) -> None:
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"{what} failed\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+
+
+def build_plugin(verbose: bool = False) -> None:
+    """The plug-in links libilm_gpu.so and finds it through $ORIGIN. The
+    sandbox links the plug-in's classes (as the reference's sandbox links its
+    core library) and still resolves the plug-in's entry symbols with dlopen."""
+    cxx = shutil.which("g++") or "g++"
+    flags = ["-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include")]
+    tmp = PLUGIN + ".tmp"
+    _run([cxx, *flags, "-fPIC", "-shared", os.path.join(PLUGIN_SRC, "ilm_plugin.cpp"), "-o", tmp,
+          "-L", HERE, "-lilm_gpu", "-Wl,-rpath,$ORIGIN", "-lpthread"], "plug-in build")
+    os.replace(tmp, PLUGIN)
+    tmp = SANDBOX + ".tmp"
+    _run([cxx, *flags, os.path.join(PLUGIN_SRC, "plugin_sandbox.cpp"), "-o", tmp,
+          "-L", HERE, "-lilm_plugin", "-lilm_gpu", "-Wl,-rpath,$ORIGIN", "-ldl"], "sandbox build")
+    os.replace(tmp, SANDBOX)
+    if verbose:
+        print(f"built {PLUGIN} and {SANDBOX}")
 
 
 if __name__ == "__main__":

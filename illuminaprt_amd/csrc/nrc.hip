@@ -145,7 +145,8 @@ __device__ __forceinline__ void lds_copy(__bf16* dst, const __bf16* src, int ele
 struct InferArgs {
     const __bf16* fwd;
     const float* queries;
-    int64_t n;
+    int64_t n;                        // query count, or capacity when n_dev is set
+    const unsigned long long* n_dev;  // query count in device memory, or null
     Bounds bd;
     float* out;  // 4 floats per query (rgb, 0)
 };
@@ -158,11 +159,12 @@ __global__ void __launch_bounds__(256) k_infer(InferArgs a) {
     __syncthreads();
     const int lane = (int)__lane_id(), g = lane >> 4, m = lane & 15;
     const int wave = threadIdx.x >> 6;
+    const int64_t n = a.n_dev ? (int64_t)min((unsigned long long)a.n, *a.n_dev) : a.n;
     const int64_t step = (int64_t)gridDim.x * 4 * 16 * kTiles;
-    for (int64_t base = ((int64_t)blockIdx.x * 4 + wave) * 16 * kTiles; base < a.n; base += step) {
+    for (int64_t base = ((int64_t)blockIdx.x * 4 + wave) * 16 * kTiles; base < n; base += step) {
         bf16x8 x[kTiles][3];
         for (int s = 0; s < kTiles; ++s) {
-            int64_t smp = min(base + 16 * s + m, a.n - 1);
+            int64_t smp = min(base + 16 * s + m, n - 1);
             float q[kQuery], pn[3];
             load_query(a.queries, smp, a.bd, q, pn);
             for (int c = 0; c < 3; ++c)
@@ -200,7 +202,7 @@ __global__ void __launch_bounds__(256) k_infer(InferArgs a) {
                 o = mfma32(A, h[s][c], o);
             }
             int64_t smp = base + 16 * s + m;
-            if (g == 0 && smp < a.n)
+            if (g == 0 && smp < n)
                 reinterpret_cast<float4*>(a.out)[smp] = make_float4(o[0], o[1], o[2], 0.f);
         }
     }
@@ -452,6 +454,25 @@ struct ilm_nrc {
     int64_t partial_blocks = 0;
 };
 
+namespace ilm {
+int nrc_infer_queue(ilm_nrc* r, const float* queries, const unsigned long long* n_dev, int64_t capacity, float* out,
+                    hipStream_t stream) {
+    if (capacity <= 0) return ILM_OK;
+    nrc::InferArgs a;
+    a.fwd = r->fwd;
+    a.queries = queries;
+    a.n = capacity;
+    a.n_dev = n_dev;
+    a.bd = r->bd;
+    a.out = out;
+    int64_t per_block = 4 * 16 * nrc::kTiles;
+    int64_t blocks = std::min<int64_t>((capacity + per_block - 1) / per_block, (int64_t)r->ctx->cu_count * 4);
+    hipLaunchKernelGGL(nrc::k_infer, dim3((unsigned)blocks), dim3(256), 0, stream, a);
+    ILM_HIP(hipGetLastError());
+    return ILM_OK;
+}
+}  // namespace ilm
+
 namespace {
 using namespace ilm::nrc;
 
@@ -617,17 +638,7 @@ int ilm_nrc_get_weights(ilm_nrc* r, float* w) {
 int ilm_nrc_infer(ilm_nrc* r, const float* queries, int64_t n, float* out, void* stream) {
     if (!r || (n > 0 && (!queries || !out)) || n < 0) return ilm::fail(ILM_ERR_INVALID, "bad arguments");
     if (n == 0) return ILM_OK;
-    InferArgs a;
-    a.fwd = r->fwd;
-    a.queries = queries;
-    a.n = n;
-    a.bd = r->bd;
-    a.out = out;
-    int64_t per_block = 4 * 16 * kTiles;
-    int64_t blocks = std::min<int64_t>((n + per_block - 1) / per_block, (int64_t)r->ctx->cu_count * 4);
-    hipLaunchKernelGGL(k_infer, dim3((unsigned)blocks), dim3(256), 0, ilm::pick_stream(r->ctx, stream), a);
-    ILM_HIP(hipGetLastError());
-    return ILM_OK;
+    return ilm::nrc_infer_queue(r, queries, nullptr, n, out, ilm::pick_stream(r->ctx, stream));
 }
 
 int ilm_nrc_train_step(ilm_nrc* r, const float* queries, const float* targets, int64_t n, float* loss, void* stream) {

@@ -131,9 +131,25 @@ struct PtArgs {
     ilm_hit* hits;
     ilm_ray* srays;
     float4* scontrib;        // rgb contribution, path index in w
+    float4* scontrib2;       // contribution to the cache target of a training path (rgb)
     uint8_t* socc;
-    unsigned long long* counts;  // [0], [1] ray queues, [2] shadow queue, [3] extension total, [4] shadow total
+    // [0], [1] ray queues, [2] shadow queue, [3] extension total, [4] shadow total,
+    // [5] cache queries of this batch, [6] training samples of this frame
+    unsigned long long* counts;
     double* accum;
+    // radiance cache (nrc_prefix < 0: plain path tracing)
+    int nrc_prefix;          // depth of the vertex whose outgoing radiance the cache supplies
+    int max_depth_train;     // depth limit of the training paths (prefix + suffix)
+    float train_frac;        // fraction of paths extended as training paths
+    int64_t train_cap;       // training samples kept per frame
+    float4* tbeta;           // throughput of a training path relative to its query vertex
+    float* T;                // 3 floats per path: radiance leaving the query vertex (training paths)
+    int32_t* tslot;          // training-sample slot of a path, or -1
+    float* queries;          // 16 floats per cache query (non-training paths)
+    float4* qbeta;           // (throughput rgb, path index) per cache query
+    float4* qout;            // cache predictions per query
+    float* tq;               // training queries (train_cap x 16)
+    float4* tt;              // training targets (train_cap x (rgb, weight))
 };
 
 __device__ __forceinline__ void path_pixel(const PtArgs& a, int64_t p, int64_t& pixel, uint32_t& sample) {
@@ -166,7 +182,10 @@ __global__ void k_generate(PtArgs a) {
     r.dx = d.x; r.dy = d.y; r.dz = d.z; r.tmax = FLT_MAX;
     a.rays[0][p] = r;
     a.qpath[0][p] = (int32_t)p;
-    a.beta[p] = make_float4(1.f, 1.f, 1.f, 0.f);
+    // beta.w flags a training path of the radiance cache (sample 6 of the block)
+    float train = (a.nrc_prefix >= 0 && b.v[6] < a.train_frac) ? 1.f : 0.f;
+    a.beta[p] = make_float4(1.f, 1.f, 1.f, train);
+    if (a.nrc_prefix >= 0) a.tslot[p] = -1;
     a.L[3 * p] = 0.f;
     a.L[3 * p + 1] = 0.f;
     a.L[3 * p + 2] = 0.f;
@@ -177,7 +196,6 @@ __global__ void __launch_bounds__(kPtBlock) k_shade(PtArgs a, int bounce, int cu
     const int nxt = cur ^ 1;
     const int64_t count = (int64_t)a.counts[cur];
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    const bool last = bounce + 1 >= a.max_depth;
     // iterate in whole-wave steps so that every lane of a wave reaches the
     // queue appends together
     for (int64_t base = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) & ~63ll; base < count; base += stride) {
@@ -194,7 +212,8 @@ __global__ void __launch_bounds__(kPtBlock) k_shade(PtArgs a, int bounce, int cu
         }
         int mat_id = -1, light_id = -1;
         f3 P = mk(0, 0, 0), Ng = mk(0, 0, 1), Ns = mk(0, 0, 1), wOut = mk(0, 0, 1);
-        f3 beta = mk(0, 0, 0);
+        f3 beta = mk(0, 0, 0), tbeta = mk(0, 0, 0);
+        bool train = false;
         int64_t pixel = 0;
         uint32_t sample = 0;
         if (live) {
@@ -211,11 +230,62 @@ __global__ void __launch_bounds__(kPtBlock) k_shade(PtArgs a, int bounce, int cu
             wOut = -normalize(rd);
             float4 b4 = a.beta[p];
             beta = mk(b4.x, b4.y, b4.z);
+            train = b4.w != 0.f;
+            if (train && bounce > a.nrc_prefix) {
+                float4 t4 = a.tbeta[p];
+                tbeta = mk(t4.x, t4.y, t4.z);
+            }
             path_pixel(a, p, pixel, sample);
             if (mat_id < 0) live = false;  // no material: the path ends (PathIntegrator.cpp:110-111)
         }
         Material mat;
         if (live) mat = load_material(a.mats, mat_id);
+        // radiance cache: at the query vertex a non-training path hands its
+        // remaining radiance to the cache; a training path records the query and
+        // keeps going, measuring that radiance from here on with tbeta
+        bool to_cache = false, new_train = false;
+        if (live && a.nrc_prefix >= 0 && bounce == a.nrc_prefix) {
+            to_cache = !train;
+            new_train = train;
+            if (train) tbeta = mk(1.f, 1.f, 1.f);
+        }
+        float qrec[16];
+        if (to_cache || new_train) {
+            float ct = fminf(fmaxf(wOut.z, -1.f), 1.f), cn = fminf(fmaxf(Ns.z, -1.f), 1.f);
+            qrec[0] = P.x; qrec[1] = P.y; qrec[2] = P.z;
+            qrec[3] = acosf(ct) * kInvPi;
+            qrec[4] = (atan2f(wOut.y, wOut.x) + kPi) * (0.5f * kInvPi);
+            qrec[5] = acosf(cn) * kInvPi;
+            qrec[6] = (atan2f(Ns.y, Ns.x) + kPi) * (0.5f * kInvPi);
+            qrec[7] = mat.type == 1 ? sqrtf(2.f / (mat.exponent + 2.f)) : 1.f;
+            qrec[8] = mat.kd.x; qrec[9] = mat.kd.y; qrec[10] = mat.kd.z;
+            qrec[11] = mat.type == 1 ? mat.ks.x : 0.f;
+            qrec[12] = mat.type == 1 ? mat.ks.y : 0.f;
+            qrec[13] = mat.type == 1 ? mat.ks.z : 0.f;
+            qrec[14] = 0.f; qrec[15] = 0.f;
+        }
+        {
+            int64_t qs = wave_append(&a.counts[5], to_cache);
+            if (to_cache) {
+                float4* d = reinterpret_cast<float4*>(a.queries + 16 * qs);
+                for (int k = 0; k < 4; ++k) d[k] = make_float4(qrec[4 * k], qrec[4 * k + 1], qrec[4 * k + 2], qrec[4 * k + 3]);
+                a.qbeta[qs] = make_float4(beta.x, beta.y, beta.z, __int_as_float(p));
+                live = false;
+            }
+            int64_t ts = wave_append(&a.counts[6], new_train);
+            if (new_train) {
+                if (ts < a.train_cap) {
+                    float4* d = reinterpret_cast<float4*>(a.tq + 16 * ts);
+                    for (int k = 0; k < 4; ++k) d[k] = make_float4(qrec[4 * k], qrec[4 * k + 1], qrec[4 * k + 2], qrec[4 * k + 3]);
+                    a.tslot[p] = (int32_t)ts;
+                }
+                a.T[3 * p] = 0.f;
+                a.T[3 * p + 1] = 0.f;
+                a.T[3 * p + 2] = 0.f;
+            }
+        }
+        const bool measure = train && bounce >= a.nrc_prefix && a.nrc_prefix >= 0;
+        const bool last = bounce + 1 >= (train ? a.max_depth_train : a.max_depth);
         // emission at depth 0 (no specular BxDFs), DiffuseAreaLight::Radiance (DiffuseAreaLight.cpp:47-50)
         if (live && bounce == 0 && light_id >= 0 && dot(Ns, wOut) > 0.f) {
             float4 le = a.lights[light_id];
@@ -229,7 +299,7 @@ __global__ void __launch_bounds__(kPtBlock) k_shade(PtArgs a, int bounce, int cu
         for (int l = 0; l < a.nlights; ++l) {
             bool want = false;
             ilm_ray sray;
-            f3 contrib = mk(0, 0, 0);
+            f3 contrib = mk(0, 0, 0), contrib2 = mk(0, 0, 0);
             if (live && l != light_id) {
                 Block8 lb = sample_block(a.seed, (uint64_t)pixel, sample, (uint32_t)bounce, 1u + (uint32_t)l);
                 int2 rg = a.lrange[l];
@@ -254,8 +324,10 @@ __global__ void __launch_bounds__(kPtBlock) k_shade(PtArgs a, int bounce, int cu
                 if (!is_black(Ls)) {
                     wIn = -wIn;
                     f3 f = bsdf_f(mat, to_local(gb, wOut), to_local(gb, wIn));
-                    contrib = mul(beta, mul(Ls * fmaxf(0.f, dot(wIn, Ns)), f));
-                    if (!is_black(contrib)) {
+                    f3 local = mul(Ls * fmaxf(0.f, dot(wIn, Ns)), f);
+                    contrib = mul(beta, local);
+                    if (measure) contrib2 = mul(tbeta, local);
+                    if (!is_black(contrib) || !is_black(contrib2)) {
                         want = true;
                         f3 sd = Q - P;  // VisibilityQuery::SetSegment (Visibility.cpp:50-53)
                         sray.ox = P.x; sray.oy = P.y; sray.oz = P.z; sray.tmin = 1e-4f;
@@ -267,6 +339,7 @@ __global__ void __launch_bounds__(kPtBlock) k_shade(PtArgs a, int bounce, int cu
             if (want) {
                 a.srays[slot] = sray;
                 a.scontrib[slot] = make_float4(contrib.x, contrib.y, contrib.z, __int_as_float(p));
+                if (a.nrc_prefix >= 0) a.scontrib2[slot] = make_float4(contrib2.x, contrib2.y, contrib2.z, 0.f);
             }
         }
         // BSDF sampling in the shading frame (PathIntegrator.cpp:138-177)
@@ -280,12 +353,17 @@ __global__ void __launch_bounds__(kPtBlock) k_shade(PtArgs a, int bounce, int cu
             f3 f = bsdf_sample(mat, to_local(sb, wOut), wiL, b.v[2], b.v[3], b.v[4], pdf);
             if (!is_black(f) && pdf != 0.0f) {
                 f3 wIn = to_world(sb, wiL);
-                beta = mul(beta, (f * fabsf(dot(wIn, Ns))) / pdf);
+                f3 wgt = (f * fabsf(dot(wIn, Ns))) / pdf;
+                beta = mul(beta, wgt);
+                if (measure) tbeta = mul(tbeta, wgt);
                 cont = true;
                 if (bounce >= a.rr_depth) {
                     float q = fminf(0.5f, luminance(beta));
                     if (b.v[5] > q) cont = false;
-                    else beta = beta / q;
+                    else {
+                        beta = beta / q;
+                        if (measure) tbeta = tbeta / q;
+                    }
                 }
                 if (last) cont = false;  // the last vertex's continuation ray is never used
                 if (cont) {
@@ -299,9 +377,42 @@ __global__ void __launch_bounds__(kPtBlock) k_shade(PtArgs a, int bounce, int cu
         if (cont) {
             a.rays[nxt][slot] = nray;
             a.qpath[nxt][slot] = p;
-            a.beta[p] = make_float4(beta.x, beta.y, beta.z, 0.f);
+            a.beta[p] = make_float4(beta.x, beta.y, beta.z, train ? 1.f : 0.f);
+            if (measure) a.tbeta[p] = make_float4(tbeta.x, tbeta.y, tbeta.z, 0.f);
         }
     }
+}
+
+// Cache predictions at the query vertices: L += throughput * max(prediction, 0).
+__global__ void k_nrc_resolve(PtArgs a) {
+    const int64_t count = (int64_t)a.counts[5];
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < count; k += stride) {
+        float4 b = a.qbeta[k], y = a.qout[k];
+        int64_t p = __float_as_int(b.w);
+        a.L[3 * p] += b.x * fmaxf(y.x, 0.f);
+        a.L[3 * p + 1] += b.y * fmaxf(y.y, 0.f);
+        a.L[3 * p + 2] += b.z * fmaxf(y.z, 0.f);
+    }
+}
+
+// Training targets of the batch's training paths (radiance leaving the query vertex).
+__global__ void k_nrc_collect(PtArgs a) {
+    int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.npaths) return;
+    int32_t s = a.tslot[p];
+    if (s >= 0) a.tt[s] = make_float4(a.T[3 * p], a.T[3 * p + 1], a.T[3 * p + 2], 1.f);
+}
+
+__global__ void k_nrc_reset_queries(unsigned long long* counts) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) counts[5] = 0ull;
+}
+
+// Start of a frame: no training samples yet (weights 0 mark unused slots).
+__global__ void k_nrc_clear(PtArgs a) {
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k == 0) a.counts[6] = 0ull;
+    if (k < a.train_cap) a.tt[k] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
 __global__ void k_shadow_accum(PtArgs a) {
@@ -314,6 +425,14 @@ __global__ void k_shadow_accum(PtArgs a) {
         atomicAdd(&a.L[3 * p], c.x);
         atomicAdd(&a.L[3 * p + 1], c.y);
         atomicAdd(&a.L[3 * p + 2], c.z);
+        if (a.nrc_prefix >= 0) {
+            float4 t = a.scontrib2[k];
+            if (t.x != 0.f || t.y != 0.f || t.z != 0.f) {
+                atomicAdd(&a.T[3 * p], t.x);
+                atomicAdd(&a.T[3 * p + 1], t.y);
+                atomicAdd(&a.T[3 * p + 2], t.z);
+            }
+        }
     }
 }
 
@@ -359,6 +478,8 @@ struct ilm_scene {
     int nlights = 0, nmat = 0;
     // path-state buffers (grown on demand)
     int64_t cap = 0;
+    bool cache_ready = false;  // radiance-cache buffers allocated for `cap` paths
+    int64_t train_cap = 0;
     std::vector<void*> bufs;
     ilm::PtArgs a{};
 };
@@ -369,6 +490,8 @@ void scene_free_state(ilm_scene* s) {
     for (void* p : s->bufs) ilm_free(s->ctx, p);
     s->bufs.clear();
     s->cap = 0;
+    s->cache_ready = false;
+    s->train_cap = 0;
 }
 
 int scene_alloc(ilm_scene* s, size_t bytes, void** out) {
@@ -377,12 +500,26 @@ int scene_alloc(ilm_scene* s, size_t bytes, void** out) {
     return rc;
 }
 
-int ensure_state(ilm_scene* s, int64_t cap) {
-    if (s->cap >= cap) return ILM_OK;
+int ensure_state(ilm_scene* s, int64_t cap, bool cache = false, int64_t train_cap = 0) {
+    if (s->cap >= cap && (!cache || (s->cache_ready && s->train_cap == train_cap))) return ILM_OK;
+    cap = std::max(cap, s->cap);
     scene_free_state(s);
     int nl = std::max(1, s->nlights);
     ilm::PtArgs& a = s->a;
     int rc = 0;
+    if (cache) {
+        rc |= scene_alloc(s, cap * sizeof(float4), (void**)&a.tbeta);
+        rc |= scene_alloc(s, cap * 3 * sizeof(float), (void**)&a.T);
+        rc |= scene_alloc(s, cap * sizeof(int32_t), (void**)&a.tslot);
+        rc |= scene_alloc(s, cap * 16 * sizeof(float), (void**)&a.queries);
+        rc |= scene_alloc(s, cap * sizeof(float4), (void**)&a.qbeta);
+        rc |= scene_alloc(s, cap * sizeof(float4), (void**)&a.qout);
+        rc |= scene_alloc(s, cap * nl * sizeof(float4), (void**)&a.scontrib2);
+        rc |= scene_alloc(s, std::max<int64_t>(train_cap, 1) * 16 * sizeof(float), (void**)&a.tq);
+        rc |= scene_alloc(s, std::max<int64_t>(train_cap, 1) * sizeof(float4), (void**)&a.tt);
+        s->cache_ready = true;
+        s->train_cap = train_cap;
+    }
     rc |= scene_alloc(s, cap * sizeof(float4), (void**)&a.beta);
     rc |= scene_alloc(s, cap * 3 * sizeof(float), (void**)&a.L);
     rc |= scene_alloc(s, cap * sizeof(ilm_ray), (void**)&a.rays[0]);
@@ -500,11 +637,28 @@ int ilm_scene_destroy(ilm_scene* s) {
     return ILM_OK;
 }
 
-int ilm_render(ilm_scene* s, const ilm_render_params* prm, double* accum, ilm_render_stats* stats) {
-    if (!s || !prm || !accum) return fail(ILM_ERR_INVALID, "null argument");
+}  // extern "C"
+
+namespace {
+
+int render_impl(ilm_scene* s, ilm_nrc* nrc, const ilm_render_params* prm, const ilm_nrc_render_params* np,
+                double* accum, ilm_render_stats* stats) {
+    if (!s || !prm || (!accum && !(np && !np->accumulate))) return fail(ILM_ERR_INVALID, "null argument");
     const ilm_render_params& p = *prm;
     if (p.width <= 0 || p.height <= 0 || p.spp <= 0 || p.max_depth <= 0)
         return fail(ILM_ERR_INVALID, "width, height, spp and max_depth must be positive");
+    const bool cache = nrc != nullptr;
+    int prefix = -1, max_depth_train = p.max_depth, train_steps = 0;
+    int64_t train_cap = 0;
+    float train_frac = 0.f;
+    if (cache) {
+        if (!np) return fail(ILM_ERR_INVALID, "cache rendering needs ilm_nrc_render_params");
+        prefix = np->prefix_depth > 0 ? np->prefix_depth : 2;
+        max_depth_train = prefix + (np->suffix_depth > 0 ? np->suffix_depth : 8);
+        train_frac = np->train_fraction > 0 ? np->train_fraction : 1.f / 16.f;
+        train_cap = np->train_batch > 0 ? np->train_batch : 65536;
+        train_steps = np->train_steps >= 0 ? np->train_steps : 1;
+    }
     ilm_ctx* ctx = s->ctx;
     ILM_HIP(hipSetDevice(ctx->device));
     const int64_t npix_total = (int64_t)p.width * p.height;
@@ -513,9 +667,15 @@ int ilm_render(ilm_scene* s, const ilm_render_params* prm, double* accum, ilm_re
     ppb = std::min(ppb, npix_total);
     const int64_t cap = ppb * p.spp;
     if (cap > (int64_t(1) << 31) - 1) return fail(ILM_ERR_INVALID, "batch too large");
-    int rc = ensure_state(s, cap);
+    int rc = ensure_state(s, cap, cache, train_cap);
     if (rc) return fail(rc, "path state allocation");
     ilm::PtArgs a = s->a;
+    a.nrc_prefix = prefix;
+    a.max_depth_train = max_depth_train;
+    a.train_frac = train_frac;
+    a.train_cap = train_cap;
+    const bool do_accum = !cache || np->accumulate;
+    const int max_bounces = cache ? max_depth_train : p.max_depth;
     a.shade = s->d_shade;
     a.mats = s->d_mats;
     a.ltris = s->d_ltris;
@@ -545,13 +705,15 @@ int ilm_render(ilm_scene* s, const ilm_render_params* prm, double* accum, ilm_re
     if (rc) return rc;
     ILM_HIP(hipMemsetAsync(a.counts, 0, 8 * sizeof(unsigned long long), st));
     const int grid = ctx->cu_count * 8;
+    if (cache)
+        hipLaunchKernelGGL(ilm::k_nrc_clear, dim3((unsigned)((train_cap + 255) / 256)), dim3(256), 0, st, a);
     for (int64_t pix0 = 0; pix0 < npix_total; pix0 += ppb) {
         int64_t npix = std::min(ppb, npix_total - pix0);
         a.pix0 = pix0;
         a.npaths = npix * p.spp;
         hipLaunchKernelGGL(ilm::k_generate, dim3((unsigned)((a.npaths + 255) / 256)), dim3(256), 0, st, a);
         int cur = 0;
-        for (int b = 0; b < p.max_depth; ++b) {
+        for (int b = 0; b < max_bounces; ++b) {
             rc = ilm::trace_queue(s->mesh, a.rays[cur], a.counts + cur, cap, a.hits, nullptr, st);
             if (rc) return rc;
             hipLaunchKernelGGL(ilm::k_shade, dim3(grid), dim3(ilm::kPtBlock), 0, st, a, b, cur);
@@ -563,8 +725,21 @@ int ilm_render(ilm_scene* s, const ilm_render_params* prm, double* accum, ilm_re
             hipLaunchKernelGGL(ilm::k_end_bounce, dim3(1), dim3(1), 0, st, a.counts, cur);
             cur ^= 1;
         }
-        hipLaunchKernelGGL(ilm::k_accumulate, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, st, a, npix);
+        if (cache) {
+            // batched cache inference at the query vertices, then training targets
+            rc = ilm::nrc_infer_queue(nrc, a.queries, a.counts + 5, cap, reinterpret_cast<float*>(a.qout), st);
+            if (rc) return rc;
+            hipLaunchKernelGGL(ilm::k_nrc_resolve, dim3(grid), dim3(256), 0, st, a);
+            hipLaunchKernelGGL(ilm::k_nrc_collect, dim3((unsigned)((a.npaths + 255) / 256)), dim3(256), 0, st, a);
+            hipLaunchKernelGGL(ilm::k_nrc_reset_queries, dim3(1), dim3(1), 0, st, a.counts);
+        }
+        if (do_accum)
+            hipLaunchKernelGGL(ilm::k_accumulate, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, st, a, npix);
         ILM_HIP(hipGetLastError());
+    }
+    for (int k = 0; k < train_steps; ++k) {
+        rc = ilm_nrc_train_step(nrc, a.tq, reinterpret_cast<const float*>(a.tt), train_cap, nullptr, st);
+        if (rc) return rc;
     }
     rc = ilm_timer_mark(ctx, &ev1);
     if (rc) return rc;
@@ -580,6 +755,20 @@ int ilm_render(ilm_scene* s, const ilm_render_params* prm, double* accum, ilm_re
         stats->ms = ms;
     }
     return ilm_sync(ctx);
+}
+
+}  // namespace
+
+extern "C" {
+
+int ilm_render(ilm_scene* s, const ilm_render_params* prm, double* accum, ilm_render_stats* stats) {
+    return render_impl(s, nullptr, prm, nullptr, accum, stats);
+}
+
+int ilm_render_nrc(ilm_scene* s, ilm_nrc* nrc, const ilm_render_params* prm, const ilm_nrc_render_params* np,
+                   double* accum, ilm_render_stats* stats) {
+    if (!nrc) return fail(ILM_ERR_INVALID, "null cache");
+    return render_impl(s, nrc, prm, np, accum, stats);
 }
 
 }  // extern "C"

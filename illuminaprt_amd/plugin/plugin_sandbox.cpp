@@ -1,0 +1,274 @@
+// A small host program that drives the GpuAccel plug-in the way the
+// reference's sandbox does: it loads the plug-in library by path and
+// resolves GetPlugInSingleton (System/PlugInManager.cpp:40-96 with
+// System/LibrarySymbols.h:15-23), registers its factories with an
+// EngineKernel, creates a shape and an integrator through the factory
+// managers from argument strings (Scene/EnvironmentLoader.cpp:567-647), and
+// runs them.
+//
+//   plugin_sandbox <plugin.so> host
+//       host-side checks (argument parsing, factory errors, registration);
+//       needs no GPU. Prints "host ok" on success.
+//   plugin_sandbox <plugin.so> trace <in.bin> <out.bin> "<shape args>"
+//       in:  int64 nv, nt, n, nper; float pos[3nv]; int32 idx[3nt]; float rays[8n]
+//       out: batched closest hits (n x {t, prim, u, v}), batched occlusion
+//            (n bytes), per-ray closest hits and occlusion for the first
+//            nper rays (IShape::Intersects one ray at a time).
+//   plugin_sandbox <plugin.so> render <in.bin> <out.bin> "<integrator args>"
+//       in:  int64 nv, nt, nmat, nlight; int32 width, height, spp, seed;
+//            float camera[11] (pos, target, up, fov, aspect); float pos[3nv];
+//            int32 idx[3nt]; float nrm[3nv]; int32 tri_mat[nt], tri_light[nt];
+//            float mats[8 nmat], lights[4 nlight]
+//       out: float image[width * height * 3] (mean radiance per pixel)
+//
+// Exit status 0 on success; a plug-in exception prints its message and exits 3.
+#include <dlfcn.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "ilm_plugin.hpp"
+
+using namespace IlluminaGpu;
+
+typedef IPlugIn* (*GetPlugInFn)(void);
+typedef void (*DisposePlugInFn)(IPlugIn*);
+
+template <class T>
+static void read_into(FILE* f, std::vector<T>& v, size_t n) {
+    v.resize(n);
+    if (n && fread(v.data(), sizeof(T), n, f) != n) throw new Exception("short input file");
+}
+
+template <class T>
+static T read_one(FILE* f) {
+    T x;
+    if (fread(&x, sizeof x, 1, f) != 1) throw new Exception("short input file");
+    return x;
+}
+
+static bool expect_throw(void (*fn)(EngineKernel&), EngineKernel& k, const char* what) {
+    try {
+        fn(k);
+    } catch (Exception* e) {
+        delete e;
+        return true;
+    }
+    fprintf(stderr, "expected an exception: %s\n", what);
+    return false;
+}
+
+static int run_host(IPlugIn* plugin) {
+    bool ok = true;
+    // Argument parsing as the loader feeds it.
+    ArgumentMap a("MaximumTreeDepth = 20; MaximumLeafObjects=4; Builder = Reference; Epsilon = 0.001;");
+    int depth = 0, leaf = 0;
+    float eps = 0;
+    std::string builder;
+    ok &= a.GetArgument("MaximumTreeDepth", depth) && depth == 20;
+    ok &= a.GetArgument("MaximumLeafObjects", leaf) && leaf == 4;
+    ok &= a.GetArgument("Builder", builder) && builder == "Reference";
+    ok &= a.GetArgument("Epsilon", eps) && eps == 0.001f;
+    ok &= !a.GetArgument("Missing", depth) && depth == 20;
+    try {
+        ArgumentMap bad("A = 1; B");
+        ok = false;
+        fprintf(stderr, "odd token count accepted\n");
+    } catch (Exception* e) {
+        delete e;
+    }
+
+    EngineKernel kernel;
+    plugin->Register(&kernel);
+    ok &= kernel.GetShapeManager().QueryFactory("GpuBVHMesh");
+    ok &= kernel.GetIntegratorManager().QueryFactory("GpuPathTracing");
+    ok &= expect_throw([](EngineKernel& k) { k.GetShapeManager().RegisterFactory("GpuBVHMesh", nullptr); }, kernel,
+                       "duplicate factory");
+    ok &= expect_throw(
+        [](EngineKernel& k) {
+            ArgumentMap m;
+            k.GetShapeManager().CreateInstance("NoSuchFactory", "x", m);
+        },
+        kernel, "unknown factory");
+    ok &= expect_throw([](EngineKernel& k) { k.GetShapeManager().RequestInstance("nothing"); }, kernel,
+                       "unknown instance");
+    int ndev = 0;
+    ilm_device_count(&ndev);
+    if (ndev == 0) {
+        // No device: creating a GPU shape must fail loudly, never fall back.
+        ok &= expect_throw(
+            [](EngineKernel& k) {
+                ArgumentMap m("Builder = SAH");
+                k.GetShapeManager().CreateInstance("GpuBVHMesh", "mesh", m);
+            },
+            kernel, "shape without a device");
+    }
+    ok &= expect_throw(
+        [](EngineKernel& k) {
+            ArgumentMap m("ShadowRays = 4");
+            k.GetIntegratorManager().CreateInstance("GpuPathTracing", "pt", m);
+        },
+        kernel, "unsupported ShadowRays");
+    plugin->Unregister(&kernel);
+    ok &= !kernel.GetShapeManager().QueryFactory("GpuBVHMesh");
+    printf(ok ? "host ok\n" : "host FAILED\n");
+    return ok ? 0 : 1;
+}
+
+static int run_trace(IPlugIn* plugin, const char* in, const char* out, const char* args) {
+    FILE* f = fopen(in, "rb");
+    if (!f) throw new Exception(std::string("cannot open ") + in);
+    const int64_t nv = read_one<int64_t>(f), nt = read_one<int64_t>(f), n = read_one<int64_t>(f),
+                  nper = read_one<int64_t>(f);
+    std::vector<float> pos, rays;
+    std::vector<int32_t> idx;
+    read_into(f, pos, (size_t)nv * 3);
+    read_into(f, idx, (size_t)nt * 3);
+    read_into(f, rays, (size_t)n * 8);
+    fclose(f);
+    if (nper > n) throw new Exception("nper > n");
+
+    EngineKernel kernel;
+    plugin->Register(&kernel);
+    plugin->Initialise();
+    ArgumentMap am(args);
+    GpuBVHMesh* mesh = dynamic_cast<GpuBVHMesh*>(kernel.GetShapeManager().CreateInstance("GpuBVHMesh", "mesh", am));
+    if (!mesh) throw new Exception("factory returned another shape type");
+    for (int64_t v = 0; v < nv; ++v) mesh->AddVertex(Vector3(pos[3 * v], pos[3 * v + 1], pos[3 * v + 2]));
+    for (int64_t t = 0; t < nt; ++t) mesh->AddIndexedTriangle(idx[3 * t], idx[3 * t + 1], idx[3 * t + 2]);
+    mesh->Compile();
+
+    std::vector<Ray> rs(n);
+    for (int64_t i = 0; i < n; ++i) {
+        const float* r = &rays[8 * i];
+        rs[i].Origin = Vector3(r[0], r[1], r[2]);
+        rs[i].Min = r[3];
+        rs[i].Direction = Vector3(r[4], r[5], r[6]);
+        rs[i].Max = r[7];
+    }
+    std::vector<DifferentialSurface> hits;
+    std::vector<uint8_t> occ;
+    mesh->IntersectBatch(rs, hits);
+    mesh->OccludedBatch(rs, occ);
+
+    FILE* o = fopen(out, "wb");
+    if (!o) throw new Exception(std::string("cannot open ") + out);
+    auto put_hit = [&](const DifferentialSurface& s) {
+        float rec[4] = {s.Distance, 0.0f, s.Beta, s.Gamma};
+        int32_t prim = s.TriangleIndex;
+        memcpy(&rec[1], &prim, 4);
+        fwrite(rec, sizeof rec, 1, o);
+    };
+    for (int64_t i = 0; i < n; ++i) put_hit(hits[i]);
+    fwrite(occ.data(), 1, (size_t)n, o);
+    for (int64_t i = 0; i < nper; ++i) {
+        DifferentialSurface s;
+        if (!mesh->Intersects(rs[i], s)) s = DifferentialSurface();
+        put_hit(s);
+    }
+    for (int64_t i = 0; i < nper; ++i) {
+        uint8_t b = mesh->Intersects(rs[i]) ? 1 : 0;
+        fwrite(&b, 1, 1, o);
+    }
+    fclose(o);
+    kernel.GetShapeManager().ReleaseInstance("mesh");
+    plugin->Shutdown();
+    plugin->Unregister(&kernel);
+    printf("trace ok: %lld rays, %lld per-ray\n", (long long)n, (long long)nper);
+    return 0;
+}
+
+static int run_render(IPlugIn* plugin, const char* in, const char* out, const char* args) {
+    FILE* f = fopen(in, "rb");
+    if (!f) throw new Exception(std::string("cannot open ") + in);
+    const int64_t nv = read_one<int64_t>(f), nt = read_one<int64_t>(f), nmat = read_one<int64_t>(f),
+                  nlight = read_one<int64_t>(f);
+    const int32_t W = read_one<int32_t>(f), H = read_one<int32_t>(f), spp = read_one<int32_t>(f),
+                  seed = read_one<int32_t>(f);
+    std::vector<float> cam, pos, nrm, mats, lights;
+    std::vector<int32_t> idx, tmat, tlight;
+    read_into(f, cam, 11);
+    read_into(f, pos, (size_t)nv * 3);
+    read_into(f, idx, (size_t)nt * 3);
+    read_into(f, nrm, (size_t)nv * 3);
+    read_into(f, tmat, (size_t)nt);
+    read_into(f, tlight, (size_t)nt);
+    read_into(f, mats, (size_t)nmat * 8);
+    read_into(f, lights, (size_t)nlight * 4);
+    fclose(f);
+
+    EngineKernel kernel;
+    plugin->Register(&kernel);
+    plugin->Initialise();
+    ArgumentMap shapeArgs("Builder = SAH");
+    GpuBVHMesh* mesh =
+        dynamic_cast<GpuBVHMesh*>(kernel.GetShapeManager().CreateInstance("GpuBVHMesh", "scene", shapeArgs));
+    for (int64_t v = 0; v < nv; ++v) mesh->AddVertex(Vector3(pos[3 * v], pos[3 * v + 1], pos[3 * v + 2]));
+    for (int64_t t = 0; t < nt; ++t) mesh->AddIndexedTriangle(idx[3 * t], idx[3 * t + 1], idx[3 * t + 2]);
+    mesh->Compile();
+    ArgumentMap am(args);
+    GpuPathTracing* pt =
+        dynamic_cast<GpuPathTracing*>(kernel.GetIntegratorManager().CreateInstance("GpuPathTracing", "pt", am));
+    if (!mesh || !pt) throw new Exception("factory returned another type");
+    pt->Initialise(mesh, nrm, tmat, tlight, mats, lights);
+    CameraArgs c;
+    c.Position = Vector3(cam[0], cam[1], cam[2]);
+    c.Target = Vector3(cam[3], cam[4], cam[5]);
+    c.Up = Vector3(cam[6], cam[7], cam[8]);
+    c.Fov = cam[9];
+    c.Aspect = cam[10];
+    std::vector<float> image;
+    pt->RenderRegion(c, W, H, spp, (uint32_t)seed, image);
+    FILE* o = fopen(out, "wb");
+    if (!o) throw new Exception(std::string("cannot open ") + out);
+    fwrite(image.data(), sizeof(float), image.size(), o);
+    fclose(o);
+    // The integrator refers to the mesh's device arrays: release it first.
+    kernel.GetIntegratorManager().ReleaseInstance("pt");
+    kernel.GetShapeManager().ReleaseInstance("scene");
+    plugin->Shutdown();
+    plugin->Unregister(&kernel);
+    printf("render ok: %dx%d, %d spp\n", W, H, spp);
+    return 0;
+}
+
+int main(int argc, char** argv) {
+    if (argc < 3) {
+        fprintf(stderr, "usage: %s <plugin.so> host | trace|render <in> <out> <args>\n", argv[0]);
+        return 2;
+    }
+    void* lib = dlopen(argv[1], RTLD_NOW | RTLD_LOCAL);
+    if (!lib) {
+        fprintf(stderr, "dlopen: %s\n", dlerror());
+        return 2;
+    }
+    GetPlugInFn get = (GetPlugInFn)dlsym(lib, "GetPlugInSingleton");
+    DisposePlugInFn dispose = (DisposePlugInFn)dlsym(lib, "DisposePlugInSingleton");
+    if (!get || !dispose) {
+        fprintf(stderr, "plug-in symbols missing\n");
+        return 2;
+    }
+    IPlugIn* plugin = get();
+    const std::string mode = argv[2];
+    int rc = 2;
+    try {
+        if (mode == "host")
+            rc = run_host(plugin);
+        else if (mode == "trace" && argc == 6)
+            rc = run_trace(plugin, argv[3], argv[4], argv[5]);
+        else if (mode == "render" && argc == 6)
+            rc = run_render(plugin, argv[3], argv[4], argv[5]);
+        else
+            fprintf(stderr, "bad mode or arguments\n");
+    } catch (Exception* e) {
+        fprintf(stderr, "exception: %s\n", e->GetMessage().c_str());
+        delete e;
+        rc = 3;
+    }
+    dispose(plugin);
+    dlclose(lib);
+    return rc;
+}

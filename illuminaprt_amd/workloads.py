@@ -168,3 +168,64 @@ def rel_rmse(img, ref):
     """sqrt(mean over pixels and channels of (I - R)^2 / (R^2 + 0.01))."""
     img, ref = np.asarray(img, np.float64), np.asarray(ref, np.float64)
     return float(np.sqrt(np.mean((img - ref) ** 2 / (ref ** 2 + 0.01))))
+
+
+# ---------------------------------------------------------------- radiance-cache workload
+# The displaced level-8 icosphere of the intersection workload (1,310,720
+# triangles) inside a closed 12-triangle room (x, z in [-3, 3], y in
+# [-1.2, 3]), lit by three emitter panels hanging from the ceiling: thin
+# boxes (12 triangles) whose underside (2 triangles) emits. A panel's sides
+# shadow the ceiling next to its emitting face, so no surface sees an
+# emitter at grazing angles. Sphere triangles are glossy (Lambertian + Phong
+# lobe, exponent 60) with probability 0.3 (seeded per triangle), else
+# Lambertian. Camera inside the room, 16:9, fov 60 deg.
+CACHE_MATS = np.array([
+    [0.70, 0.70, 0.70, 0, 0, 0, 1, 0],        # 0 room white
+    [0.60, 0.20, 0.15, 0, 0, 0, 1, 0],        # 1 left wall
+    [0.15, 0.25, 0.60, 0, 0, 0, 1, 0],        # 2 right wall
+    [0.60, 0.55, 0.50, 0, 0, 0, 1, 0],        # 3 sphere, diffuse
+    [0.30, 0.28, 0.25, 0.45, 0.45, 0.45, 60, 1],  # 4 sphere, glossy
+    [0.0, 0.0, 0.0, 0, 0, 0, 1, 0],           # 5 panel body / emitter surface
+], np.float32)
+CACHE_LIGHTS = np.array([[14, 12, 9, 0], [8, 10, 15, 0], [12, 12, 12, 0]], np.float32)
+CACHE_PANELS = [((-1.6, 2.9, -1.0), (-0.8, 3.0, -0.2)), ((0.6, 2.9, 0.4), (1.4, 3.0, 1.2)),
+                ((-0.3, 2.9, 1.8), (0.3, 3.0, 2.4))]
+
+
+def cache_scene(level=8, glossy_fraction=0.3, seed=21):
+    """Returns pos, nrm, idx, tri_mat, tri_light, mats, lights."""
+    from . import vertex_normals
+    sp, si = icosphere(level, 0.1, 4, 7)
+    sn = vertex_normals(sp, si)
+    rng = np.random.default_rng(seed)
+    smat = np.where(rng.random(len(si)) < glossy_fraction, 4, 3).astype(np.int32)
+    quads = []
+    lo, hi = np.array([-3.0, -1.2, -3.0]), np.array([3.0, 3.0, 3.0])
+    c = (lo + hi) / 2
+    for p, n in _box_faces(lo, hi):
+        n = -n                                     # room faces look inward
+        p = p[[0, 3, 2, 1]]
+        m = 1 if n[0] > 0.5 else (2 if n[0] < -0.5 else 0)
+        quads.append((p, n, m, -1))
+    for li, (plo, phi) in enumerate(CACHE_PANELS):
+        for p, n in _box_faces(plo, phi):
+            emit = n[1] < -0.5                     # the underside emits
+            quads.append((p, n, 5, li if emit else -1))
+    pos, nrm, idx, tm, tl = [sp], [sn], [si], [smat], [np.full(len(si), -1, np.int32)]
+    base = len(sp)
+    for p, n, m, l in quads:
+        pos.append(np.array(p, np.float32))
+        nrm.append(np.tile(np.array(n, np.float32), (4, 1)))
+        idx.append(np.array([[base, base + 1, base + 2], [base, base + 2, base + 3]], np.int32))
+        tm.append(np.array([m, m], np.int32))
+        tl.append(np.array([l, l], np.int32))
+        base += 4
+    return (np.concatenate(pos).astype(np.float32), np.concatenate(nrm).astype(np.float32),
+            np.concatenate(idx), np.concatenate(tm), np.concatenate(tl), CACHE_MATS.copy(), CACHE_LIGHTS.copy())
+
+
+CACHE_BOUNDS = ((-3.0, -1.2, -3.0), (3.0, 3.0, 3.0))
+
+
+def cache_camera(aspect=16 / 9):
+    return Camera.make((0.0, 0.6, -2.8), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 60.0, aspect)

@@ -314,6 +314,26 @@ int ilm_nrc_infer(ilm_nrc* nrc, const float* queries, int64_t n, float* out, voi
 int ilm_nrc_train_step(ilm_nrc* nrc, const float* queries, const float* targets, int64_t n,
                        float* loss, void* stream);
 
+typedef struct {
+    int32_t prefix_depth;   /* depth of the vertex answered by the cache (default 2, >= 1) */
+    int32_t suffix_depth;   /* further bounces of training paths (default 8) */
+    float train_fraction;   /* fraction of paths extended as training paths (default 1/16) */
+    int32_t train_batch;    /* training samples kept per frame (default 65536) */
+    int32_t train_steps;    /* Adam steps per frame (0 = no training, negative = 1) */
+    int32_t accumulate;     /* add this frame's radiance to accum */
+} ilm_nrc_render_params;
+
+/* Renders like ilm_render, but paths that are not training paths stop at
+ * prefix_depth and take their remaining radiance from the cache (batched
+ * inference at the end of each wavefront batch); training paths (a
+ * counter-based fraction) are traced prefix_depth + suffix_depth deep, and the
+ * radiance they measure leaving their query vertex trains the cache
+ * (train_steps Adam steps on train_batch samples after the frame). The
+ * image adds the training paths' own estimates and the cache's answers.
+ * Synchronous; accum may be NULL when accumulate is 0. */
+int ilm_render_nrc(ilm_scene* scene, ilm_nrc* nrc, const ilm_render_params* params,
+                   const ilm_nrc_render_params* nrc_params, double* accum, ilm_render_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif
