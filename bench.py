@@ -116,6 +116,145 @@ def path_tracing_room(ctx, ilm, workloads, args):
     return out
 
 
+MLP_FLOP_PER_QUERY = 2 * (96 * 64 + 4 * 64 * 64 + 64 * 16)  # padded network as executed
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def _ref_noise2(r1, r2):
+    """Squared relRMSE of the mean of two independent renders R = (R1+R2)/2
+    against the truth, estimated from their difference:
+    E[(R1-R2)^2] = 4 Var(R). Subtracting it from a measured squared relRMSE
+    removes the reference's own noise from the figure."""
+    ref = 0.5 * (r1 + r2)
+    return float(np.mean((r1 - r2) ** 2 / (ref ** 2 + 0.01)) / 4.0), ref
+
+
+def _corrected(img, ref, noise2):
+    e2 = float(np.mean((img - ref) ** 2 / (ref ** 2 + 0.01)))
+    return float(np.sqrt(max(e2 - noise2, 0.0))), float(np.sqrt(e2))
+
+
+def cache_convergence(ctx, ilm, workloads, args):
+    """Time to relRMSE 0.05 on the radiance-cache workload (displaced level-8
+    sphere, 30% glossy, in a closed room with three panel lights), plain
+    wavefront PT against the neural-cache renderer, both timed with events
+    on the render stream from a cold start (the cache starts untrained).
+    The reference is the mean of two independent plain renders; its own
+    noise is estimated from their difference and removed (_ref_noise2)."""
+    W, H = args.cache_width, args.cache_height
+    depth, prefix = 10, 2
+    pos, nrm, idx, tm, tl, mats, lights = workloads.cache_scene()
+    mesh = ilm.Mesh(ctx, pos, idx)
+    mesh.build()
+    sc = ilm.Scene(ctx, mesh, nrm, tm, tl, mats, lights)
+    cam = workloads.cache_camera(W / H)
+    n3 = W * H * 3
+    buf = ctx.buffer(n3 * 8)
+
+    def plain(spp, seed, spp_offset=0):
+        buf.zero()
+        done, ms = 0, 0.0
+        while done < spp:
+            k = min(args.cache_ref_chunk, spp - done)
+            ms += sc.render(cam, W, H, k, buf, max_depth=depth, seed=seed, spp_offset=spp_offset + done)["ms"]
+            done += k
+        return buf.download(np.float64, n3).reshape(H, W, 3) / spp, ms
+
+    t0 = time.perf_counter()
+    r1, ms_ref = plain(args.cache_ref_spp, seed=90001)
+    r2, _ = plain(args.cache_ref_spp, seed=90002)
+    noise2, ref = _ref_noise2(r1, r2)
+    del r1, r2
+    log(f"cache reference: 2 x {args.cache_ref_spp} spp in {time.perf_counter() - t0:.1f} s, "
+        f"ref noise relRMSE {np.sqrt(noise2):.4f}, {ms_ref / args.cache_ref_spp:.2f} ms/spp")
+
+    target = args.cache_target
+    # Plain PT: progressive 1-spp frames, error checked at doubling sample counts.
+    buf.zero()
+    plain_ms, spp, trace = 0.0, 0, []
+    check = 1
+    plain_hit = None
+    while spp < args.cache_max_spp:
+        st = sc.render(cam, W, H, 1, buf, max_depth=depth, seed=7001, spp_offset=spp)
+        plain_ms += st["ms"]
+        spp += 1
+        if spp == check:
+            e, raw = _corrected(buf.download(np.float64, n3).reshape(H, W, 3) / spp, ref, noise2)
+            trace.append((spp, round(plain_ms, 2), round(e, 5)))
+            if e <= target:
+                plain_hit = (spp, plain_ms)
+                break
+            check = spp * 2 if spp < 8 else spp + max(1, spp // 4)
+    # 1/sqrt(spp) fit of the last point: spp needed for the target.
+    s_last, ms_last, e_last = trace[-1]
+    spp_fit = s_last * (e_last / target) ** 2
+    plain_out = {"ms_per_spp": round(plain_ms / spp, 3), "spp_to_target": plain_hit[0] if plain_hit else None,
+                 "ms_to_target": round(plain_hit[1], 1) if plain_hit else None,
+                 "spp_fit": round(spp_fit, 1), "ms_fit": round(spp_fit * plain_ms / spp, 1), "trace": trace}
+    log(f"cache plain: {plain_out}")
+
+    # Neural cache: 1-spp frames with one training step each; the image is the
+    # running mean of frames from `skip` on (earlier frames see a cold cache).
+    net = ilm.Nrc(ctx, *workloads.CACHE_BOUNDS, lr=args.cache_lr, seed=5)
+    buf.zero()
+    nrc_ms, frames, acc_frames, ntrace = 0.0, 0, 0, []
+    nrc_hit = None
+    skip = args.cache_skip
+    while frames < args.cache_max_frames:
+        st = sc.render_nrc(net, cam, W, H, 1, buf if frames >= skip else None, seed=20001 + frames,
+                           prefix_depth=prefix, suffix_depth=depth - prefix, train_fraction=1 / 16,
+                           train_batch=65536, train_steps=1)
+        nrc_ms += st["ms"]
+        frames += 1
+        if frames > skip:
+            acc_frames += 1
+            if acc_frames in (1, 2, 4) or acc_frames % 4 == 0:
+                e, raw = _corrected(buf.download(np.float64, n3).reshape(H, W, 3) / acc_frames, ref, noise2)
+                ntrace.append((frames, round(nrc_ms, 2), round(e, 5)))
+                if e <= target:
+                    nrc_hit = (frames, nrc_ms)
+                    break
+    nrc_out = {"ms_per_frame": round(nrc_ms / frames, 3), "frames_to_target": nrc_hit[0] if nrc_hit else None,
+               "ms_to_target": round(nrc_hit[1], 1) if nrc_hit else None, "skip_frames": skip,
+               "trace": ntrace}
+    log(f"cache nrc: {nrc_out}")
+
+    # MLP inference throughput on one frame's worth of queries.
+    nq = W * H
+    rng = np.random.default_rng(3)
+    q = np.zeros((nq, 16), np.float32)
+    lo, hi = np.array(workloads.CACHE_BOUNDS[0]), np.array(workloads.CACHE_BOUNDS[1])
+    q[:, :3] = lo + (hi - lo) * rng.random((nq, 3))
+    q[:, 3:7] = rng.random((nq, 4))
+    q[:, 7:14] = rng.random((nq, 7))
+    qd = ctx.buffer(q.nbytes)
+    qd.upload(q)
+    od = ctx.buffer(nq * 16)
+    for _ in range(2):
+        net.infer(qd, nq, od)
+    _, per = time_launches(ctx, lambda: net.infer(qd, nq, od), 10)
+    inf_ms = float(np.mean(per))
+    tflops = nq * MLP_FLOP_PER_QUERY / (inf_ms * 1e-3) / 1e12
+    mlp = {"queries": nq, "kernel_ms": round(inf_ms, 4), "tflops": round(tflops, 1),
+           "flop_per_query": MLP_FLOP_PER_QUERY, "roofline_frac": round(tflops / MFMA_BF16_PEAK_TFLOPS, 4),
+           "peak_tflops": MFMA_BF16_PEAK_TFLOPS, "kernel": "ilm::k_infer"}
+    log(f"cache mlp inference: {mlp}")
+    for b in (qd, od, buf):
+        b.free()
+    net.close()
+    sc.close()
+    mesh.close()
+    speedup = None
+    p_ms = plain_out["ms_to_target"] or plain_out["ms_fit"]
+    if nrc_out["ms_to_target"] and p_ms:
+        speedup = round(p_ms / nrc_out["ms_to_target"], 3)
+    return {"workload": f"radiance-cache scene {len(idx)} tris, {W}x{H}, depth {depth} "
+                        f"(cache after {prefix} bounces)",
+            "target_rel_rmse": target, "reference_spp": 2 * args.cache_ref_spp,
+            "reference_noise_rel_rmse": round(float(np.sqrt(noise2)), 5),
+            "plain_pt": plain_out, "neural_cache": nrc_out, "speedup_time_to_target": speedup, "mlp": mlp}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -127,6 +266,17 @@ def main():
                     help="time one ray set only (profiling runs)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run the plain copy kernel over MIB mebibytes (bandwidth / counter calibration)")
+    ap.add_argument("--no-cache", action="store_true", help="skip the time-to-relRMSE leg")
+    ap.add_argument("--cache-only", action="store_true", help="run only the time-to-relRMSE leg")
+    ap.add_argument("--cache-width", type=int, default=1920)
+    ap.add_argument("--cache-height", type=int, default=1080)
+    ap.add_argument("--cache-ref-spp", type=int, default=512, help="spp of each of the two reference renders")
+    ap.add_argument("--cache-ref-chunk", type=int, default=64)
+    ap.add_argument("--cache-target", type=float, default=0.05)
+    ap.add_argument("--cache-max-spp", type=int, default=1024)
+    ap.add_argument("--cache-max-frames", type=int, default=400)
+    ap.add_argument("--cache-skip", type=int, default=16, help="neural-cache frames rendered before accumulating")
+    ap.add_argument("--cache-lr", type=float, default=1e-2)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -146,6 +296,10 @@ def main():
         _, per = time_launches(ctx, lambda: ctx.copy(dst, src, nbytes), args.steps)
         ms = float(np.mean(per))
         print(json.dumps({"copy_bytes": nbytes, "kernel_ms": ms, "gbs_read_plus_write": 2 * nbytes / ms / 1e6}))
+        return
+    if args.cache_only:
+        print(json.dumps({"time_to_rel_rmse": cache_convergence(ctx, ilm, workloads, args)}), flush=True)
+        ctx.close()
         return
     t0 = time.perf_counter()
     pos, idx = workloads.intersection_scene()
@@ -200,6 +354,7 @@ def main():
         cpu = cpu_baseline(pos, idx, rays_host, args.cpu_budget)
         log(f"cpu baseline: {cpu}")
     room = None if args.only else path_tracing_room(ctx, ilm, workloads, args)
+    cache = None if (args.only or args.no_cache) else cache_convergence(ctx, ilm, workloads, args)
     out = {
         "metric": METRIC,
         "value": round(inc["wall_mrays_s"], 3),
@@ -236,6 +391,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "path_tracing_room": room,
+        "time_to_rel_rmse": cache,
     }
     print(json.dumps(out), flush=True)
     for b in (hits, occ, visits):

@@ -19,6 +19,8 @@ _lib = None
 
 ILM_BUILDER_SAH = 0
 ILM_BUILDER_REFERENCE = 1
+ILM_BUILDER_KDTREE = 2
+ILM_BUILDER_KDTREE_EX = 3
 ILM_SPAWN_SHADOW = 0
 ILM_SPAWN_COSINE = 1
 RAY_BYTES = 32
@@ -282,7 +284,7 @@ class Mesh:
     def export(self):
         info = self.info()
         nodes = np.zeros(info["node_bytes"], np.uint8)
-        order = np.zeros(self.ntri, np.int32)
+        order = np.zeros(info["tri_bytes"] // 48, np.int32)
         check(load().ilm_bvh_export(self.h, _ptr(nodes), _ptr(order)))
         return nodes, order
 

@@ -236,6 +236,14 @@ int ilm_bvh_build(ilm_mesh* m, const ilm_bvh_params* params) {
                                      p.max_depth > 0 ? p.max_depth : 35);
         } else if (p.builder == ILM_BUILDER_SAH) {
             b = ilm::build_sah(m->pos.data(), m->idx.data(), m->nt, p.max_leaf > 0 ? p.max_leaf : 4, p.threads);
+        } else if (p.builder == ILM_BUILDER_KDTREE || p.builder == ILM_BUILDER_KDTREE_EX) {
+            const int leaf = p.max_leaf > 0 ? p.max_leaf : 10;
+            // KDTreeMeshEx's constructors take the depth limit from the leaf size
+            // (KDTreeMeshEx.cpp:36-44).
+            const int depth = p.builder == ILM_BUILDER_KDTREE_EX ? leaf : (p.max_depth > 0 ? p.max_depth : 35);
+            if (depth > 40) return fail(ILM_ERR_INVALID, "kd-tree depth limit above 40");
+            b = ilm::build_kdtree(m->pos.data(), m->idx.data(), m->nt, leaf, depth,
+                                  p.builder == ILM_BUILDER_KDTREE_EX);
         } else {
             return fail(ILM_ERR_INVALID, "unknown builder");
         }

@@ -1,5 +1,6 @@
-// Host BVH construction: binned SAH and the reference topology, plus the
-// collapse to wide nodes that the GPU traverses.
+// Host BVH construction: binned SAH, the reference BVHMesh topology and the
+// reference kd-tree topologies, plus the collapse to wide nodes that the GPU
+// traverses.
 #include "bvh_build.hpp"
 
 #include <algorithm>
@@ -13,6 +14,11 @@
 #include <thread>
 
 #include "geom.hpp"
+
+// The reference topologies depend on float comparisons of box radii and split
+// planes; keep every product and sum rounded separately, as the reference's
+// scalar code does.
+#pragma clang fp contract(off)
 
 namespace ilm {
 namespace {
@@ -162,6 +168,52 @@ struct RefBox {
     }
 };
 
+inline int trunc_x86(float f) {
+    if (!(f > -2147483649.0f && f < 2147483648.0f)) return INT_MIN;
+    return (int)f;
+}
+
+// 128-bin count heuristic over the node box (TreeMesh.cpp:159-212), used by
+// BVHMesh and KDTreeMesh. The reference's O(bins^2) loops sum the same
+// integers as these prefix sums.
+float count_plane(const std::vector<Prim>& prims, const int32_t* ids, int32_t n, const RefBox& box, int axis) {
+    const int Bins = 128;
+    int lo_bins[Bins] = {0}, hi_bins[Bins] = {0};
+    float extent = box.mx[axis] - box.mn[axis], start = box.mn[axis];
+    for (int32_t i = 0; i < n; ++i) {
+        const Prim& p = prims[ids[i]];
+        int l = trunc_x86(Bins * ((p.lo[axis] - start) / extent));
+        int r = trunc_x86(Bins * ((p.hi[axis] - start) / extent));
+        if (l >= 0 && l < Bins) lo_bins[l]++;
+        if (r >= 0 && r < Bins) hi_bins[r]++;
+    }
+    int suffix[Bins + 1];
+    suffix[Bins] = 0;
+    for (int j = Bins - 1; j >= 0; --j) suffix[j] = suffix[j + 1] + hi_bins[j];
+    int prefix = 0, best = 0;
+    float best_cost = FLT_MAX;
+    for (int j = 0; j < Bins; ++j) {
+        prefix += lo_bins[j];
+        float cost = (float)((1 + suffix[j]) * (Bins - j) + (1 + prefix) * j) / Bins;
+        if (cost < best_cost) { best_cost = cost; best = j; }
+    }
+    return start + (best * extent) / Bins;
+}
+
+RefBox bounds_of(const std::vector<Prim>& prims, const int32_t* ids, int32_t n, float pad) {
+    RefBox r;
+    for (int k = 0; k < 3; ++k) { r.mn[k] = FLT_MAX; r.mx[k] = -FLT_MAX; }
+    for (int32_t i = 0; i < n; ++i) {
+        const Prim& p = prims[ids[i]];
+        for (int k = 0; k < 3; ++k) {
+            r.mn[k] = p.lo[k] < r.mn[k] ? p.lo[k] : r.mn[k];
+            r.mx[k] = p.hi[k] > r.mx[k] ? p.hi[k] : r.mx[k];
+        }
+    }
+    if (n > 0) for (int k = 0; k < 3; ++k) { r.mn[k] -= pad; r.mx[k] += pad; }
+    return r;
+}
+
 struct RefBuilder {
     const std::vector<Prim>& prims;
     std::vector<int32_t>& order;
@@ -169,48 +221,10 @@ struct RefBuilder {
     int max_leaf, max_depth;
     float min_width = 0;
 
-    RefBox bounds(int32_t b, int32_t e, float pad) const {
-        RefBox r;
-        for (int k = 0; k < 3; ++k) { r.mn[k] = FLT_MAX; r.mx[k] = -FLT_MAX; }
-        for (int32_t i = b; i < e; ++i) {
-            const Prim& p = prims[order[i]];
-            for (int k = 0; k < 3; ++k) {
-                r.mn[k] = p.lo[k] < r.mn[k] ? p.lo[k] : r.mn[k];
-                r.mx[k] = p.hi[k] > r.mx[k] ? p.hi[k] : r.mx[k];
-            }
-        }
-        if (e > b) for (int k = 0; k < 3; ++k) { r.mn[k] -= pad; r.mx[k] += pad; }
-        return r;
-    }
+    RefBox bounds(int32_t b, int32_t e, float pad) const { return bounds_of(prims, order.data() + b, e - b, pad); }
 
-    static int trunc_x86(float f) {
-        if (!(f > -2147483649.0f && f < 2147483648.0f)) return INT_MIN;
-        return (int)f;
-    }
-
-    // 128-bin count heuristic over the node box (TreeMesh.cpp:159-212)
     float partition_plane(int32_t b, int32_t e, const RefBox& box, int axis) const {
-        const int Bins = 128;
-        int lo_bins[Bins] = {0}, hi_bins[Bins] = {0};
-        float extent = box.mx[axis] - box.mn[axis], start = box.mn[axis];
-        for (int32_t i = b; i < e; ++i) {
-            const Prim& p = prims[order[i]];
-            int l = trunc_x86(Bins * ((p.lo[axis] - start) / extent));
-            int r = trunc_x86(Bins * ((p.hi[axis] - start) / extent));
-            if (l >= 0 && l < Bins) lo_bins[l]++;
-            if (r >= 0 && r < Bins) hi_bins[r]++;
-        }
-        int suffix[Bins + 1];
-        suffix[Bins] = 0;
-        for (int j = Bins - 1; j >= 0; --j) suffix[j] = suffix[j + 1] + hi_bins[j];
-        int prefix = 0, best = 0;
-        float best_cost = FLT_MAX;
-        for (int j = 0; j < Bins; ++j) {
-            prefix += lo_bins[j];
-            float cost = (float)((1 + suffix[j]) * (Bins - j) + (1 + prefix) * j) / Bins;
-            if (cost < best_cost) { best_cost = cost; best = j; }
-        }
-        return start + (best * extent) / Bins;
+        return count_plane(prims, order.data() + b, e - b, box, axis);
     }
 
     void build(int32_t ni, int32_t b, int32_t e, const RefBox& box, int axis, int depth) {
@@ -243,7 +257,82 @@ struct RefBuilder {
     }
 };
 
+// ---------------------------------------------------------------- kd-tree topologies
+// KDTreeMesh::BuildHierarchy_S2 (KDTreeMesh.cpp:961-1005) and
+// KDTreeMeshEx::BuildHierarchy_S2 (KDTreeMeshEx.cpp:68-112) as binary trees
+// of cells: a node's box is its cell (not the tight box of its triangles),
+// the split is the count-heuristic plane (KDTreeMesh) or the cell centre
+// (KDTreeMeshEx), and a triangle goes to every half-cell its box reaches
+// (Distribute, TreeMesh.cpp:91-106), so leaves may share triangles.
+struct KdBuilder {
+    const std::vector<Prim>& prims;
+    Bvh2& out;
+    int max_leaf, max_depth;
+    bool median;
+    float min_width = 0;
+
+    void build(int32_t ni, std::vector<int32_t>& list, const RefBox& box, int axis, int depth) {
+        for (int k = 0; k < 3; ++k) { out.nodes[ni].lo[k] = box.mn[k]; out.nodes[ni].hi[k] = box.mx[k]; }
+        const int32_t n = (int32_t)list.size();
+        if (n <= max_leaf || depth == max_depth || box.radius() <= min_width) {
+            out.nodes[ni].left = out.nodes[ni].right = -1;
+            out.nodes[ni].first = (int32_t)out.order.size();
+            out.nodes[ni].count = n;
+            if ((int64_t)out.order.size() + n > (int64_t)(1 << 26) - 1)
+                throw std::invalid_argument("kd-tree references exceed 2^26 - 1 triangle slots");
+            out.order.insert(out.order.end(), list.begin(), list.end());
+            return;
+        }
+        // FindPartitionPlaneSpatialMedian returns the box centre, which the
+        // reference keeps as min + (max - min) / 2 (BoundingBox.cpp:421-429).
+        float plane = median ? box.mn[axis] + (box.mx[axis] - box.mn[axis]) * 0.5f
+                             : count_plane(prims, list.data(), n, box, axis);
+        std::vector<int32_t> left, right;
+        for (int32_t id : list) {
+            const Prim& p = prims[id];
+            if (plane >= p.lo[axis]) left.push_back(id);
+            if (plane <= p.hi[axis]) right.push_back(id);
+        }
+        std::vector<int32_t>().swap(list);
+        int32_t l = (int32_t)out.nodes.size();
+        out.nodes.emplace_back();
+        out.nodes.emplace_back();
+        out.nodes[ni].left = l;
+        out.nodes[ni].right = l + 1;
+        out.nodes[ni].count = 0;
+        RefBox lb = box, rb = box;
+        lb.mx[axis] = plane;
+        rb.mn[axis] = plane;
+        int na = (axis + 1) % 3;
+        build(l, left, lb, na, depth + 1);
+        build(l + 1, right, rb, na, depth + 1);
+    }
+};
+
 }  // namespace
+
+Bvh2 build_kdtree(const float* pos, const int32_t* idx, int64_t nt, int max_leaf, int max_depth, bool median) {
+    if (nt <= 0) throw std::invalid_argument("empty mesh");
+    if (nt > (1 << 26) - 1) throw std::invalid_argument("more than 2^26 - 1 triangles");
+    Bvh2 out;
+    std::vector<Prim> prims = make_prims(pos, idx, nt);
+    std::vector<int32_t> all(nt);
+    for (int64_t i = 0; i < nt; ++i) all[i] = (int32_t)i;
+    out.nodes.emplace_back();
+    KdBuilder b{prims, out, max_leaf, max_depth, median};
+    RefBox tight = bounds_of(prims, all.data(), (int32_t)nt, 0.0f);
+    b.min_width = tight.radius() / 1000.0f;                              // KDTreeMesh.cpp:129
+    RefBox root = bounds_of(prims, all.data(), (int32_t)nt, 1e-4f);      // KDTreeMesh.cpp:951
+    // Vector3::ArgMaxAbsComponent of the half-extent (Vector3.inl:140-149)
+    float e[3];
+    for (int k = 0; k < 3; ++k) e[k] = (root.mx[k] - root.mn[k]) * 0.5f;
+    int axis = 0;
+    float mx = std::fabs(e[0]);
+    if (std::fabs(e[1]) > mx) { mx = std::fabs(e[1]); axis = 1; }
+    if (std::fabs(e[2]) > mx) axis = 2;
+    b.build(0, all, root, axis, 0);
+    return out;
+}
 
 Bvh2 build_sah(const float* pos, const int32_t* idx, int64_t nt, int max_leaf, int threads) {
     if (nt <= 0) throw std::invalid_argument("empty mesh");
@@ -383,6 +472,7 @@ struct Collapser {
         for (int k = 0; k < W; ++k) {
             if (k >= nk) { set_empty(ni, k); continue; }
             const Bvh2Node& n = b.nodes[kids[k]];
+            if (n.leaf() && n.count == 0) { set_empty(ni, k); continue; }  // empty kd cell
             int cn = 0;
             uint32_t slot;
             if (n.leaf()) slot = leaf_slot(n.first, n.count, n.lo, n.hi, depth + 1, cn);
@@ -399,6 +489,7 @@ struct Collapser {
 
 WideBvh collapse(const Bvh2& b, int width) {
     if (width != 2 && width != 4 && width != 8) throw std::invalid_argument("width must be 2, 4 or 8");
+    if (b.order.size() > (size_t)(1 << 26) - 1) throw std::invalid_argument("more than 2^26 - 1 triangle slots");
     WideBvh w;
     w.width = width;
     w.order = b.order;

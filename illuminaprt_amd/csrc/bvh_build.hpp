@@ -16,7 +16,7 @@ struct Bvh2Node {
 
 struct Bvh2 {
     std::vector<Bvh2Node> nodes;   // nodes[0] is the root
-    std::vector<int32_t> order;    // primitive index of each leaf slot
+    std::vector<int32_t> order;    // primitive index of each leaf slot (kd-trees repeat primitives)
 };
 
 // Binned SAH over centroids (32 bins, 3 axes), leaf size <= max_leaf.
@@ -28,6 +28,12 @@ Bvh2 build_sah(const float* pos, const int32_t* idx, int64_t nt, int max_leaf, i
 // distribution, leaf when n <= max_leaf, depth == max_depth or the node
 // radius <= root radius / 1000.
 Bvh2 build_reference(const float* pos, const int32_t* idx, int64_t nt, int max_leaf, int max_depth);
+
+// The reference kd-tree topologies as trees of cells with shared triangle
+// references: KDTreeMesh (count-heuristic planes; Shape/KDTreeMesh.cpp:114-135,
+// 941-1005) when median is false, KDTreeMeshEx (cell-centre planes;
+// Shape/KDTreeMeshEx.cpp:59-146) when it is true.
+Bvh2 build_kdtree(const float* pos, const int32_t* idx, int64_t nt, int max_leaf, int max_depth, bool median);
 
 struct WideBvh {
     int width = 4;

@@ -514,8 +514,9 @@ int ilm_raygen_secondary(ilm_mesh* mesh, const ilm_ray* rays_in, const ilm_hit* 
     if (n == 0) return ILM_OK;
     ilm_ctx* ctx = mesh->ctx;
     if (!mesh->d_slot_of) {
+        // kd-tree leaves may repeat a triangle: any of its slots holds it
         std::vector<int32_t> slot(mesh->nt);
-        for (int64_t k = 0; k < mesh->nt; ++k) slot[mesh->h_order[k]] = (int32_t)k;
+        for (size_t k = 0; k < mesh->h_order.size(); ++k) slot[mesh->h_order[k]] = (int32_t)k;
         int rc = ilm_alloc(ctx, slot.size() * sizeof(int32_t), (void**)&mesh->d_slot_of);
         if (rc) return rc;
         rc = ilm_upload(ctx, mesh->d_slot_of, slot.data(), slot.size() * sizeof(int32_t));

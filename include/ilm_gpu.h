@@ -110,13 +110,21 @@ int ilm_mesh_destroy(ilm_mesh* mesh);
 
 #define ILM_BUILDER_SAH 0        /* binned SAH over centroids (default) */
 #define ILM_BUILDER_REFERENCE 1  /* the reference BVHMesh topology */
+/* The reference kd-tree topologies, traversed as trees of cells whose leaves
+ * may share triangles: KDTreeMesh (Shape/KDTreeMesh.cpp:114-135,941-1005)
+ * and KDTreeMeshEx (Shape/KDTreeMeshEx.cpp:59-146; its depth limit is
+ * max_leaf, as the reference's constructors set it). Hits keep the ray window
+ * of every other topology (KDTreeMeshEx's own traversal widens it to the
+ * root cell's exit: KDTreeMeshEx.cpp:258-259,371-372). */
+#define ILM_BUILDER_KDTREE 2
+#define ILM_BUILDER_KDTREE_EX 3
 
 typedef struct {
     int32_t builder;     /* ILM_BUILDER_* */
     int32_t width;       /* 4 (default when 0); 2 and 8 also accepted */
     int32_t max_leaf;    /* SAH: max triangles per leaf (default 4);
-                            REFERENCE: MaximumLeafObjects (default 10) */
-    int32_t max_depth;   /* REFERENCE: MaximumTreeDepth (default 35) */
+                            REFERENCE, KDTREE*: MaximumLeafObjects (default 10) */
+    int32_t max_depth;   /* REFERENCE, KDTREE: MaximumTreeDepth (default 35, KDTREE at most 40) */
     int32_t threads;     /* host build threads (0 = all) */
 } ilm_bvh_params;
 
@@ -141,7 +149,8 @@ int ilm_bvh_get_info(ilm_mesh* mesh, ilm_bvh_info* info);
 
 /* Host copy of the flattened tree for validation: nodes as in HBM
  * (node_bytes), and the original triangle index of each slot of the
- * reordered triangle array (ntri int32). Either pointer may be NULL. */
+ * reordered triangle array (tri_bytes / 48 int32: ntri, or more when kd-tree
+ * leaves share triangles). Either pointer may be NULL. */
 int ilm_bvh_export(ilm_mesh* mesh, void* nodes, int32_t* tri_order);
 
 /* ---------------------------------------------------------------- tracing */

@@ -68,6 +68,10 @@ def lib():
     L.orc_bvh_build.restype = C.c_int64
     L.orc_bvh_stats.argtypes = [C.c_void_p, _i64p]
     L.orc_bvh_export.argtypes = [C.c_void_p, _f32p, _i32p, _i32p, _i32p, _i32p]
+    L.orc_kd_build.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
+    L.orc_kd_build.restype = C.c_int64
+    L.orc_kd_stats.argtypes = [C.c_void_p, _i64p]
+    L.orc_kd_export.argtypes = [C.c_void_p, _f32p, _i32p, _f32p, _i32p, _i32p, _i32p]
     L.orc_trace.argtypes = [C.c_void_p, _f32p, C.c_int64, C.c_int, C.c_int, _i32p, C.c_void_p, C.c_void_p, C.c_int]
     L.orc_trace.restype = C.c_int
     L.orc_scene_create.argtypes = [_f32p, _f32p, C.c_int64, _i32p, _i32p, _i32p, C.c_int64, _f32p, C.c_int, _f32p, C.c_int]
@@ -161,6 +165,30 @@ class Mesh:
         refs = np.zeros(max(1, self.stats()["refs"]), np.int32)
         lib().orc_bvh_export(self._h, box, child, lf, lc, refs)
         return box.reshape(n, 6), child.reshape(n, 2), lf, lc, refs
+
+    def kd_build(self, kind=1, max_leaf=10, max_depth=35):
+        """kind 1: KDTreeMesh, kind 2: KDTreeMeshEx. Trace with accel=2."""
+        n = lib().orc_kd_build(self._h, kind, max_leaf, max_depth)
+        if n < 0:
+            raise ValueError("bad kd-tree kind or depth")
+        self.kd_nodes = n
+        return n
+
+    def kd_stats(self):
+        out = np.zeros(4, np.int64)
+        lib().orc_kd_stats(self._h, out)
+        return dict(zip(["nodes", "leaves", "max_depth", "refs"], out.tolist()))
+
+    def kd_export(self):
+        n = self.kd_nodes
+        box = np.zeros(6 * n, np.float32)
+        child = np.zeros(2 * n, np.int32)
+        split = np.zeros(2 * n, np.float32)
+        lf = np.zeros(n, np.int32)
+        lc = np.zeros(n, np.int32)
+        refs = np.zeros(max(1, self.kd_stats()["refs"]), np.int32)
+        lib().orc_kd_export(self._h, box, child, split, lf, lc, refs)
+        return box.reshape(n, 6), child.reshape(n, 2), split.reshape(n, 2), lf, lc, refs
 
     def trace(self, rays, mode=0, accel=1, nthreads=0, want_bary=False):
         rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)

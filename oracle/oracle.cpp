@@ -20,6 +20,7 @@
 #include <cfloat>
 #include <climits>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <random>
@@ -150,6 +151,12 @@ struct orc_mesh {
     std::vector<Node> nodes;
     int max_leaf = 10, max_depth = 35;
     float min_node_width = 0;
+    // KDTreeMesh / KDTreeMeshEx nodes (KDTreeMesh.h:25-100, KDTreeMeshEx.h:19-81):
+    // cell boxes, split planes, duplicated triangle references.
+    std::vector<Node> kd;
+    int kd_kind = 0;      // 1: KDTreeMesh, 2: KDTreeMeshEx
+    Box kd_bounds;        // KDTreeMeshEx::m_boundingBox (the padded root cell)
+    int64_t kd_refs = 0, kd_leaves = 0, kd_maxdepth = 0;
     int64_t st_internal = 0, st_leaves = 0, st_maxdepth = 0, st_refs = 0, st_maxleaf = 0, st_empty = 0;
 };
 
@@ -283,6 +290,180 @@ bool bvh_any(const orc_mesh& m, int ni, const Ray& r, int& prim) {
     return false;
 }
 
+// ---------------------------------------------------------------- kd-trees
+// KDTreeMesh::BuildHierarchy_S2 (KDTreeMesh.cpp:961-1005) and
+// KDTreeMeshEx::BuildHierarchy_S2 (KDTreeMeshEx.cpp:68-112): leaf when
+// n <= MaxLeafObjects, depth == MaxTreeDepth or the cell radius <= root
+// radius / 1000; otherwise split the cell at the count heuristic's plane
+// (KDTreeMesh) or at its centre (KDTreeMeshEx, FindPartitionPlaneSpatialMedian,
+// TreeMesh.cpp:149-153), children are the two half-cells, and every triangle
+// whose box reaches a half-cell goes there (Distribute, TreeMesh.cpp:91-106).
+void kd_build(orc_mesh& m, int root, std::vector<int>&& rootlist, int axis0, bool median) {
+    struct Item { int node; std::vector<int> list; int axis; int depth; };
+    std::vector<Item> stack;
+    stack.push_back(Item{root, std::move(rootlist), axis0, 0});
+    while (!stack.empty()) {
+        Item it = std::move(stack.back());
+        stack.pop_back();
+        m.kd_maxdepth = std::max<int64_t>(m.kd_maxdepth, it.depth);
+        orc_mesh::Node* node = &m.kd[it.node];
+        if ((int)it.list.size() <= m.max_leaf || it.depth == m.max_depth ||
+            node->box.radius <= m.min_node_width) {
+            node->leaf = 1;
+            m.kd_leaves++;
+            m.kd_refs += it.list.size();
+            node->list = std::move(it.list);
+            continue;
+        }
+        node->leaf = 0;
+        node->axis = it.axis;
+        node->partition = median ? node->box.ctr[it.axis] : partition_sah(m, it.list, node->box, it.axis);
+        std::vector<int> left, right;
+        for (int idx : it.list) {
+            const Box& tb = m.tris[idx].box;
+            if (node->partition >= tb.mn[it.axis]) left.push_back(idx);
+            if (node->partition <= tb.mx[it.axis]) right.push_back(idx);
+        }
+        std::vector<int>().swap(it.list);
+        int c0 = (int)m.kd.size();
+        m.kd.emplace_back();
+        int c1 = (int)m.kd.size();
+        m.kd.emplace_back();
+        node = &m.kd[it.node];
+        node->child[0] = c0; node->child[1] = c1;
+        Box lb = node->box, rb = node->box;
+        lb.mx[it.axis] = node->partition; lb.update();  // SetMaxExtent(axis, partition)
+        rb.mn[it.axis] = node->partition; rb.update();  // SetMinExtent(axis, partition)
+        m.kd[c0].box = lb;
+        m.kd[c1].box = rb;
+        int na = (it.axis + 1) % 3;
+        stack.push_back(Item{c1, std::move(right), na, it.depth + 1});
+        stack.push_back(Item{c0, std::move(left), na, it.depth + 1});
+    }
+}
+
+// KDTreeMesh::Intersect_New, closest (KDTreeMesh.cpp:467-577) and any
+// (:313-388): ordered traversal with a todo stack; a node is left when the
+// current hit is nearer than its entry distance.
+bool kd_trace(const orc_mesh& m, const Ray& in, bool any, int& prim, float& t, float& b, float& g) {
+    float tmin, tmax;
+    if (!box_hit(m.kd[0].box, in, tmin, tmax)) return false;
+    Ray ray(in);
+    struct Todo { int node; float mn, mx; };
+    Todo todo[64];
+    int todoPos = 0;
+    bool hit = false;
+    int ni = 0;
+    while (ni >= 0) {
+        if (ray.tmax < tmin) break;
+        const orc_mesh::Node& node = m.kd[ni];
+        if (!node.leaf) {
+            int axis = node.axis;
+            float tplane = (node.partition - ray.o[axis]) * ray.inv[axis];
+            bool belowFirst = (ray.o[axis] < node.partition) ||
+                              (ray.o[axis] == node.partition && ray.d[axis] <= 0.0f);
+            int first = belowFirst ? node.child[0] : node.child[1];
+            int second = belowFirst ? node.child[1] : node.child[0];
+            if (tplane > tmax || tplane <= 0) {
+                ni = first;
+            } else if (tplane < tmin) {
+                ni = second;
+            } else {
+                if (todoPos == 64) std::abort();  // unreachable: depth <= 64 (orc_kd_build)
+                todo[todoPos++] = Todo{second, tplane, tmax};
+                ni = first;
+                tmax = tplane;
+            }
+        } else {
+            for (int idx : node.list) {
+                const Tri& tr = m.tris[idx];
+                float tt, bb, gg;
+                if (tri_hit(m.pos[tr.v[0]], tr.e1, tr.e2, ray, tt, bb, gg)) {
+                    prim = idx;
+                    if (any) return true;
+                    hit = true; t = tt; b = bb; g = gg;
+                    ray.tmax = rmin(ray.tmax, tt);
+                }
+            }
+            if (todoPos > 0) {
+                --todoPos;
+                ni = todo[todoPos].node;
+                tmin = todo[todoPos].mn;
+                tmax = todo[todoPos].mx;
+            } else {
+                break;
+            }
+        }
+    }
+    return hit;
+}
+
+// KDTreeMeshEx::Intersects_Stack, closest (KDTreeMeshEx.cpp:170-272) and any
+// (:276-390). Each leaf resets the ray window to (entry, tTemp), where tTemp
+// starts at the exit of the root cell: the caller's ray.Max is not kept, so a
+// segment ray can report a hit beyond its end (a property of the reference
+// traversal, restated as is).
+bool kdex_trace(const orc_mesh& m, const Ray& in, bool any, int& prim, float& t, float& b, float& g) {
+    float a, bb;
+    if (!box_hit(m.kd_bounds, in, a, bb)) return false;
+    Ray ray(in);
+    float tTemp = bb;
+    bool ret = false;
+    struct Elem { int node; float a, b; };
+    Elem st[30];
+    unsigned sp = 0;
+    st[sp++] = Elem{0, a, bb};
+    while (sp) {
+        const Elem& e = st[--sp];
+        int cur = e.node;
+        a = e.a;
+        bb = e.b;
+        while (!m.kd[cur].leaf) {
+            const orc_mesh::Node& n = m.kd[cur];
+            float diff = n.partition - ray.o[n.axis];
+            if (diff == 0) {
+                cur = ray.d[n.axis] > 0 ? n.child[1] : n.child[0];
+                continue;
+            }
+            int nearc = diff > 0 ? n.child[0] : n.child[1];
+            int farc = diff > 0 ? n.child[1] : n.child[0];
+            if (ray.d[n.axis] != 0) {
+                float tSplit = diff / ray.d[n.axis];
+                if (tSplit > bb || tSplit < 0) {
+                    cur = nearc;
+                } else if (tSplit < a) {
+                    cur = farc;
+                } else {
+                    if (sp + 1 >= 30) std::abort();  // unreachable: depth < 29 (orc_kd_build)
+                    st[sp++] = Elem{farc, tSplit, bb};
+                    cur = nearc;
+                    bb = tSplit;
+                }
+            } else {
+                cur = nearc;
+            }
+        }
+        const orc_mesh::Node& leaf = m.kd[cur];
+        if (!leaf.list.empty()) {
+            ray.tmin = a;  // ignored by the triangle test
+            ray.tmax = tTemp;
+            for (int idx : leaf.list) {
+                const Tri& tr = m.tris[idx];
+                float tt, b2, g2;
+                if (tri_hit(m.pos[tr.v[0]], tr.e1, tr.e2, ray, tt, b2, g2)) {
+                    prim = idx;
+                    if (any) return true;
+                    ray.tmax = tTemp = rmin(tTemp, tt);
+                    t = tt; b = b2; g = g2;
+                    ret = true;
+                }
+            }
+        }
+        if (tTemp <= bb && tTemp >= a) return any ? false : ret;
+    }
+    return false;
+}
+
 // BasicMesh::Intersects (BasicMesh.cpp:16-30 closest, :47-56 any).
 bool brute_closest(const orc_mesh& m, Ray r, int& prim, float& t, float& b, float& g) {
     bool hit = false;
@@ -306,11 +487,16 @@ bool brute_any(const orc_mesh& m, const Ray& r, int& prim) {
 }
 
 bool mesh_closest(const orc_mesh& m, const Ray& ray, int accel, int& prim, float& t, float& b, float& g) {
+    if (accel == 2 && m.kd_kind == 1) return kd_trace(m, ray, false, prim, t, b, g);
+    if (accel == 2 && m.kd_kind == 2) return kdex_trace(m, ray, false, prim, t, b, g);
     if (accel == 0 || m.nodes.empty()) return brute_closest(m, ray, prim, t, b, g);
     Ray r(ray);  // BVHMesh::Intersects copies the ray (BVHMesh.cpp:114-119)
     return bvh_closest(m, 0, r, prim, t, b, g);
 }
 bool mesh_any(const orc_mesh& m, const Ray& ray, int accel, int& prim) {
+    float t, b, g;
+    if (accel == 2 && m.kd_kind == 1) return kd_trace(m, ray, true, prim, t, b, g);
+    if (accel == 2 && m.kd_kind == 2) return kdex_trace(m, ray, true, prim, t, b, g);
     if (accel == 0 || m.nodes.empty()) return brute_any(m, ray, prim);
     return bvh_any(m, 0, ray, prim);
 }
@@ -791,6 +977,57 @@ int64_t orc_bvh_build(orc_mesh* m, int max_leaf, int max_depth) {
     else axis = size.y > size.z ? 1 : 2;
     build_s2(*m, 0, std::move(list), axis);
     return (int64_t)m->nodes.size();
+}
+
+int64_t orc_kd_build(orc_mesh* m, int kind, int max_leaf, int max_depth) {
+    if (kind != 1 && kind != 2) return -1;
+    // Traversal stacks: 64 entries (KDTreeMesh) and 30 (KDTreeMeshEx, which
+    // throws when it fills); deeper trees are refused here.
+    if ((kind == 1 && max_depth > 64) || (kind == 2 && max_leaf >= 29)) return -1;
+    m->kd.clear();
+    m->kd_kind = kind;
+    m->kd_refs = m->kd_leaves = m->kd_maxdepth = 0;
+    m->max_leaf = max_leaf;
+    // KDTreeMeshEx's constructors set the depth limit from the leaf size
+    // (KDTreeMeshEx.cpp:36-44); KDTreeMesh keeps MaximumTreeDepth.
+    m->max_depth = kind == 2 ? max_leaf : max_depth;
+    std::vector<int> list(m->tris.size());
+    for (size_t i = 0; i < list.size(); ++i) list[i] = (int)i;
+    Box tight;
+    compute_bounds(*m, list, tight, 0, 0);                 // Compile (KDTreeMesh.cpp:125, KDTreeMeshEx.cpp:138)
+    m->min_node_width = tight.radius / 1000.0f;
+    m->kd.emplace_back();
+    compute_bounds(*m, list, m->kd[0].box, 1e-4f, 1e-4f);  // BuildHierarchy (KDTreeMesh.cpp:951, KDTreeMeshEx.cpp:61)
+    m->kd_bounds = m->kd[0].box;
+    // Vector3::ArgMaxAbsComponent of the half-extent (Vector3.inl:140-149)
+    V3 e = m->kd[0].box.ext;
+    int axis = 0;
+    float mx = std::fabs(e.x);
+    if (std::fabs(e.y) > mx) { mx = std::fabs(e.y); axis = 1; }
+    if (std::fabs(e.z) > mx) axis = 2;
+    kd_build(*m, 0, std::move(list), axis, kind == 2);
+    return (int64_t)m->kd.size();
+}
+
+void orc_kd_stats(const orc_mesh* m, int64_t out[4]) {
+    out[0] = (int64_t)m->kd.size(); out[1] = m->kd_leaves; out[2] = m->kd_maxdepth; out[3] = m->kd_refs;
+}
+
+void orc_kd_export(const orc_mesh* m, float* box, int32_t* child, float* split, int32_t* leaf_first,
+                   int32_t* leaf_count, int32_t* refs) {
+    int32_t r = 0;
+    for (size_t i = 0; i < m->kd.size(); ++i) {
+        const orc_mesh::Node& n = m->kd[i];
+        box[6 * i + 0] = n.box.mn.x; box[6 * i + 1] = n.box.mn.y; box[6 * i + 2] = n.box.mn.z;
+        box[6 * i + 3] = n.box.mx.x; box[6 * i + 4] = n.box.mx.y; box[6 * i + 5] = n.box.mx.z;
+        child[2 * i] = n.leaf ? -1 : n.child[0];
+        child[2 * i + 1] = n.leaf ? -1 : n.child[1];
+        split[2 * i] = n.leaf ? 0.0f : (float)n.axis;
+        split[2 * i + 1] = n.leaf ? 0.0f : n.partition;
+        leaf_first[i] = r;
+        leaf_count[i] = n.leaf ? (int32_t)n.list.size() : 0;
+        if (n.leaf) for (int x : n.list) refs[r++] = x;
+    }
 }
 
 void orc_bvh_stats(const orc_mesh* m, int64_t out[6]) {

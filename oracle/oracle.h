@@ -77,10 +77,25 @@ void orc_bvh_stats(const orc_mesh* m, int64_t out[6]);
 void orc_bvh_export(const orc_mesh* m, float* box, int32_t* child,
                     int32_t* leaf_first, int32_t* leaf_count, int32_t* refs);
 
+/* KDTreeMesh (kind 1; Core/Shape/KDTreeMesh.cpp:114-135,941-1005) or
+ * KDTreeMeshEx (kind 2; Core/Shape/KDTreeMeshEx.cpp:36-112,118-146) over the
+ * same triangles. KDTreeMeshEx takes its depth limit from max_leaf, as the
+ * reference's constructors do. Returns the node count (-1 for a bad kind). */
+int64_t orc_kd_build(orc_mesh* m, int kind, int max_leaf, int max_depth);
+/* nodes, leaves, max depth, triangle references */
+void orc_kd_stats(const orc_mesh* m, int64_t out[4]);
+/* box[6n], child[2n] (-1 for leaves), split[2n] = (axis, plane), leaf ranges
+ * into refs (duplicated triangle references included). */
+void orc_kd_export(const orc_mesh* m, float* box, int32_t* child, float* split, int32_t* leaf_first,
+                   int32_t* leaf_count, int32_t* refs);
+
 /* Trace n rays; rays are 8 floats each: ox oy oz tmin dx dy dz tmax.
  * mode 0: closest hit (BVHMesh.cpp:114-119,169-198 / BasicMesh.cpp:16-45),
  * mode 1: any hit (BVHMesh.cpp:125-130,143-165 / BasicMesh.cpp:47-64).
- * accel 0: brute force (BasicMesh restatement), 1: BVHMesh restatement.
+ * accel 0: brute force (BasicMesh restatement), 1: BVHMesh restatement,
+ * 2: the kd-tree built last by orc_kd_build (KDTreeMesh::Intersect_New,
+ * KDTreeMesh.cpp:313-577, or KDTreeMeshEx::Intersects_Stack,
+ * KDTreeMeshEx.cpp:170-390).
  * prim[i] = hit triangle (-1 on miss); t and bary (2 floats) may be NULL.
  * nthreads <= 0 uses all hardware threads. Returns the threads used. */
 int orc_trace(const orc_mesh* m, const float* rays, int64_t n, int mode, int accel,

@@ -41,7 +41,7 @@ def _small_scene():
     return meshgen.merge([(sp, si), (tp, ti), (bp, bi)])
 
 
-@pytest.mark.parametrize("builder", [0, 1])
+@pytest.mark.parametrize("builder", [0, 1, 2, 3])
 def test_closest_matches_oracle_brute_force(gpu_ctx, ilm, orc, builder):
     pos, idx = _small_scene()
     rays = np.concatenate([meshgen.rays_toward(30000, pos, seed=7), meshgen.random_rays(20000, seed=8)])
@@ -67,19 +67,26 @@ def test_closest_matches_oracle_brute_force(gpu_ctx, ilm, orc, builder):
     m.close()
 
 
-def test_any_matches_oracle(gpu_ctx, ilm, orc):
+@pytest.mark.parametrize("builder", [0, 2])
+def test_any_matches_oracle(gpu_ctx, ilm, orc, builder):
     pos, idx = _small_scene()
     rays = meshgen.rays_toward(40000, pos, seed=17)
     rays[:, 7] = np.random.default_rng(3).uniform(0.5, 4.0, len(rays)).astype(np.float32)  # finite segments
     n = len(rays)
     m = ilm.Mesh(gpu_ctx, pos, idx)
-    m.build()
+    m.build(builder=builder)
     rb = ilm.rays_to_device(gpu_ctx, rays)
     ob = gpu_ctx.buffer(n)
     m.trace_any(rb, n, ob)
     gpu_ctx.sync()
     occ = ob.download(np.uint8, n)
-    pc, _ = orc.Mesh(pos, idx).trace(rays, mode=1, accel=0)
+    om = orc.Mesh(pos, idx)
+    pc, _ = om.trace(rays, mode=1, accel=0)
+    if builder == 2:
+        # the KDTreeMesh restatement's any-hit agrees with brute force too
+        om.kd_build(1, 10, 35)
+        pk, _ = om.trace(rays, mode=1, accel=2)
+        assert ((pk >= 0) == (pc >= 0)).mean() >= 0.9999
     agree = (occ == (pc >= 0)).mean()
     print(f"[any] agree={agree:.6f} occluded={occ.mean():.4f}")
     assert agree >= 0.9999

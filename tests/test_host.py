@@ -136,6 +136,44 @@ def test_reference_builder_matches_oracle_bvhmesh(ilm, orc):
     m.close()
 
 
+def _product_leaves(ilm, m):
+    nodes, order = m.export()
+    _, child = ilm.decode_bvh4(nodes)
+    leaves = []
+    for c in child.reshape(-1):
+        c = int(c)
+        if c & 0x80000000 and (c & 31):
+            first = (c & 0x7FFFFFFF) >> 5
+            leaves.append(tuple(order[first:first + (c & 31)].tolist()))
+    return leaves, order
+
+
+@pytest.mark.parametrize("kind,builder", [(1, 2), (2, 3)])
+def test_kdtree_builders_match_oracle(ilm, orc, kind, builder):
+    # ILM_BUILDER_KDTREE / _EX restate KDTreeMesh / KDTreeMeshEx: the same
+    # cells with the same (ordered) triangle lists as the oracle's
+    # restatement. The flattener splits leaves above 31 triangles into
+    # chunks, so those are compared as a multiset of references.
+    sp, si = meshgen.icosphere(3)
+    tp, ti = meshgen.random_triangles(1500, seed=9)
+    pos, idx = meshgen.merge([(sp, si), (tp, ti)])
+    om = orc.Mesh(pos, idx)
+    om.kd_build(kind, 10, 35)
+    _, ochild, _, olf, olc, orefs = om.kd_export()
+    oleaves = [tuple(orefs[olf[i]:olf[i] + olc[i]].tolist()) for i in range(len(ochild))
+               if ochild[i, 0] < 0 and olc[i] > 0]
+    m = ilm.Mesh(None, pos, idx)
+    info = m.build(builder=builder)
+    leaves, order = _product_leaves(ilm, m)
+    assert np.array_equal(np.sort(order), np.sort(orefs))
+    small = [l for l in oleaves if len(l) <= 31]
+    assert sorted(l for l in leaves if l in set(small)) == sorted(small)
+    if kind == 1:
+        assert sorted(leaves) == sorted(oleaves)
+    assert len(order) == info["tri_bytes"] // 48 > len(idx)
+    m.close()
+
+
 def test_mesh_without_context_cannot_trace(ilm):
     pos, idx = meshgen.icosphere(1)
     m = ilm.Mesh(None, pos, idx)

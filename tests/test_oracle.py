@@ -227,6 +227,92 @@ def test_bvhmesh_traversal_equals_brute_force(orc, mode):
     assert (pb >= 0).mean() > 0.5
 
 
+# ---------------------------------------------------------------- kd-trees
+def _kd_scene():
+    sp, si = meshgen.icosphere(4)
+    tp, ti = meshgen.random_triangles(2000, seed=5, extent=1.5)
+    return meshgen.merge([(sp, si), (tp, ti)])
+
+
+@pytest.mark.parametrize("kind", [1, 2])
+def test_kdtree_invariants(orc, kind):
+    pos, idx = _kd_scene()
+    m = orc.Mesh(pos, idx)
+    m.kd_build(kind, 10, 35)
+    box, child, split, lf, lc, refs = m.kd_export()
+    n = len(child)
+    tb_lo = pos[idx].min(axis=1)
+    tb_hi = pos[idx].max(axis=1)
+    seen = np.zeros(len(idx), np.int64)
+    depth = np.zeros(n, np.int64)
+    for i in range(n):
+        if child[i, 0] < 0:
+            ids = refs[lf[i]:lf[i] + lc[i]]
+            seen[ids] += 1
+            # every referenced triangle's box reaches the leaf cell
+            assert (tb_lo[ids] <= box[i, 3:] + 1e-6).all() and (tb_hi[ids] >= box[i, :3] - 1e-6).all()
+            continue
+        a, plane = int(split[i, 0]), split[i, 1]
+        l, r = child[i]
+        depth[l] = depth[r] = depth[i] + 1
+        # the two children are the parent cell cut at the plane
+        lo, hi = box[i, :3].copy(), box[i, 3:].copy()
+        hi_l = hi.copy(); hi_l[a] = plane
+        lo_r = lo.copy(); lo_r[a] = plane
+        assert np.array_equal(box[l], np.concatenate([lo, hi_l]))
+        assert np.array_equal(box[r], np.concatenate([lo_r, hi]))
+        assert lo[a] <= plane <= hi[a]
+    assert (seen >= 1).all()          # Distribute keeps every triangle
+    assert seen.sum() == m.kd_stats()["refs"] > len(idx)   # and duplicates straddlers
+    # KDTreeMeshEx's depth limit is the leaf size (KDTreeMeshEx.cpp:36-44)
+    assert depth.max() <= (35 if kind == 1 else 10)
+    if kind == 2:
+        # cell-centre planes
+        for i in np.nonzero(child[:, 0] >= 0)[0][:200]:
+            a = int(split[i, 0])
+            c = box[i, a] + (box[i, 3 + a] - box[i, a]) * np.float32(0.5)
+            assert split[i, 1] == np.float32(c)
+
+
+@pytest.mark.parametrize("kind", [1, 2])
+@pytest.mark.parametrize("mode", [0, 1])
+def test_kdtree_traversal_equals_brute_force(orc, kind, mode):
+    pos, idx = _kd_scene()
+    m = orc.Mesh(pos, idx)
+    m.kd_build(kind, 10, 35)
+    rays = np.concatenate([meshgen.rays_toward(6000, pos, seed=7), meshgen.random_rays(4000, seed=8)])
+    pb, tb = m.trace(rays, mode=mode, accel=0)
+    pk, tk = m.trace(rays, mode=mode, accel=2)
+    assert ((pb >= 0) == (pk >= 0)).all()
+    if mode == 0:
+        diff = pb != pk
+        assert (np.abs(tb[diff] - tk[diff]) <= 1e-5).all()
+        assert diff.mean() < 1e-3
+    assert (pb >= 0).mean() > 0.5
+
+
+def test_kdtree_segments(orc):
+    # KDTreeMesh keeps the ray window; KDTreeMeshEx replaces ray.Max by the
+    # root cell's exit (KDTreeMeshEx.cpp:258-259,371-372), so a segment that
+    # ends before a triangle still reports it.
+    pos, idx = meshgen.quad([-1, -1, 2], [1, -1, 2], [1, 1, 2], [-1, 1, 2])
+    tp, ti = meshgen.random_triangles(400, seed=2, extent=0.5)
+    pos, idx = meshgen.merge([(pos, idx), (tp + np.float32([0, 0, -3]), ti)])
+    ray = np.array([[0.05, 0.07, 0.0, 0.0, 0.0, 0.0, 1.0, 1.0]], np.float32)   # ends at z = 1
+    m = orc.Mesh(pos, idx)
+    assert m.trace(ray, mode=1, accel=0)[0][0] == -1
+    m.kd_build(1, 2, 35)
+    assert m.trace(ray, mode=1, accel=2)[0][0] == -1
+    assert m.trace(ray, mode=0, accel=2)[0][0] == -1
+    m.kd_build(2, 2, 35)
+    assert m.trace(ray, mode=1, accel=2)[0][0] >= 0
+    p, t = m.trace(ray, mode=0, accel=2)
+    assert p[0] >= 0 and abs(t[0] - 2.0) < 1e-5
+    # full-length rays agree with brute force
+    ray[0, 7] = F32MAX
+    assert m.trace(ray, mode=0, accel=2)[0][0] == m.trace(ray, mode=0, accel=0)[0][0]
+
+
 # ---------------------------------------------------------------- integrator
 def _closed_box(albedo, le):
     """Unit cube seen from inside: 6 walls, normals inward, each wall a light."""
