@@ -116,6 +116,16 @@ SIGNATURES = {
     "ilm_scenegen_icosphere": (C.c_int, [C.c_int, _f32, C.c_int, _u32, _vp, _vp]),
     "ilm_scenegen_boxes": (C.c_int, [C.c_int, _f32, _f32, _f32, _u32, _vp, _vp, _i64]),
     "ilm_scenegen_vertex_normals": (C.c_int, [_vp, _i64, _vp, _i64, _vp]),
+    "ilm_space_create": (C.c_int, [_vp, C.POINTER(_vp)]),
+    "ilm_space_destroy": (C.c_int, [_vp]),
+    "ilm_space_add": (C.c_int, [_vp, _vp, _i64, _vp, _i64, _vp, _vp, _vp, _vp, C.c_int32, C.c_int32,
+                                C.POINTER(C.c_int32)]),
+    "ilm_space_build": (C.c_int, [_vp, C.POINTER(BvhParams)]),
+    "ilm_space_mesh": (C.c_int, [_vp, C.POINTER(_vp)]),
+    "ilm_space_size": (C.c_int, [_vp, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64)]),
+    "ilm_space_shading": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "ilm_space_resolve": (C.c_int, [_vp, _vp, _i64, _vp, _vp, _vp]),
+    "ilm_space_geometry": (C.c_int, [_vp, _vp, _vp]),
     "ilm_scene_create": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.c_int, _vp, C.c_int, C.POINTER(_vp)]),
     "ilm_scene_destroy": (C.c_int, [_vp]),
     "ilm_render": (C.c_int, [_vp, C.POINTER(RenderParams), _vp, C.POINTER(RenderStats)]),
@@ -307,9 +317,66 @@ class Mesh:
                                           eps, seed, out.ptr, None))
 
     def close(self):
-        if self.h:
+        if self.h and not getattr(self, "borrowed", False):
             check(load().ilm_mesh_destroy(self.h))
+        self.h = None
+
+
+class Space:
+    """Primitives (mesh + the reference's Transformation + material + light)
+    flattened into one world-space mesh (replaces BasicSpace)."""
+
+    def __init__(self, ctx: Context | None):
+        self.ctx = ctx
+        h = C.c_void_p()
+        check(load().ilm_space_create(ctx.h if ctx else None, C.byref(h)))
+        self.h = h.value
+        self.mesh = None
+
+    def add(self, pos, idx, nrm=None, rot=None, scl=None, trn=None, material=0, light=-1) -> int:
+        pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+        idx = np.ascontiguousarray(idx, np.int32).reshape(-1, 3)
+        arrs = [None if a is None else np.ascontiguousarray(a, np.float32).reshape(-1) for a in (nrm, rot, scl, trn)]
+        out = C.c_int32()
+        check(load().ilm_space_add(self.h, _ptr(pos), len(pos), _ptr(idx), len(idx),
+                                   *[None if a is None else _ptr(a) for a in arrs], material, light, C.byref(out)))
+        return out.value
+
+    def build(self, builder=ILM_BUILDER_SAH, width=4, max_leaf=0, max_depth=0, threads=0) -> dict:
+        p = BvhParams(builder, width, max_leaf, max_depth, threads)
+        check(load().ilm_space_build(self.h, C.byref(p)))
+        h = C.c_void_p()
+        check(load().ilm_space_mesh(self.h, C.byref(h)))
+        m = Mesh.__new__(Mesh)       # a view of the space's mesh; the space owns it
+        m.ctx, m.h, m.borrowed = self.ctx, h.value, True
+        np_, nv, nt = C.c_int64(), C.c_int64(), C.c_int64()
+        check(load().ilm_space_size(self.h, C.byref(np_), C.byref(nv), C.byref(nt)))
+        self.nprims, self.nv, self.nt = np_.value, nv.value, nt.value
+        m.pos, m.idx = None, np.zeros((self.nt, 3), np.int32)
+        self.mesh = m
+        return m.info()
+
+    def shading(self):
+        nrm = np.zeros((self.nv, 3), np.float32)
+        tm = np.zeros(self.nt, np.int32)
+        tl = np.zeros(self.nt, np.int32)
+        check(load().ilm_space_shading(self.h, _ptr(nrm), _ptr(tm), _ptr(tl)))
+        return nrm, tm, tl
+
+    def geometry(self):
+        pos = np.zeros((self.nv, 3), np.float32)
+        idx = np.zeros((self.nt, 3), np.int32)
+        check(load().ilm_space_geometry(self.h, _ptr(pos), _ptr(idx)))
+        return pos, idx
+
+    def resolve(self, hits: DeviceBuffer, n: int, prim: DeviceBuffer, local: DeviceBuffer):
+        check(load().ilm_space_resolve(self.h, hits.ptr, n, prim.ptr, local.ptr, None))
+
+    def close(self):
+        if self.h:
+            load().ilm_space_destroy(self.h)
             self.h = None
+            self.mesh = None
 
 
 class Scene:

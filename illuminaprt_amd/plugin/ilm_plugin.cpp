@@ -201,6 +201,102 @@ void GpuBVHMesh::OccludedBatch(const std::vector<Ray>& rays, std::vector<uint8_t
     Check(rc);
 }
 
+// ---------------------------------------------------------------- GpuSpace
+GpuSpace::GpuSpace(ilm_ctx* ctx) : m_ctx(ctx) {
+    if (!ctx) throw new Exception("GpuSpace needs a device context");
+    Check(ilm_alloc(m_ctx, sizeof(ilm_ray) + sizeof(ilm_hit) + 2 * sizeof(int32_t), &m_stage));
+}
+
+GpuSpace::~GpuSpace() {
+    if (m_space) ilm_space_destroy(m_space);
+    if (m_stage) ilm_free(m_ctx, m_stage);
+}
+
+bool GpuSpace::Build() {
+    if (PrimitiveList.empty()) throw new Exception("GpuSpace::Build: no primitives");
+    if (m_space) {
+        ilm_space_destroy(m_space);
+        m_space = nullptr;
+        m_mesh = nullptr;
+    }
+    Check(ilm_space_create(m_ctx, &m_space));
+    for (const Primitive& p : PrimitiveList) {
+        if (!p.Shape) throw new Exception("GpuSpace::Build: primitive without a shape");
+        const std::vector<float>& pos = p.Shape->Positions();
+        const std::vector<int32_t>& idx = p.Shape->Indices();
+        Check(ilm_space_add(m_space, pos.data(), (int64_t)pos.size() / 3, idx.data(), (int64_t)idx.size() / 3,
+                            nullptr, p.Rotation, p.Scaling, p.Translation, p.Material, p.Light, nullptr));
+    }
+    ilm_bvh_params bp;
+    std::memset(&bp, 0, sizeof bp);
+    bp.builder = ILM_BUILDER_SAH;
+    Check(ilm_space_build(m_space, &bp));
+    Check(ilm_space_mesh(m_space, &m_mesh));
+    return true;
+}
+
+bool GpuSpace::Intersects(const Ray& ray, Intersection& intersection) {
+    if (!m_mesh) throw new Exception("GpuSpace: Build() has not been called");
+    ilm_ray q = ToRay(ray);
+    ilm_ray* dr = (ilm_ray*)m_stage;
+    ilm_hit* dh = (ilm_hit*)(dr + 1);
+    int32_t* dp = (int32_t*)(dh + 1);
+    Check(ilm_upload(m_ctx, dr, &q, sizeof q));
+    Check(ilm_trace_closest(m_mesh, dr, 1, dh, nullptr));
+    Check(ilm_space_resolve(m_space, dh, 1, dp, dp + 1, nullptr));
+    struct {
+        ilm_hit h;
+        int32_t prim, local;
+    } r;
+    Check(ilm_download(m_ctx, &r, dh, sizeof r));
+    if (r.h.prim < 0) return false;
+    FillSurface(ray, r.h, intersection.Surface);
+    intersection.Surface.TriangleIndex = r.local;
+    intersection.PrimitiveIndex = r.prim;
+    return true;
+}
+
+bool GpuSpace::Intersects(const Ray& ray) {
+    if (!m_mesh) throw new Exception("GpuSpace: Build() has not been called");
+    ilm_ray q = ToRay(ray);
+    uint8_t occ = 0;
+    ilm_ray* dr = (ilm_ray*)m_stage;
+    uint8_t* dh = (uint8_t*)(dr + 1);
+    Check(ilm_upload(m_ctx, dr, &q, sizeof q));
+    Check(ilm_trace_any(m_mesh, dr, 1, dh, nullptr));
+    Check(ilm_download(m_ctx, &occ, dh, 1));
+    return occ != 0;
+}
+
+void GpuSpace::IntersectBatch(const std::vector<Ray>& rays, std::vector<Intersection>& out) {
+    if (!m_mesh) throw new Exception("GpuSpace: Build() has not been called");
+    const int64_t n = (int64_t)rays.size();
+    out.assign(rays.size(), Intersection());
+    if (n == 0) return;
+    std::vector<ilm_ray> q(n);
+    for (int64_t i = 0; i < n; ++i) q[i] = ToRay(rays[i]);
+    std::vector<ilm_hit> h(n);
+    std::vector<int32_t> pl(2 * n);
+    void* buf = nullptr;
+    Check(ilm_alloc(m_ctx, n * (sizeof(ilm_ray) + sizeof(ilm_hit) + 2 * sizeof(int32_t)), &buf));
+    ilm_ray* dr = (ilm_ray*)buf;
+    ilm_hit* dh = (ilm_hit*)(dr + n);
+    int32_t* dp = (int32_t*)(dh + n);
+    int rc = ilm_upload(m_ctx, dr, q.data(), n * sizeof(ilm_ray));
+    if (rc == ILM_OK) rc = ilm_trace_closest(m_mesh, dr, n, dh, nullptr);
+    if (rc == ILM_OK) rc = ilm_space_resolve(m_space, dh, n, dp, dp + n, nullptr);
+    if (rc == ILM_OK) rc = ilm_download(m_ctx, h.data(), dh, n * sizeof(ilm_hit));
+    if (rc == ILM_OK) rc = ilm_download(m_ctx, pl.data(), dp, 2 * n * sizeof(int32_t));
+    ilm_free(m_ctx, buf);
+    Check(rc);
+    for (int64_t i = 0; i < n; ++i) {
+        if (h[i].prim < 0) continue;
+        FillSurface(rays[i], h[i], out[i].Surface);
+        out[i].Surface.TriangleIndex = pl[n + i];
+        out[i].PrimitiveIndex = pl[i];
+    }
+}
+
 // ---------------------------------------------------------------- GpuPathTracing
 GpuPathTracing::GpuPathTracing(ilm_ctx* ctx, int rayDepth, float epsilon)
     : m_ctx(ctx), m_rayDepth(rayDepth), m_epsilon(epsilon) {
@@ -286,6 +382,13 @@ public:
     }
 };
 
+// Registered as "GpuSpace" in the place of the "Basic" space factory
+// (Space/SpaceFactories.h); it takes no arguments.
+class GpuSpaceFactory : public Factory<ISpace> {
+public:
+    ISpace* CreateInstance(ArgumentMap&) override { return new GpuSpace(GpuContext()); }
+};
+
 // Integrator arguments as PathIntegratorFactory reads them
 // (Integrator/IntegratorFactories.h:41-57). One shadow ray per light per
 // vertex is what the GPU integrator traces; another ShadowRays value throws.
@@ -311,18 +414,21 @@ public:
     const std::string& GetVersion() const override { return m_version; }
     void Register(EngineKernel* engine) override {
         engine->GetShapeManager().RegisterFactory("GpuBVHMesh", &m_shape);
+        engine->GetSpaceManager().RegisterFactory("GpuSpace", &m_space);
         engine->GetIntegratorManager().RegisterFactory("GpuPathTracing", &m_integrator);
     }
     void Initialise() override { GpuContext(); }
     void Shutdown() override {}
     void Unregister(EngineKernel* engine) override {
         engine->GetShapeManager().UnregisterFactory("GpuBVHMesh");
+        engine->GetSpaceManager().UnregisterFactory("GpuSpace");
         engine->GetIntegratorManager().UnregisterFactory("GpuPathTracing");
     }
 private:
     std::string m_name = "GpuAccel";
     std::string m_version = "1.0.0";
     GpuBVHMeshFactory m_shape;
+    GpuSpaceFactory m_space;
     GpuPathTracingFactory m_integrator;
 };
 

@@ -98,6 +98,59 @@ private:
     void* m_stage = nullptr;  // one-ray staging (ray + hit)
 };
 
+// The subset of Intersection (Core/Geometry/Intersection.h) a space fills:
+// the surface, and which primitive of the space's list was hit.
+struct Intersection {
+    DifferentialSurface Surface;  // TriangleIndex is the triangle within the primitive
+    int PrimitiveIndex = -1;
+};
+
+// A primitive of a space: GeometricPrimitive (Scene/GeometricPrimitive.h) =
+// shape + WorldTransform (rotation as the reference's Matrix3x3 elements in
+// storage order, scaling, translation) + material; EmissivePrimitive adds a
+// light (Light >= 0).
+struct Primitive {
+    GpuBVHMesh* Shape = nullptr;
+    float Rotation[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    float Scaling[3] = {1, 1, 1};
+    float Translation[3] = {0, 0, 0};
+    int Material = 0, Light = -1;
+};
+
+// ISpace (Space/Space.h:31-43) without the exclude-primitive overloads.
+class ISpace {
+public:
+    virtual ~ISpace() {}
+    std::vector<Primitive> PrimitiveList;
+    virtual bool Initialise() = 0;
+    virtual void Shutdown() = 0;
+    virtual bool Update() = 0;
+    virtual bool Build() = 0;
+    virtual bool Intersects(const Ray& ray, Intersection& intersection) = 0;
+    virtual bool Intersects(const Ray& ray) = 0;
+};
+
+// BasicSpace (Space/BasicSpace.cpp:58-114) on the GPU: Build flattens the
+// primitive list into one world-space BVH (ilm_space_*).
+class GpuSpace : public ISpace {
+public:
+    explicit GpuSpace(ilm_ctx* ctx);
+    ~GpuSpace();
+    bool Initialise() override { return true; }
+    void Shutdown() override {}
+    bool Update() override { return true; }
+    bool Build() override;
+    bool Intersects(const Ray& ray, Intersection& intersection) override;
+    bool Intersects(const Ray& ray) override;
+    void IntersectBatch(const std::vector<Ray>& rays, std::vector<Intersection>& out);
+    ilm_space* Handle() const { return m_space; }
+private:
+    ilm_ctx* m_ctx;
+    ilm_space* m_space = nullptr;
+    ilm_mesh* m_mesh = nullptr;
+    void* m_stage = nullptr;  // one ray, hit, primitive and triangle index
+};
+
 // Camera arguments as CameraFactories.h:45-63 reads them.
 struct CameraArgs {
     Vector3 Position, Target, Up = Vector3(0, 1, 0);
@@ -185,9 +238,12 @@ private:
 class EngineKernel {
 public:
     FactoryManager<IShape>& GetShapeManager() { return m_shapes; }
+    FactoryManager<ISpace>& GetSpaceManager() { return m_spaces; }
     FactoryManager<IIntegrator>& GetIntegratorManager() { return m_integrators; }
 private:
+    // declared so that integrators and spaces are released before shapes
     FactoryManager<IShape> m_shapes;
+    FactoryManager<ISpace> m_spaces;
     FactoryManager<IIntegrator> m_integrators;
 };
 

@@ -14,6 +14,12 @@
 //       out: batched closest hits (n x {t, prim, u, v}), batched occlusion
 //            (n bytes), per-ray closest hits and occlusion for the first
 //            nper rays (IShape::Intersects one ray at a time).
+//   plugin_sandbox <plugin.so> space <in.bin> <out.bin> ""
+//       in:  int64 nprims, n, nper; per primitive: int64 nv, nt; float rot[9],
+//            scl[3], trn[3]; float pos[3nv]; int32 idx[3nt]; then float rays[8n]
+//       out: n x {t, primitive, triangle} (batched), then the same for the
+//            first nper rays through the per-ray ISpace::Intersects, then
+//            nper occlusion bytes.
 //   plugin_sandbox <plugin.so> render <in.bin> <out.bin> "<integrator args>"
 //       in:  int64 nv, nt, nmat, nlight; int32 width, height, spp, seed;
 //            float camera[11] (pos, target, up, fov, aspect); float pos[3nv];
@@ -85,6 +91,7 @@ static int run_host(IPlugIn* plugin) {
     plugin->Register(&kernel);
     ok &= kernel.GetShapeManager().QueryFactory("GpuBVHMesh");
     ok &= kernel.GetIntegratorManager().QueryFactory("GpuPathTracing");
+    ok &= kernel.GetSpaceManager().QueryFactory("GpuSpace");
     ok &= expect_throw([](EngineKernel& k) { k.GetShapeManager().RegisterFactory("GpuBVHMesh", nullptr); }, kernel,
                        "duplicate factory");
     ok &= expect_throw(
@@ -181,6 +188,78 @@ static int run_trace(IPlugIn* plugin, const char* in, const char* out, const cha
     return 0;
 }
 
+static int run_space(IPlugIn* plugin, const char* in, const char* out) {
+    FILE* f = fopen(in, "rb");
+    if (!f) throw new Exception(std::string("cannot open ") + in);
+    const int64_t np = read_one<int64_t>(f), n = read_one<int64_t>(f), nper = read_one<int64_t>(f);
+    if (nper > n) throw new Exception("nper > n");
+    EngineKernel kernel;
+    plugin->Register(&kernel);
+    plugin->Initialise();
+    ArgumentMap none;
+    GpuSpace* space = dynamic_cast<GpuSpace*>(kernel.GetSpaceManager().CreateInstance("GpuSpace", "space", none));
+    if (!space) throw new Exception("factory returned another space type");
+    for (int64_t k = 0; k < np; ++k) {
+        const int64_t nv = read_one<int64_t>(f), nt = read_one<int64_t>(f);
+        std::vector<float> xf, pos;
+        std::vector<int32_t> idx;
+        read_into(f, xf, 15);
+        read_into(f, pos, (size_t)nv * 3);
+        read_into(f, idx, (size_t)nt * 3);
+        ArgumentMap shapeArgs("Builder = SAH");
+        GpuBVHMesh* mesh = dynamic_cast<GpuBVHMesh*>(
+            kernel.GetShapeManager().CreateInstance("GpuBVHMesh", "shape" + std::to_string(k), shapeArgs));
+        for (int64_t v = 0; v < nv; ++v) mesh->AddVertex(Vector3(pos[3 * v], pos[3 * v + 1], pos[3 * v + 2]));
+        for (int64_t t = 0; t < nt; ++t) mesh->AddIndexedTriangle(idx[3 * t], idx[3 * t + 1], idx[3 * t + 2]);
+        Primitive p;
+        p.Shape = mesh;
+        memcpy(p.Rotation, &xf[0], sizeof p.Rotation);
+        memcpy(p.Scaling, &xf[9], sizeof p.Scaling);
+        memcpy(p.Translation, &xf[12], sizeof p.Translation);
+        space->PrimitiveList.push_back(p);
+    }
+    std::vector<float> rays;
+    read_into(f, rays, (size_t)n * 8);
+    fclose(f);
+    space->Initialise();
+    space->Build();
+    std::vector<Ray> rs(n);
+    for (int64_t i = 0; i < n; ++i) {
+        const float* r = &rays[8 * i];
+        rs[i].Origin = Vector3(r[0], r[1], r[2]);
+        rs[i].Min = r[3];
+        rs[i].Direction = Vector3(r[4], r[5], r[6]);
+        rs[i].Max = r[7];
+    }
+    std::vector<Intersection> hits;
+    space->IntersectBatch(rs, hits);
+    FILE* o = fopen(out, "wb");
+    if (!o) throw new Exception(std::string("cannot open ") + out);
+    auto put = [&](const Intersection& x) {
+        float t = x.Surface.Distance;
+        int32_t rec[2] = {x.PrimitiveIndex, x.Surface.TriangleIndex};
+        fwrite(&t, 4, 1, o);
+        fwrite(rec, 4, 2, o);
+    };
+    for (int64_t i = 0; i < n; ++i) put(hits[i]);
+    for (int64_t i = 0; i < nper; ++i) {
+        Intersection x;
+        if (!space->Intersects(rs[i], x)) x = Intersection();
+        put(x);
+    }
+    for (int64_t i = 0; i < nper; ++i) {
+        uint8_t b = space->Intersects(rs[i]) ? 1 : 0;
+        fwrite(&b, 1, 1, o);
+    }
+    fclose(o);
+    space->Shutdown();
+    kernel.GetSpaceManager().ReleaseInstance("space");
+    plugin->Shutdown();
+    plugin->Unregister(&kernel);
+    printf("space ok: %lld primitives, %lld rays\n", (long long)np, (long long)n);
+    return 0;
+}
+
 static int run_render(IPlugIn* plugin, const char* in, const char* out, const char* args) {
     FILE* f = fopen(in, "rb");
     if (!f) throw new Exception(std::string("cannot open ") + in);
@@ -259,6 +338,8 @@ int main(int argc, char** argv) {
             rc = run_host(plugin);
         else if (mode == "trace" && argc == 6)
             rc = run_trace(plugin, argv[3], argv[4], argv[5]);
+        else if (mode == "space" && argc == 6)
+            rc = run_space(plugin, argv[3], argv[4]);
         else if (mode == "render" && argc == 6)
             rc = run_render(plugin, argv[3], argv[4], argv[5]);
         else

@@ -172,6 +172,57 @@ int ilm_trace_any(ilm_mesh* mesh, const ilm_ray* rays, int64_t n, uint8_t* occlu
 int ilm_trace_stats(ilm_mesh* mesh, const ilm_ray* rays, int64_t n, int any_hit,
                     int32_t* visits, void* stream);
 
+/* ---------------------------------------------------------------- spaces */
+
+/* A space holds the primitives of a scene: a triangle shape placed by the
+ * reference's Transformation, with a material index and a light index (-1:
+ * not emissive). Building flattens every primitive to world space into one
+ * mesh with one BVH, which ilm_trace_closest/any traverse in a single launch.
+ * Replaces BasicSpace::Intersects (Space/BasicSpace.cpp:58-114), a linear loop
+ * that transforms each ray into each primitive's object space
+ * (GeometricPrimitive::Intersects, Scene/GeometricPrimitive.cpp:68-200; the
+ * ray parameter t is the same in both spaces, so hits keep the reference's
+ * t window). The exclude-primitive overloads (BasicSpace.cpp:75-114) are not
+ * provided: the integrator on this path does not use them
+ * (Integrator.cpp:124). */
+typedef struct ilm_space ilm_space;
+
+/* ctx may be NULL: the space can then be flattened and built on the host
+ * (for export) but not traced or resolved. */
+int ilm_space_create(ilm_ctx* ctx, ilm_space** out);
+int ilm_space_destroy(ilm_space* space);
+/* Adds a primitive (GeometricPrimitive: shape + WorldTransform + material,
+ * EmissivePrimitive when light >= 0). Arrays are copied. nrm = object-space
+ * vertex normals or NULL (area-weighted, as ITriangleMesh::UpdateNormals).
+ * rotation = the 9 elements of the reference's Matrix3x3 in storage order,
+ * applied as Matrix3x3::Product (Geometry/Matrix3x3.h:586-597), i.e.
+ * world_i = sum_j rotation[3j+i] * (scaling_j * p_j) + translation_i
+ * (Transformation::Apply, Geometry/Transform.cpp:133-145); NULL = identity.
+ * Normals are scaled and rotated (Transformation::Rotate, :178-194) and
+ * normalised. *prim_id (may be NULL) receives the primitive's index. */
+int ilm_space_add(ilm_space* space, const float* pos, int64_t nv, const int32_t* idx, int64_t nt,
+                  const float* nrm, const float* rotation, const float* scaling, const float* translation,
+                  int32_t material, int32_t light, int32_t* prim_id);
+/* Flattens and builds (ISpace::Build). Synchronous. */
+int ilm_space_build(ilm_space* space, const ilm_bvh_params* params);
+/* The flattened world-space mesh (owned by the space, valid until the next
+ * build or destroy); hits report global triangle indices. */
+int ilm_space_mesh(ilm_space* space, ilm_mesh** mesh);
+/* Primitive count and flattened vertex / triangle counts. */
+int ilm_space_size(ilm_space* space, int64_t* nprims, int64_t* nv, int64_t* nt);
+/* Host copies of the flattened shading arrays, ready for ilm_scene_create:
+ * world vertex normals (nv*3), per-triangle material and light (nt each).
+ * Any pointer may be NULL. */
+int ilm_space_shading(ilm_space* space, float* nrm, int32_t* tri_mat, int32_t* tri_light);
+/* Host copies of the flattened world positions (nv*3) and triangle indices
+ * (nt*3, global vertex indices). Either pointer may be NULL. */
+int ilm_space_geometry(ilm_space* space, float* pos, int32_t* idx);
+/* prim[i], local[i] = primitive and triangle within it of hits[i] (-1, -1 on
+ * a miss), as Intersection::GetPrimitive and the surface's triangle report
+ * them. hits, prim and local are dev pointers. */
+int ilm_space_resolve(ilm_space* space, const ilm_hit* hits, int64_t n, int32_t* prim, int32_t* local,
+                      void* stream);
+
 /* ---------------------------------------------------------------- ray sets */
 
 /* Camera description: PerspectiveCamera with the factory's frame

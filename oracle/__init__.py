@@ -72,6 +72,12 @@ def lib():
     L.orc_kd_build.restype = C.c_int64
     L.orc_kd_stats.argtypes = [C.c_void_p, _i64p]
     L.orc_kd_export.argtypes = [C.c_void_p, _f32p, _i32p, _f32p, _i32p, _i32p, _i32p]
+    L.orc_space_create.restype = C.c_void_p
+    L.orc_space_destroy.argtypes = [C.c_void_p]
+    L.orc_space_add.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.orc_space_add.restype = C.c_int
+    L.orc_space_trace.argtypes = [C.c_void_p, _f32p, C.c_int64, C.c_int, _i32p, _i32p, C.c_void_p, C.c_int]
+    L.orc_space_trace.restype = C.c_int
     L.orc_trace.argtypes = [C.c_void_p, _f32p, C.c_int64, C.c_int, C.c_int, _i32p, C.c_void_p, C.c_void_p, C.c_int]
     L.orc_trace.restype = C.c_int
     L.orc_scene_create.argtypes = [_f32p, _f32p, C.c_int64, _i32p, _i32p, _i32p, C.c_int64, _f32p, C.c_int, _f32p, C.c_int]
@@ -206,6 +212,35 @@ class Mesh:
     def __del__(self):
         if getattr(self, "_h", None) and _lib is not None:
             _lib.orc_mesh_destroy(self._h)
+            self._h = None
+
+
+class Space:
+    """BasicSpace restatement: primitives = (Mesh, rotation elements, scaling,
+    translation) traced in object space after Ray::ApplyInverse."""
+
+    def __init__(self):
+        self._h = lib().orc_space_create()
+        self._keep = []
+
+    def add(self, mesh: "Mesh", rot=None, scl=None, trn=None) -> int:
+        arrs = [None if a is None else np.ascontiguousarray(a, np.float32).reshape(-1) for a in (rot, scl, trn)]
+        self._keep.append((mesh, arrs))
+        ptr = [None if a is None else a.ctypes.data for a in arrs]
+        return lib().orc_space_add(self._h, mesh._h, *ptr)
+
+    def trace(self, rays, mode=0, nthreads=0):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        n = len(rays)
+        prim = np.zeros(n, np.int32)
+        tri = np.zeros(n, np.int32)
+        t = np.zeros(n, np.float32)
+        lib().orc_space_trace(self._h, rays, n, mode, prim, tri, t.ctypes.data, nthreads)
+        return prim, tri, t
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.orc_space_destroy(self._h)
             self._h = None
 
 

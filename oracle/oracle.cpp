@@ -1072,6 +1072,106 @@ int orc_trace(const orc_mesh* m, const float* rays, int64_t n, int mode, int acc
     return nt;
 }
 
+// ---------------------------------------------------------------- spaces
+// BasicSpace over GeometricPrimitives (Space/BasicSpace.cpp:58-114,
+// Scene/GeometricPrimitive.cpp:68-200): each primitive is a mesh placed by a
+// Transformation (rotation elements in the reference's Matrix3x3 storage
+// order, scaling, translation; Geometry/Transform.cpp:133-199).
+struct orc_space {
+    struct Prim {
+        const orc_mesh* mesh;
+        float rot[9], scl[3], trn[3];
+        bool has_rot, has_scl, has_trn;
+    };
+    std::vector<Prim> prims;
+};
+
+namespace {
+
+// Ray::ApplyInverse (Geometry/Ray.cpp:50-60) with Transformation::RotateInverse,
+// ScaleInverse and ApplyInverse (Transform.cpp:147-176,196-199); the rotation
+// inverse is the transpose (Transform.cpp:42), so Product(inverse, v) is the
+// row-major product of the stored elements with v.
+Ray inverse_ray(const orc_space::Prim& p, const Ray& r) {
+    auto rot_inv = [&](V3 v) {
+        return V3(p.rot[0] * v.x + p.rot[1] * v.y + p.rot[2] * v.z,
+                  p.rot[3] * v.x + p.rot[4] * v.y + p.rot[5] * v.z,
+                  p.rot[6] * v.x + p.rot[7] * v.y + p.rot[8] * v.z);
+    };
+    V3 d = rot_inv(r.d);
+    if (p.has_scl) d = V3(d.x / p.scl[0], d.y / p.scl[1], d.z / p.scl[2]);
+    V3 o = rot_inv(r.o - V3(p.trn[0], p.trn[1], p.trn[2]));
+    float yz = p.scl[1] * p.scl[2], xz = p.scl[0] * p.scl[2], xy = p.scl[0] * p.scl[1];
+    float ixyz = 1 / (xy * p.scl[2]);
+    o.x *= (ixyz * yz);
+    o.y *= (ixyz * xz);
+    o.z *= (ixyz * xy);
+    return Ray(o, d, r.tmin, r.tmax);
+}
+
+bool prim_closest(const orc_space::Prim& p, const Ray& r, int& tri, float& t) {
+    float b, g;
+    int accel = p.mesh->nodes.empty() ? 0 : 1;
+    if (!(p.has_rot || p.has_scl || p.has_trn)) return mesh_closest(*p.mesh, r, accel, tri, t, b, g);
+    return mesh_closest(*p.mesh, inverse_ray(p, r), accel, tri, t, b, g);
+}
+
+bool prim_any(const orc_space::Prim& p, const Ray& r, int& tri) {
+    int accel = p.mesh->nodes.empty() ? 0 : 1;
+    if (!(p.has_rot || p.has_scl || p.has_trn)) return mesh_any(*p.mesh, r, accel, tri);
+    return mesh_any(*p.mesh, inverse_ray(p, r), accel, tri);
+}
+
+}  // namespace
+
+orc_space* orc_space_create(void) { return new orc_space(); }
+void orc_space_destroy(orc_space* s) { delete s; }
+
+int orc_space_add(orc_space* s, const orc_mesh* m, const float* rot, const float* scl, const float* trn) {
+    orc_space::Prim p;
+    p.mesh = m;
+    const float I[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, one[3] = {1, 1, 1}, zero[3] = {0, 0, 0};
+    std::memcpy(p.rot, rot ? rot : I, sizeof p.rot);
+    std::memcpy(p.scl, scl ? scl : one, sizeof p.scl);
+    std::memcpy(p.trn, trn ? trn : zero, sizeof p.trn);
+    // Transformation::SetRotation / SetScaling / SetTranslation flags (Transform.cpp:48-50)
+    p.has_rot = std::memcmp(p.rot, I, sizeof I) != 0;
+    p.has_scl = !(p.scl[0] == 1 && p.scl[1] == 1 && p.scl[2] == 1);
+    p.has_trn = !(p.trn[0] == 0 && p.trn[1] == 0 && p.trn[2] == 0);
+    s->prims.push_back(p);
+    return (int)s->prims.size() - 1;
+}
+
+int orc_space_trace(const orc_space* s, const float* rays, int64_t n, int mode, int32_t* prim, int32_t* tri,
+                    float* t, int nthreads) {
+    int nt = resolve_threads(nthreads);
+    parallel_chunks(n, 1024, nt, [&](int64_t b, int64_t e) {
+        for (int64_t i = b; i < e; ++i) {
+            const float* q = rays + 8 * i;
+            Ray ray(V3(q[0], q[1], q[2]), V3(q[4], q[5], q[6]), q[3], q[7]);
+            int hp = -1, ht = -1;
+            float tt = 0;
+            for (int k = 0; k < (int)s->prims.size(); ++k) {
+                int tr = -1;
+                float tk = 0;
+                if (mode == 0) {
+                    if (prim_closest(s->prims[k], ray, tr, tk)) {   // BasicSpace.cpp:58-73
+                        hp = k; ht = tr; tt = tk;
+                        ray.tmax = rmin(ray.tmax, tk);
+                    }
+                } else if (prim_any(s->prims[k], ray, tr)) {         // BasicSpace.cpp:92-101
+                    hp = k; ht = tr;
+                    break;
+                }
+            }
+            prim[i] = hp;
+            tri[i] = ht;
+            if (t) t[i] = tt;
+        }
+    });
+    return nt;
+}
+
 orc_scene* orc_scene_create(const float* pos, const float* nrm, int64_t nv, const int32_t* idx,
                             const int32_t* tri_mat, const int32_t* tri_light, int64_t nt,
                             const float* mats, int nmat, const float* lights, int nlight) {

@@ -101,6 +101,22 @@ void orc_kd_export(const orc_mesh* m, float* box, int32_t* child, float* split, 
 int orc_trace(const orc_mesh* m, const float* rays, int64_t n, int mode, int accel,
               int32_t* prim, float* t, float* bary, int nthreads);
 
+/* ---- spaces ------------------------------------------------------------ */
+
+/* BasicSpace over GeometricPrimitives (Core/Space/BasicSpace.cpp:58-114,
+ * Core/Scene/GeometricPrimitive.cpp:68-200). A primitive is a mesh (its
+ * BVHMesh restatement is used when built) placed by the reference's
+ * Transformation: rot = Matrix3x3 elements _00.._22 in storage order (applied
+ * as Matrix3x3::Product, Geometry/Matrix3x3.h:586-597), scl, trn; NULL means
+ * identity. Rays are transformed to object space (Ray::ApplyInverse). */
+typedef struct orc_space orc_space;
+orc_space* orc_space_create(void);
+void orc_space_destroy(orc_space* s);
+int orc_space_add(orc_space* s, const orc_mesh* m, const float* rot, const float* scl, const float* trn);
+/* prim[i] = primitive hit (-1 on miss), tri[i] = triangle within it, t may be NULL. */
+int orc_space_trace(const orc_space* s, const float* rays, int64_t n, int mode, int32_t* prim, int32_t* tri,
+                    float* t, int nthreads);
+
 /* ---- scenes and the path integrator ---------------------------------- */
 
 typedef struct orc_scene orc_scene;

@@ -72,3 +72,55 @@ def rays_toward(n, targets_pos, seed=2, dist=3.0):
     r = np.zeros((n, 8), np.float32)
     r[:, 0:3], r[:, 3], r[:, 4:7], r[:, 7] = o, 0.0, d, np.finfo(np.float32).max
     return r
+
+
+def random_rotation(rng):
+    """A uniformly random rotation matrix (QR of a Gaussian matrix)."""
+    q, r = np.linalg.qr(rng.normal(size=(3, 3)))
+    q = q * np.sign(np.diag(r))
+    if np.linalg.det(q) < 0:
+        q[:, 0] = -q[:, 0]
+    return q
+
+
+def instances(seed=0, count=8):
+    """Primitives for space tests: (pos, idx, rot9, scl, trn), alternating a
+    sphere and a triangle soup, the first untransformed. rot9 holds the
+    reference Matrix3x3 elements in storage order, applied as
+    world_i = sum_j rot9[3j+i] * scl_j * p_j + trn_i."""
+    rng = np.random.default_rng(seed)
+    sp, si = icosphere(3)
+    tp, ti = random_triangles(300, seed=seed + 2, extent=0.5)
+    out = []
+    for k in range(count):
+        pos, idx = (sp, si) if k % 2 == 0 else (tp, ti)
+        if k == 0:
+            out.append((pos, idx, None, None, None))
+            continue
+        rot = random_rotation(rng).astype(np.float32).reshape(-1)
+        scl = rng.uniform(0.3, 0.8, 3).astype(np.float32)
+        trn = rng.uniform(-3, 3, 3).astype(np.float32)
+        out.append((pos, idx, rot, scl, trn))
+    return out
+
+
+def flatten_instances(inst):
+    """World-space positions (float64 transform), indices, and the owning
+    primitive and local triangle of every flattened triangle."""
+    wp, wi, owner, local = [], [], [], []
+    off = 0
+    for k, (pos, idx, rot, scl, trn) in enumerate(inst):
+        p = pos.astype(np.float64)
+        if scl is not None:
+            p = p * scl.astype(np.float64)
+        if rot is not None:
+            p = p @ rot.astype(np.float64).reshape(3, 3)
+        if trn is not None:
+            p = p + trn.astype(np.float64)
+        wp.append(p)
+        wi.append(idx + off)
+        off += len(pos)
+        owner.append(np.full(len(idx), k, np.int32))
+        local.append(np.arange(len(idx), dtype=np.int32))
+    return (np.concatenate(wp).astype(np.float32), np.concatenate(wi).astype(np.int32),
+            np.concatenate(owner), np.concatenate(local))

@@ -93,6 +93,42 @@ def test_any_matches_oracle(gpu_ctx, ilm, orc, builder):
     m.close()
 
 
+def test_space_matches_oracle_basicspace(gpu_ctx, ilm, orc):
+    # One flattened world-space BVH against the BasicSpace restatement, which
+    # transforms every ray into every primitive's object space.
+    inst = meshgen.instances(seed=1)
+    space = orc.Space()
+    keep = []
+    sp = ilm.Space(gpu_ctx)
+    for pos, idx, rot, scl, trn in inst:
+        m = orc.Mesh(pos, idx)
+        m.build(10, 35)
+        keep.append(m)
+        space.add(m, rot, scl, trn)
+        sp.add(pos, idx, None, rot, scl, trn)
+    sp.build()
+    wp, _, _, _ = meshgen.flatten_instances(inst)
+    rays = np.concatenate([meshgen.rays_toward(30000, wp, seed=3, dist=6.0), meshgen.random_rays(10000, seed=4)])
+    n = len(rays)
+    rb = ilm.rays_to_device(gpu_ctx, rays)
+    hb, pb_, lb, ob = gpu_ctx.buffer(n * 16), gpu_ctx.buffer(n * 4), gpu_ctx.buffer(n * 4), gpu_ctx.buffer(n)
+    sp.mesh.trace_closest(rb, n, hb)
+    sp.resolve(hb, n, pb_, lb)
+    sp.mesh.trace_any(rb, n, ob)
+    gpu_ctx.sync()
+    h = hb.download(ilm.HIT_DTYPE, n)
+    gp, gl, occ = pb_.download(np.int32, n), lb.download(np.int32, n), ob.download(np.uint8, n)
+    p, tr, t = space.trace(rays, mode=0)
+    key_g = np.where(gp >= 0, gp * 100000 + gl, -1)
+    key_c = np.where(p >= 0, p * 100000 + tr, -1)
+    assert _compare_closest(key_g, h["t"], key_c, t, "space") >= 0.9999
+    pa, _, _ = space.trace(rays, mode=1)
+    assert (occ == (pa >= 0)).mean() >= 0.9999
+    for b in (rb, hb, pb_, lb, ob):
+        b.free()
+    sp.close()
+
+
 def test_empty_and_tiny_inputs(gpu_ctx, ilm):
     pos, idx = meshgen.quad([0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0])
     m = ilm.Mesh(gpu_ctx, pos, idx)

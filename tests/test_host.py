@@ -174,6 +174,34 @@ def test_kdtree_builders_match_oracle(ilm, orc, kind, builder):
     m.close()
 
 
+def test_space_flattening(ilm):
+    # ilm_space_build applies Transformation::Apply to every vertex and
+    # Transformation::Rotate (then normalisation) to every normal.
+    inst = meshgen.instances(seed=4, count=5)
+    sp = ilm.Space(None)
+    for k, (pos, idx, rot, scl, trn) in enumerate(inst):
+        nrm = ilm.vertex_normals(pos, idx)
+        assert sp.add(pos, idx, nrm, rot, scl, trn, material=k % 3, light=(0 if k == 4 else -1)) == k
+    sp.build()
+    wp_ref, wi_ref, owner, _ = meshgen.flatten_instances(inst)
+    wp, wi = sp.geometry()
+    assert np.array_equal(wi, wi_ref)
+    assert np.abs(wp - wp_ref).max() < 1e-5
+    nrm, tm, tl = sp.shading()
+    assert np.array_equal(tm, owner % 3) and np.array_equal(tl, np.where(owner == 4, 0, -1))
+    vo = 0
+    for pos, idx, rot, scl, trn in inst:
+        n = ilm.vertex_normals(pos, idx).astype(np.float64)
+        if scl is not None:
+            n = n * scl
+        if rot is not None:
+            n = n @ rot.astype(np.float64).reshape(3, 3)
+        n /= np.linalg.norm(n, axis=1, keepdims=True)
+        assert np.abs(nrm[vo:vo + len(pos)] - n).max() < 1e-5
+        vo += len(pos)
+    sp.close()
+
+
 def test_mesh_without_context_cannot_trace(ilm):
     pos, idx = meshgen.icosphere(1)
     m = ilm.Mesh(None, pos, idx)

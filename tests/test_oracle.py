@@ -313,6 +313,33 @@ def test_kdtree_segments(orc):
     assert m.trace(ray, mode=0, accel=2)[0][0] == m.trace(ray, mode=0, accel=0)[0][0]
 
 
+# ---------------------------------------------------------------- spaces
+def test_space_matches_flattened_brute_force(orc):
+    # BasicSpace transforms each ray into each primitive's object space
+    # (Ray::ApplyInverse); brute force over the primitives transformed to
+    # world space in float64 must find the same (primitive, triangle).
+    inst = meshgen.instances(seed=1)
+    space = orc.Space()
+    meshes = []
+    for pos, idx, rot, scl, trn in inst:
+        m = orc.Mesh(pos, idx)
+        m.build(10, 35)
+        meshes.append(m)
+        space.add(m, rot, scl, trn)
+    wp, wi, owner, local = meshgen.flatten_instances(inst)
+    rays = meshgen.rays_toward(20000, wp, seed=3, dist=6.0)
+    p, tr, t = space.trace(rays, mode=0)
+    pb, tb = orc.Mesh(wp, wi).trace(rays, mode=0, accel=0)
+    assert ((p >= 0) == (pb >= 0)).all()
+    both = (p >= 0) & (pb >= 0)
+    same = both & (owner[np.maximum(pb, 0)] == p) & (local[np.maximum(pb, 0)] == tr)
+    assert same.sum() / both.sum() > 0.9995
+    assert np.abs(t[both] - tb[both]).max() < 1e-4 * max(1.0, float(tb[both].max()))
+    pa, _, _ = space.trace(rays, mode=1)
+    assert ((pa >= 0) == (pb >= 0)).all()
+    assert both.mean() > 0.5
+
+
 # ---------------------------------------------------------------- integrator
 def _closed_box(albedo, le):
     """Unit cube seen from inside: 6 walls, normals inward, each wall a light."""

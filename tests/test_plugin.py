@@ -131,3 +131,48 @@ def test_plugin_render_matches_binding_and_oracle(gpu_ctx, ilm, orc):
     c = orc.Scene(pos, nrm, idx, tm, tl, mats, lights).render(cam.as_list(), W, H, spp, max_depth=depth,
                                                               rr_depth=4, seed=seed)
     _same_sample_agreement(img * spp, c, spp, "plug-in room 64x64 16spp")
+
+
+@pytest.mark.gpu
+def test_plugin_space_matches_oracle(gpu_ctx, orc):
+    # GpuSpace (flattened primitives, one BVH) against the BasicSpace
+    # restatement (rays transformed into each primitive's object space).
+    inst = meshgen.instances(seed=1)
+    wp, _, _, _ = meshgen.flatten_instances(inst)
+    rays = meshgen.rays_toward(8000, wp, seed=3, dist=6.0)
+    n, nper = len(rays), 200
+    with tempfile.TemporaryDirectory() as tmp:
+        inp, out = os.path.join(tmp, "in.bin"), os.path.join(tmp, "out.bin")
+        with open(inp, "wb") as f:
+            np.array([len(inst), n, nper], np.int64).tofile(f)
+            for pos, idx, rot, scl, trn in inst:
+                np.array([len(pos), len(idx)], np.int64).tofile(f)
+                xf = np.concatenate([np.eye(3, dtype=np.float32).reshape(-1) if rot is None else rot,
+                                     np.ones(3, np.float32) if scl is None else scl,
+                                     np.zeros(3, np.float32) if trn is None else trn]).astype(np.float32)
+                xf.tofile(f)
+                np.ascontiguousarray(pos, np.float32).tofile(f)
+                np.ascontiguousarray(idx, np.int32).tofile(f)
+            np.ascontiguousarray(rays, np.float32).tofile(f)
+        r = _run("space", inp, out, "")
+        assert r.returncode == 0, r.stderr
+        raw = np.fromfile(out, np.uint8)
+    dt = np.dtype([("t", "<f4"), ("prim", "<i4"), ("tri", "<i4")])
+    batch = raw[:12 * n].view(dt)
+    per = raw[12 * n:12 * (n + nper)].view(dt)
+    occ = raw[12 * (n + nper):]
+    space = orc.Space()
+    keep = []
+    for pos, idx, rot, scl, trn in inst:
+        m = orc.Mesh(pos, idx)
+        m.build(10, 35)
+        keep.append(m)
+        space.add(m, rot, scl, trn)
+    p, tr, t = space.trace(rays, mode=0)
+    same = (batch["prim"] == p) & (batch["tri"] == tr)
+    near = (batch["prim"] >= 0) & (p >= 0) & (np.abs(batch["t"] - t) <= 1e-5)
+    print(f"plug-in space agreement {(same | near).mean():.6f}", flush=True)
+    assert (same | near).mean() >= 0.9995
+    assert np.array_equal(per["prim"], batch["prim"][:nper]) and np.array_equal(per["tri"], batch["tri"][:nper])
+    pa, _, _ = space.trace(rays[:nper], mode=1)
+    assert ((occ != 0) == (pa >= 0)).mean() >= 0.99
