@@ -270,7 +270,7 @@ def main():
     ap.add_argument("--cache-only", action="store_true", help="run only the time-to-relRMSE leg")
     ap.add_argument("--cache-width", type=int, default=1920)
     ap.add_argument("--cache-height", type=int, default=1080)
-    ap.add_argument("--cache-ref-spp", type=int, default=512, help="spp of each of the two reference renders")
+    ap.add_argument("--cache-ref-spp", type=int, default=256, help="spp of each of the two reference renders")
     ap.add_argument("--cache-ref-chunk", type=int, default=64)
     ap.add_argument("--cache-target", type=float, default=0.05)
     ap.add_argument("--cache-max-spp", type=int, default=1024)

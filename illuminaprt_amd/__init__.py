@@ -21,6 +21,7 @@ ILM_BUILDER_SAH = 0
 ILM_BUILDER_REFERENCE = 1
 ILM_BUILDER_KDTREE = 2
 ILM_BUILDER_KDTREE_EX = 3
+ILM_BUILDER_LBVH = 4
 ILM_SPAWN_SHADOW = 0
 ILM_SPAWN_COSINE = 1
 RAY_BYTES = 32

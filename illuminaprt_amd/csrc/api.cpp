@@ -227,6 +227,10 @@ int ilm_bvh_build(ilm_mesh* m, const ilm_bvh_params* params) {
     if (params) p = *params;
     if (p.width == 0) p.width = 4;
     if (p.width != 4) return fail(ILM_ERR_INVALID, "this build traverses BVH4 only");
+    if (p.builder == ILM_BUILDER_LBVH) {
+        if (!m->ctx) return fail(ILM_ERR_STATE, "the GPU builder needs a device context");
+        return ilm::build_lbvh(m, p.max_leaf > 0 ? p.max_leaf : 4);
+    }
     auto t0 = std::chrono::steady_clock::now();
     ilm::WideBvh w;
     try {

@@ -67,6 +67,9 @@ void camera_frame(const ilm_camera& c, float out[12]);
 // device memory (*n_dev, at most `capacity`), without a host round trip.
 int trace_queue(ilm_mesh* m, const ilm_ray* rays, const unsigned long long* n_dev, int64_t capacity, ilm_hit* hits,
                 uint8_t* occ, hipStream_t stream);
+// Builds m's BVH4 on its device (linear BVH + collapse); fills the device
+// arrays, info and host copies as ilm_bvh_build does.
+int build_lbvh(ilm_mesh* m, int max_leaf);
 // Cache inference over a query queue whose length lives in device memory.
 int nrc_infer_queue(ilm_nrc* r, const float* queries, const unsigned long long* n_dev, int64_t capacity, float* out,
                     hipStream_t stream);

@@ -118,6 +118,11 @@ int ilm_mesh_destroy(ilm_mesh* mesh);
  * root cell's exit: KDTreeMeshEx.cpp:258-259,371-372). */
 #define ILM_BUILDER_KDTREE 2
 #define ILM_BUILDER_KDTREE_EX 3
+/* Built on the device: a linear BVH over 63-bit Morton codes (Karras 2012),
+ * refitted and collapsed to BVH4 on the GPU; max_leaf = largest subtree kept
+ * as one leaf (default 4). Needs a mesh created with a context; build_ms is
+ * the device time. Faster to build, slower to trace than ILM_BUILDER_SAH. */
+#define ILM_BUILDER_LBVH 4
 
 typedef struct {
     int32_t builder;     /* ILM_BUILDER_* */
