@@ -116,6 +116,64 @@ def path_tracing_room(ctx, ilm, workloads, args):
     return out
 
 
+def _trace_rate(ctx, mesh, buf, n, steps, warmup, hits):
+    for _ in range(warmup):
+        mesh.trace_closest(buf, n, hits)
+    _, per = time_launches(ctx, lambda: mesh.trace_closest(buf, n, hits), steps)
+    ms = float(np.mean(per))
+    return round(n / ms / 1e3, 1), round(ms, 4)
+
+
+def lbvh_leg(ctx, ilm, pos, idx, rs, sah_info, args):
+    """The device builder on the same 1,334,720 triangles: build time against
+    the host SAH build, and the incoherent trace rate on its tree."""
+    m = ilm.Mesh(ctx, pos, idx)
+    builds = []
+    for _ in range(3):
+        builds.append(m.build(builder=ilm.ILM_BUILDER_LBVH)["build_ms"])
+    info = m.info()
+    hits = ctx.buffer(rs.n * HIT_BYTES)
+    rate, ms = _trace_rate(ctx, m, rs.incoherent, rs.n, max(3, args.steps // 4), args.warmup, hits)
+    hits.free()
+    m.close()
+    out = {"device_lbvh_build_ms": round(min(builds), 2), "host_sah_build_ms": round(sah_info["build_ms"], 1),
+           "lbvh_incoherent_mrays_s": rate, "lbvh_kernel_ms": ms, "lbvh_sah_cost": round(info["sah_cost"], 2),
+           "sah_cost": round(sah_info["sah_cost"], 2), "lbvh_nodes": info["nodes"]}
+    log(f"bvh build: {out}")
+    return out
+
+
+def large_scene_leg(ctx, ilm, workloads, args):
+    """Two displaced level-9 icospheres and 50,000 boxes (11,085,760
+    triangles), one 3840 x 2160 camera: build times (device LBVH, host SAH),
+    HBM footprint and primary / incoherent trace rates on both trees."""
+    t0 = time.perf_counter()
+    pos, idx = workloads.large_scene()
+    gen_s = time.perf_counter() - t0
+    W, H = 3840, 2160
+    n = W * H
+    out = {"workload": f"{len(idx)} tris, {W}x{H} camera rays and their cosine bounces", "triangles": int(len(idx)),
+           "rays_per_set": n, "generate_s": round(gen_s, 1)}
+    light = np.array([-0.5, 4.0, -0.5, 1.0, 0.0, 0.0, 0.0, 0.0, 1.0], np.float32)
+    for name, builder in (("lbvh", ilm.ILM_BUILDER_LBVH), ("sah", ilm.ILM_BUILDER_SAH)):
+        m = ilm.Mesh(ctx, pos, idx)
+        info = m.build(builder=builder)
+        rs = workloads.intersection_rays(ctx, m, W, H, cam=workloads.large_camera(W / H), light=light)
+        hits = ctx.buffer(n * HIT_BYTES)
+        steps = max(3, args.steps // 4)
+        p_rate, _ = _trace_rate(ctx, m, rs.primary, n, steps, args.warmup, hits)
+        i_rate, i_ms = _trace_rate(ctx, m, rs.incoherent, n, steps, args.warmup, hits)
+        out[name] = {"build_ms": round(info["build_ms"], 1), "nodes": info["nodes"],
+                     "bvh_bytes": info["node_bytes"] + info["tri_bytes"], "sah_cost": round(info["sah_cost"], 2),
+                     "primary_mrays_s": p_rate, "incoherent_mrays_s": i_rate, "incoherent_kernel_ms": i_ms}
+        out["ray_bytes"] = n * (3 * RAY_BYTES + 2 * HIT_BYTES)
+        hits.free()
+        rs.free()
+        m.close()
+        log(f"large scene {name}: {out[name]}")
+    return out
+
+
 MLP_FLOP_PER_QUERY = 2 * (96 * 64 + 4 * 64 * 64 + 64 * 16)  # padded network as executed
 MFMA_BF16_PEAK_TFLOPS = 2500.0
 
@@ -267,6 +325,7 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run the plain copy kernel over MIB mebibytes (bandwidth / counter calibration)")
     ap.add_argument("--no-cache", action="store_true", help="skip the time-to-relRMSE leg")
+    ap.add_argument("--no-large", action="store_true", help="skip the 11M-triangle leg")
     ap.add_argument("--cache-only", action="store_true", help="run only the time-to-relRMSE leg")
     ap.add_argument("--cache-width", type=int, default=1920)
     ap.add_argument("--cache-height", type=int, default=1080)
@@ -353,8 +412,10 @@ def main():
         rays_host = rs.incoherent.download(np.float32, 8 * n).reshape(n, 8)
         cpu = cpu_baseline(pos, idx, rays_host, args.cpu_budget)
         log(f"cpu baseline: {cpu}")
+    build = None if args.only else lbvh_leg(ctx, ilm, pos, idx, rs, info, args)
     room = None if args.only else path_tracing_room(ctx, ilm, workloads, args)
     cache = None if (args.only or args.no_cache) else cache_convergence(ctx, ilm, workloads, args)
+    large = None if (args.only or args.no_large) else large_scene_leg(ctx, ilm, workloads, args)
     out = {
         "metric": METRIC,
         "value": round(inc["wall_mrays_s"], 3),
@@ -392,6 +453,8 @@ def main():
         "cpu_baseline": cpu,
         "path_tracing_room": room,
         "time_to_rel_rmse": cache,
+        "bvh_build": build,
+        "large_scene": large,
     }
     print(json.dumps(out), flush=True)
     for b in (hits, occ, visits):

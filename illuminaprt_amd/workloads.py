@@ -45,9 +45,27 @@ class RaySets:
             b.free()
 
 
-def intersection_rays(ctx: Context, mesh: Mesh, width=2048, height=2048, seed=1234) -> RaySets:
+def large_scene(level=9, nboxes=50000, noise_seed=7, box_seed=11):
+    """Two displaced level-9 icospheres (5,242,880 triangles each, centres at
+    x = -1.2 and x = +1.2) and 50,000 boxes (600,000 triangles) in a radius-4
+    ball: 11,085,760 triangles."""
+    sp, si = icosphere(level, 0.1, 4, noise_seed)
+    sp2, si2 = icosphere(level, 0.1, 4, noise_seed + 1)
+    a = sp + np.float32([-1.2, 0.0, 0.0])
+    b = sp2 + np.float32([1.2, 0.0, 0.0])
+    bp, bi = boxes(nboxes, 0.01, 0.1, 4.0, box_seed, vertex_offset=2 * len(sp))
+    return np.concatenate([a, b, bp]), np.concatenate([si, si2 + len(sp), bi])
+
+
+def large_camera(aspect=16 / 9):
+    return Camera.make((0.0, 0.3, -6.5), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 50.0, aspect)
+
+
+def intersection_rays(ctx: Context, mesh: Mesh, width=2048, height=2048, seed=1234, cam=None,
+                      light=None) -> RaySets:
     n = width * height
-    cam = intersection_camera()
+    cam = cam if cam is not None else intersection_camera()
+    light = light if light is not None else C2_LIGHT
     r0 = ctx.buffer(n * RAY_BYTES)
     h0 = ctx.buffer(n * HIT_BYTES)
     ctx.raygen_camera(cam, width, height, r0)
@@ -70,7 +88,7 @@ def intersection_rays(ctx: Context, mesh: Mesh, width=2048, height=2048, seed=12
         prim.upload(rays)
         hits.upload(h)
     shadow = ctx.buffer(n * RAY_BYTES)
-    mesh.spawn(prim, hits, n, ILM_SPAWN_SHADOW, shadow, light=C2_LIGHT, seed=seed + 1)
+    mesh.spawn(prim, hits, n, ILM_SPAWN_SHADOW, shadow, light=light, seed=seed + 1)
     inco = ctx.buffer(n * RAY_BYTES)
     mesh.spawn(prim, hits, n, ILM_SPAWN_COSINE, inco, eps=1e-4, seed=seed + 2)
     ctx.sync()
