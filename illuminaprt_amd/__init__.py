@@ -22,6 +22,8 @@ ILM_BUILDER_REFERENCE = 1
 ILM_BUILDER_KDTREE = 2
 ILM_BUILDER_KDTREE_EX = 3
 ILM_BUILDER_LBVH = 4
+ILM_TONE_GLOBAL, ILM_TONE_DRAGO = 0, 1
+ILM_IMAGE_PFM, ILM_IMAGE_PPM = 0, 1
 ILM_SPAWN_SHADOW = 0
 ILM_SPAWN_COSINE = 1
 RAY_BYTES = 32
@@ -127,6 +129,9 @@ SIGNATURES = {
     "ilm_space_shading": (C.c_int, [_vp, _vp, _vp, _vp]),
     "ilm_space_resolve": (C.c_int, [_vp, _vp, _i64, _vp, _vp, _vp]),
     "ilm_space_geometry": (C.c_int, [_vp, _vp, _vp]),
+    "ilm_image_resolve": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_double, C.c_int, _vp, _vp]),
+    "ilm_image_tonemap": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _vp]),
+    "ilm_image_write": (C.c_int, [C.c_char_p, _vp, C.c_int, C.c_int, C.c_int]),
     "ilm_scene_create": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.c_int, _vp, C.c_int, C.POINTER(_vp)]),
     "ilm_scene_destroy": (C.c_int, [_vp]),
     "ilm_render": (C.c_int, [_vp, C.POINTER(RenderParams), _vp, C.POINTER(RenderStats)]),
@@ -470,6 +475,21 @@ class Nrc:
 def rays_to_device(ctx: Context, rays: np.ndarray) -> DeviceBuffer:
     rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
     return ctx.to_device(rays)
+
+
+def image_resolve(ctx: Context, accum: DeviceBuffer, width: int, height: int, samples: int, out: DeviceBuffer,
+                  flip=False):
+    check(load().ilm_image_resolve(ctx.h, accum.ptr, width, height, 1.0 / samples, int(flip), out.ptr, None))
+
+
+def image_tonemap(ctx: Context, src: DeviceBuffer, width: int, height: int, op: int, out: DeviceBuffer):
+    check(load().ilm_image_tonemap(ctx.h, src.ptr, width, height, op, out.ptr, None))
+
+
+def image_write(path: str, rgb: np.ndarray, fmt=ILM_IMAGE_PFM):
+    rgb = np.ascontiguousarray(rgb, np.float32)
+    h, w = rgb.shape[:2]
+    check(load().ilm_image_write(path.encode(), _ptr(rgb), w, h, fmt))
 
 
 def icosphere(level: int, amplitude=0.0, octaves=0, seed=0):

@@ -103,17 +103,33 @@ __device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
-// Feature f of a query (encoding above); p is the position normalised to [0,1].
-__device__ __forceinline__ float feature(const float* q, const float* pn, int f) {
-    if (f < 72) {
-        int d = f / 24, j = f % 24, fr = j >> 1;
-        float x = pn[d] * (float)(1 << fr);
-        float s, c;
-        sincospif(x, &s, &c);
-        return (j & 1) ? c : s;
+// The 24 features lane group g holds for one sample, as three 8-feature
+// chunks: chunk c holds features 32c + 8g .. 32c + 8g + 7. A chunk below 72
+// is four (sin, cos) pairs of consecutive frequencies of one coordinate; one
+// sincospi gives the first pair and the double-angle identities
+// sin 2a = 2 sin a cos a, cos 2a = cos^2 a - sin^2 a the next three
+// (three doublings add a few float ulps, far below bf16 rounding).
+__device__ __forceinline__ void lane_features(const float* q, const float* pn, int g, bf16x8 x[3]) {
+    for (int c = 0; c < 3; ++c) {
+        const int fb = 32 * c + 8 * g;
+        if (fb < 72) {
+            const int d = fb / 24, fr = (fb % 24) >> 1;
+            float sn, cs;
+            sincospif(pn[d] * (float)(1 << fr), &sn, &cs);
+            for (int k = 0; k < 4; ++k) {
+                x[c][2 * k] = (__bf16)sn;
+                x[c][2 * k + 1] = (__bf16)cs;
+                float s2 = 2.0f * sn * cs, c2 = cs * cs - sn * sn;
+                sn = s2;
+                cs = c2;
+            }
+        } else {
+            for (int e = 0; e < 8; ++e) {
+                const int f = fb + e;
+                x[c][e] = (__bf16)(f < 83 ? q[3 + (f - 72)] : 0.0f);  // dir(2), normal(2), roughness, kd(3), ks(3)
+            }
+        }
     }
-    if (f < 83) return q[3 + (f - 72)];  // dir(2), normal(2), roughness, kd(3), ks(3)
-    return 0.f;
 }
 
 struct Bounds { float lo[3], inv[3]; };
@@ -134,6 +150,12 @@ __device__ __forceinline__ bf16x8 relu_pack(f32x4 a, f32x4 b) {
     r[6] = (__bf16)fmaxf(b[2], 0.f); r[7] = (__bf16)fmaxf(b[3], 0.f);
     return r;
 }
+
+// Makes `off` depend on `dep` in a way the compiler cannot see through. Weight
+// fragments addressed with `off` are then read from LDS only after `dep` (the
+// previous layer's result) exists, so one layer's fragments are live in
+// registers at a time instead of the whole network's.
+__device__ __forceinline__ void order_after(int& off, float dep) { asm volatile("" : "+v"(off) : "v"(dep)); }
 
 __device__ __forceinline__ void lds_copy(__bf16* dst, const __bf16* src, int elems) {
     const uint4* s = reinterpret_cast<const uint4*>(src);
@@ -162,43 +184,49 @@ __global__ void __launch_bounds__(256) k_infer(InferArgs a) {
     const int64_t n = a.n_dev ? (int64_t)min((unsigned long long)a.n, *a.n_dev) : a.n;
     const int64_t step = (int64_t)gridDim.x * 4 * 16 * kTiles;
     for (int64_t base = ((int64_t)blockIdx.x * 4 + wave) * 16 * kTiles; base < n; base += step) {
+        // An opaque per-iteration copy of the lane's fragment offset keeps the
+        // compiler from hoisting all 46 weight fragments out of the loop into
+        // registers (184 VGPRs, one wave per SIMD); they are re-read from LDS.
+        int lo8 = lane * 8;
+        asm volatile("" : "+v"(lo8));
         bf16x8 x[kTiles][3];
         for (int s = 0; s < kTiles; ++s) {
             int64_t smp = min(base + 16 * s + m, n - 1);
             float q[kQuery], pn[3];
             load_query(a.queries, smp, a.bd, q, pn);
-            for (int c = 0; c < 3; ++c)
-                for (int e = 0; e < 8; ++e) x[s][c][e] = (__bf16)feature(q, pn, 32 * c + 8 * g + e);
+            lane_features(q, pn, g, x[s]);
         }
         f32x4 acc[4][kTiles];
         // layer 0: 96 -> 64
         for (int t = 0; t < 4; ++t) {
             for (int s = 0; s < kTiles; ++s) acc[t][s] = f32x4{0.f, 0.f, 0.f, 0.f};
             for (int c = 0; c < 3; ++c) {
-                bf16x8 A = *reinterpret_cast<const bf16x8*>(w + kF0 + ((t * 3 + c) * 64 + lane) * 8);
+                bf16x8 A = *reinterpret_cast<const bf16x8*>(w + kF0 + (t * 3 + c) * 512 + lo8);
                 for (int s = 0; s < kTiles; ++s) acc[t][s] = mfma32(A, x[s][c], acc[t][s]);
             }
         }
         bf16x8 h[kTiles][2];
         for (int s = 0; s < kTiles; ++s)
             for (int c = 0; c < 2; ++c) h[s][c] = relu_pack(acc[2 * c][s], acc[2 * c + 1][s]);
+        order_after(lo8, acc[3][kTiles - 1][3]);
         // hidden layers 1..4: 64 -> 64
         for (int l = 0; l < 4; ++l) {
             for (int t = 0; t < 4; ++t) {
                 for (int s = 0; s < kTiles; ++s) acc[t][s] = f32x4{0.f, 0.f, 0.f, 0.f};
                 for (int c = 0; c < 2; ++c) {
-                    bf16x8 A = *reinterpret_cast<const bf16x8*>(w + kFh + l * 8 * 512 + ((t * 2 + c) * 64 + lane) * 8);
+                    bf16x8 A = *reinterpret_cast<const bf16x8*>(w + kFh + l * 8 * 512 + (t * 2 + c) * 512 + lo8);
                     for (int s = 0; s < kTiles; ++s) acc[t][s] = mfma32(A, h[s][c], acc[t][s]);
                 }
             }
             for (int s = 0; s < kTiles; ++s)
                 for (int c = 0; c < 2; ++c) h[s][c] = relu_pack(acc[2 * c][s], acc[2 * c + 1][s]);
+            order_after(lo8, acc[3][kTiles - 1][3]);
         }
         // output layer: 64 -> 16
         for (int s = 0; s < kTiles; ++s) {
             f32x4 o = f32x4{0.f, 0.f, 0.f, 0.f};
             for (int c = 0; c < 2; ++c) {
-                bf16x8 A = *reinterpret_cast<const bf16x8*>(w + kF5 + (c * 64 + lane) * 8);
+                bf16x8 A = *reinterpret_cast<const bf16x8*>(w + kF5 + c * 512 + lo8);
                 o = mfma32(A, h[s][c], o);
             }
             int64_t smp = base + 16 * s + m;
@@ -237,42 +265,48 @@ __global__ void __launch_bounds__(256) k_train_fb(TrainArgs a) {
     const int64_t n = a.n, ld = a.ld;
     const int64_t step = (int64_t)gridDim.x * 4 * 16;
     for (int64_t base = ((int64_t)blockIdx.x * 4 + wave) * 16; base < n; base += step) {
+        int lo8 = lane * 8, lo4 = lane * 4;  // opaque offsets: weights stay in LDS (see k_infer)
+        asm volatile("" : "+v"(lo8), "+v"(lo4));
         const int64_t smp = base + m;
         const bool valid = smp < n;
         const int64_t sl = valid ? smp : n - 1;
         float q[kQuery], pn[3];
         load_query(a.queries, sl, a.bd, q, pn);
         bf16x8 x[3];
-        for (int c = 0; c < 3; ++c)
-            for (int e = 0; e < 8; ++e) {
-                int f = 32 * c + 8 * g + e;
-                x[c][e] = (__bf16)feature(q, pn, f);
-                if (valid) a.H[hoff(0, ld) + (int64_t)f * ld + smp] = x[c][e];
-            }
+        lane_features(q, pn, g, x);
+        // Stores address a uniform column base plus one 32-bit lane offset
+        // (the lane's sample, shifted by its group's 4 or 8 features), so
+        // that they need no 64-bit address per value.
+        const uint32_t off4 = (uint32_t)(4 * g * ld + smp), off8 = (uint32_t)(8 * g * ld + smp);
+        if (valid)
+            for (int c = 0; c < 3; ++c)
+                for (int e = 0; e < 8; ++e) (a.H + hoff(0, ld) + (int64_t)(32 * c + e) * ld)[off8] = x[c][e];
         bf16x8 h[6][2];  // h[l] = output of layer l-1 (layers 1..5), as B chunks
         f32x4 acc[4];
         for (int t = 0; t < 4; ++t) {
             acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
             for (int c = 0; c < 3; ++c)
-                acc[t] = mfma32(*reinterpret_cast<const bf16x8*>(wf + kF0 + ((t * 3 + c) * 64 + lane) * 8), x[c], acc[t]);
+                acc[t] = mfma32(*reinterpret_cast<const bf16x8*>(wf + kF0 + (t * 3 + c) * 512 + lo8), x[c], acc[t]);
         }
         for (int c = 0; c < 2; ++c) h[1][c] = relu_pack(acc[2 * c], acc[2 * c + 1]);
+        order_after(lo8, acc[3][3]);
         for (int l = 1; l <= 4; ++l) {
             for (int t = 0; t < 4; ++t) {
                 acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
                 for (int c = 0; c < 2; ++c)
-                    acc[t] = mfma32(*reinterpret_cast<const bf16x8*>(wf + kFh + (l - 1) * 8 * 512 + ((t * 2 + c) * 64 + lane) * 8),
+                    acc[t] = mfma32(*reinterpret_cast<const bf16x8*>(wf + kFh + (l - 1) * 8 * 512 + (t * 2 + c) * 512 + lo8),
                                     h[l][c], acc[t]);
             }
             for (int c = 0; c < 2; ++c) h[l + 1][c] = relu_pack(acc[2 * c], acc[2 * c + 1]);
+            order_after(lo8, acc[3][3]);
         }
         // store H1..H5 (feature of K slot (c, g, e) is perm(c, g, e))
         if (valid)
             for (int l = 1; l <= 5; ++l)
                 for (int c = 0; c < 2; ++c)
-                    for (int e = 0; e < 8; ++e) a.H[hoff(l, ld) + (int64_t)perm(c, g, e) * ld + smp] = h[l][c][e];
+                    for (int e = 0; e < 8; ++e) (a.H + hoff(l, ld) + (int64_t)perm(c, 0, e) * ld)[off4] = h[l][c][e];
         f32x4 o = f32x4{0.f, 0.f, 0.f, 0.f};
-        for (int c = 0; c < 2; ++c) o = mfma32(*reinterpret_cast<const bf16x8*>(wf + kF5 + (c * 64 + lane) * 8), h[5][c], o);
+        for (int c = 0; c < 2; ++c) o = mfma32(*reinterpret_cast<const bf16x8*>(wf + kF5 + c * 512 + lo8), h[5][c], o);
         // relative L2 loss on rgb: (y - t)^2 / (sg(y)^2 + 0.01); lanes g == 0 hold outputs 0..3
         bf16x4 g6;
         float lsum = 0.f;
@@ -289,11 +323,12 @@ __global__ void __launch_bounds__(256) k_train_fb(TrainArgs a) {
         }
         if (a.loss && g == 0 && valid) a.loss[smp] = lsum;
         if (valid)
-            for (int r = 0; r < 4; ++r) a.G[goff(6, ld) + (int64_t)(4 * g + r) * ld + smp] = g6[r];
+            for (int r = 0; r < 4; ++r) (a.G + goff(6, ld) + (int64_t)r * ld)[off4] = g6[r];
         // backward through layer 5 (x16 fragments): dH5 = W5^T g6
         f32x4 d[4];
+        order_after(lo4, (float)g6[0]);
         for (int t = 0; t < 4; ++t)
-            d[t] = mfma16(*reinterpret_cast<const bf16x4*>(wb + kB5 + (t * 64 + lane) * 4), g6, f32x4{0.f, 0.f, 0.f, 0.f});
+            d[t] = mfma16(*reinterpret_cast<const bf16x4*>(wb + kB5 + t * 256 + lo4), g6, f32x4{0.f, 0.f, 0.f, 0.f});
         for (int l = 5; l >= 1; --l) {
             // G_l = dH_l * relu'(H_l), back in B-chunk form
             bf16x8 gl[2];
@@ -304,13 +339,14 @@ __global__ void __launch_bounds__(256) k_train_fb(TrainArgs a) {
                 }
             if (valid)
                 for (int c = 0; c < 2; ++c)
-                    for (int e = 0; e < 8; ++e) a.G[goff(l, ld) + (int64_t)perm(c, g, e) * ld + smp] = gl[c][e];
+                    for (int e = 0; e < 8; ++e) (a.G + goff(l, ld) + (int64_t)perm(c, 0, e) * ld)[off4] = gl[c][e];
             if (l == 1) break;
+            order_after(lo8, d[3][3]);
             // dH_{l-1} = W_{l-1}^T G_l  (layer index of the weights is l-1, 1..4)
             for (int t = 0; t < 4; ++t) {
                 d[t] = f32x4{0.f, 0.f, 0.f, 0.f};
                 for (int c = 0; c < 2; ++c)
-                    d[t] = mfma32(*reinterpret_cast<const bf16x8*>(wb + kBh + (l - 2) * 8 * 512 + ((t * 2 + c) * 64 + lane) * 8),
+                    d[t] = mfma32(*reinterpret_cast<const bf16x8*>(wb + kBh + (l - 2) * 8 * 512 + (t * 2 + c) * 512 + lo8),
                                   gl[c], d[t]);
             }
         }
@@ -643,6 +679,8 @@ int ilm_nrc_infer(ilm_nrc* r, const float* queries, int64_t n, float* out, void*
 
 int ilm_nrc_train_step(ilm_nrc* r, const float* queries, const float* targets, int64_t n, float* loss, void* stream) {
     if (!r || n <= 0 || !queries || !targets) return ilm::fail(ILM_ERR_INVALID, "bad arguments");
+    // the training kernel addresses activations with 32-bit lane offsets
+    if (n > (int64_t(1) << 26)) return ilm::fail(ILM_ERR_INVALID, "training batch above 2^26 samples");
     const int64_t chunk = 512;
     const int64_t dw_blocks = (n + chunk - 1) / chunk;
     int rc = ensure_train(r, n, dw_blocks);

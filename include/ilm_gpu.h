@@ -335,6 +335,28 @@ typedef struct {
 int ilm_render(ilm_scene* scene, const ilm_render_params* params, double* accum,
                ilm_render_stats* stats);
 
+/* ---------------------------------------------------------------- frames and images */
+
+/* Mean radiance per pixel from an FP64 accumulator (width*height*3 doubles,
+ * dev): out (dev, width*height*3 floats) = accum * inv_samples, as
+ * AccumulationBuffer::Apply averages frames (Postproc/AccumulationBuffer.h:59-88).
+ * flip = 1 writes frame pixel (x, y) to (width-1-x, height-1-y), the mirrored
+ * order ImageDevice writes to its image (Device/ImageDevice.cpp:137-148). */
+int ilm_image_resolve(ilm_ctx* ctx, const double* accum, int width, int height, double inv_samples, int flip,
+                      float* out, void* stream);
+
+#define ILM_TONE_GLOBAL 0  /* GlobalTone: c / (c + 1) per channel (Postproc/GlobalTone.h:37-53) */
+#define ILM_TONE_DRAGO 1   /* DragoTone with its fixed bias 0.8 and scale 95%, per-channel maximum
+                              over the image, clamped to [0, 1] (Postproc/DragoTone.h:37-80) */
+/* Tone maps width*height RGB floats (dev) into out (dev; may equal in). */
+int ilm_image_tonemap(ilm_ctx* ctx, const float* in, int width, int height, int op, float* out, void* stream);
+
+#define ILM_IMAGE_PFM 0  /* "PF w h -1 " + little-endian RGB floats (Image/ImagePFM.h:89-111) */
+#define ILM_IMAGE_PPM 1  /* "P6 w h 255 " + bytes min(255, (int)(256 v)) (Image/ImagePPM.h:85-110);
+                            negative values write 0 */
+/* Writes host RGB floats (rows in the given order) to path. */
+int ilm_image_write(const char* path, const float* rgb, int width, int height, int format);
+
 /* ---------------------------------------------------------------- neural radiance cache */
 
 /* A small MLP (96 -> 5 x 64 ReLU -> 16, bf16 weights and activations, FP32
@@ -375,7 +397,8 @@ int ilm_nrc_infer(ilm_nrc* nrc, const float* queries, int64_t n, float* out, voi
 /* One Adam step on n (query, target) pairs (dev). targets = n x 4 floats
  * (rgb, weight). Loss: mean over samples and channels of
  * weight * (y - t)^2 / (y^2 + 0.01), the denominator held constant in the
- * gradient. loss (dev, n floats) receives per-sample losses, or NULL. */
+ * gradient. loss (dev, n floats) receives per-sample losses, or NULL.
+ * n is at most 2^26. */
 int ilm_nrc_train_step(ilm_nrc* nrc, const float* queries, const float* targets, int64_t n,
                        float* loss, void* stream);
 
