@@ -8,7 +8,8 @@ sets are timed the same way and reported beside it.
 
 Prints one JSON line. `value` = incoherent Mrays/s. `roofline` prices the
 traversal kernel against HBM bandwidth with algorithmic bytes per ray
-= 32 (ray) + 16 (hit) + N_node * 112 + N_tri * 48, where N_node and N_tri are
+= 32 (ray) + 16 (hit) + N_node * S_node + N_tri * 48 (S_node = 112 or 64 bytes by
+node format), where N_node and N_tri are
 the mean node visits and triangle tests of the same traversal over the same
 rays (counted by the kernel's statistics variant). `cpu_baseline` times the
 oracle (the C++ restatement of the reference BVHMesh build + recursive
@@ -27,7 +28,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Mrays/s incoherent @1M tris; time-to-0.05 relRMSE, neural cache vs plain PT"
 HBM_PEAK_GBS = 8000.0
-NODE_BYTES, TRI_BYTES, RAY_BYTES, HIT_BYTES = 112, 48, 32, 16
+TRI_BYTES, RAY_BYTES, HIT_BYTES = 48, 32, 16
 
 
 def log(*a):
@@ -326,6 +327,8 @@ def main():
                     help="only run the plain copy kernel over MIB mebibytes (bandwidth / counter calibration)")
     ap.add_argument("--no-cache", action="store_true", help="skip the time-to-relRMSE leg")
     ap.add_argument("--no-large", action="store_true", help="skip the 11M-triangle leg")
+    ap.add_argument("--nodes", choices=["float", "quantized"], default="float",
+                    help="BVH4 node format of the traced trees")
     ap.add_argument("--cache-only", action="store_true", help="run only the time-to-relRMSE leg")
     ap.add_argument("--cache-width", type=int, default=1920)
     ap.add_argument("--cache-height", type=int, default=1080)
@@ -361,9 +364,10 @@ def main():
         ctx.close()
         return
     t0 = time.perf_counter()
+    nodes_fmt = ilm.ILM_NODES_QUANTIZED if args.nodes == "quantized" else ilm.ILM_NODES_FLOAT
     pos, idx = workloads.intersection_scene()
     mesh = ilm.Mesh(ctx, pos, idx)
-    info = mesh.build()
+    info = mesh.build(node_format=nodes_fmt)
     rs = workloads.intersection_rays(ctx, mesh)
     n = rs.n
     log(f"setup {time.perf_counter() - t0:.1f} s: {len(idx)} tris, bvh {info}")
@@ -396,7 +400,7 @@ def main():
             "wall_ms_per_step": wall * 1e3 / steps,
             "nodes_per_ray": nn,
             "tris_per_ray": nt,
-            "bytes_per_ray": RAY_BYTES + (1 if anyhit else HIT_BYTES) + nn * NODE_BYTES + nt * TRI_BYTES,
+            "bytes_per_ray": RAY_BYTES + (1 if anyhit else HIT_BYTES) + nn * info["node_stride"] + nt * TRI_BYTES,
             "steps": steps,
         }
         log(f"{name}: {res[name]}")

@@ -22,6 +22,7 @@ ILM_BUILDER_REFERENCE = 1
 ILM_BUILDER_KDTREE = 2
 ILM_BUILDER_KDTREE_EX = 3
 ILM_BUILDER_LBVH = 4
+ILM_NODES_FLOAT, ILM_NODES_QUANTIZED = 0, 1
 ILM_TONE_GLOBAL, ILM_TONE_DRAGO = 0, 1
 ILM_IMAGE_PFM, ILM_IMAGE_PPM = 0, 1
 ILM_SPAWN_SHADOW = 0
@@ -37,13 +38,14 @@ class IlmError(RuntimeError):
 
 class BvhParams(C.Structure):
     _fields_ = [("builder", C.c_int32), ("width", C.c_int32), ("max_leaf", C.c_int32),
-                ("max_depth", C.c_int32), ("threads", C.c_int32)]
+                ("max_depth", C.c_int32), ("threads", C.c_int32), ("node_format", C.c_int32)]
 
 
 class BvhInfo(C.Structure):
     _fields_ = [("nodes", C.c_int64), ("node_bytes", C.c_int64), ("tri_bytes", C.c_int64),
                 ("width", C.c_int32), ("depth", C.c_int32), ("max_leaf", C.c_int32),
-                ("stack_need", C.c_int32), ("build_ms", C.c_double), ("sah_cost", C.c_double)]
+                ("stack_need", C.c_int32), ("build_ms", C.c_double), ("sah_cost", C.c_double),
+                ("node_format", C.c_int32), ("node_stride", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -129,6 +131,10 @@ SIGNATURES = {
     "ilm_space_shading": (C.c_int, [_vp, _vp, _vp, _vp]),
     "ilm_space_resolve": (C.c_int, [_vp, _vp, _i64, _vp, _vp, _vp]),
     "ilm_space_geometry": (C.c_int, [_vp, _vp, _vp]),
+    "ilm_obj_load": (C.c_int, [C.c_char_p, C.POINTER(_vp)]),
+    "ilm_obj_counts": (C.c_int, [_vp, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "ilm_obj_arrays": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "ilm_obj_destroy": (C.c_int, [_vp]),
     "ilm_image_resolve": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_double, C.c_int, _vp, _vp]),
     "ilm_image_tonemap": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _vp]),
     "ilm_image_write": (C.c_int, [C.c_char_p, _vp, C.c_int, C.c_int, C.c_int]),
@@ -287,8 +293,9 @@ class Mesh:
     def ntri(self):
         return len(self.idx)
 
-    def build(self, builder=ILM_BUILDER_SAH, width=4, max_leaf=0, max_depth=0, threads=0) -> dict:
-        p = BvhParams(builder, width, max_leaf, max_depth, threads)
+    def build(self, builder=ILM_BUILDER_SAH, width=4, max_leaf=0, max_depth=0, threads=0,
+              node_format=ILM_NODES_FLOAT) -> dict:
+        p = BvhParams(builder, width, max_leaf, max_depth, threads, node_format)
         check(load().ilm_bvh_build(self.h, C.byref(p)))
         return self.info()
 
@@ -348,8 +355,9 @@ class Space:
                                    *[None if a is None else _ptr(a) for a in arrs], material, light, C.byref(out)))
         return out.value
 
-    def build(self, builder=ILM_BUILDER_SAH, width=4, max_leaf=0, max_depth=0, threads=0) -> dict:
-        p = BvhParams(builder, width, max_leaf, max_depth, threads)
+    def build(self, builder=ILM_BUILDER_SAH, width=4, max_leaf=0, max_depth=0, threads=0,
+              node_format=ILM_NODES_FLOAT) -> dict:
+        p = BvhParams(builder, width, max_leaf, max_depth, threads, node_format)
         check(load().ilm_space_build(self.h, C.byref(p)))
         h = C.c_void_p()
         check(load().ilm_space_mesh(self.h, C.byref(h)))
@@ -477,6 +485,29 @@ def rays_to_device(ctx: Context, rays: np.ndarray) -> DeviceBuffer:
     return ctx.to_device(rays)
 
 
+def load_obj(path: str):
+    """Reads an OBJ (+ MTL) file: returns pos, nrm, idx, tri_mat, tri_light,
+    mats, lights in the order Scene takes them."""
+    L = load()
+    h = C.c_void_p()
+    check(L.ilm_obj_load(path.encode(), C.byref(h)))
+    try:
+        nv, nt, nm, nl = C.c_int64(), C.c_int64(), C.c_int32(), C.c_int32()
+        check(L.ilm_obj_counts(h, C.byref(nv), C.byref(nt), C.byref(nm), C.byref(nl)))
+        pos = np.zeros((nv.value, 3), np.float32)
+        nrm = np.zeros((nv.value, 3), np.float32)
+        idx = np.zeros((nt.value, 3), np.int32)
+        tm = np.zeros(nt.value, np.int32)
+        tl = np.zeros(nt.value, np.int32)
+        mats = np.zeros((nm.value, 8), np.float32)
+        lights = np.zeros((nl.value, 4), np.float32)
+        check(L.ilm_obj_arrays(h, _ptr(pos), _ptr(nrm), _ptr(idx), _ptr(tm), _ptr(tl), _ptr(mats),
+                               _ptr(lights) if nl.value else None))
+    finally:
+        L.ilm_obj_destroy(h)
+    return pos, nrm, idx, tm, tl, mats, lights
+
+
 def image_resolve(ctx: Context, accum: DeviceBuffer, width: int, height: int, samples: int, out: DeviceBuffer,
                   flip=False):
     check(load().ilm_image_resolve(ctx.h, accum.ptr, width, height, 1.0 / samples, int(flip), out.ptr, None))
@@ -517,8 +548,20 @@ def vertex_normals(pos, idx):
     return nrm
 
 
-def decode_bvh4(nodes_bytes: np.ndarray):
-    """Splits an exported BVH4 node array into boxes (n,4,6) and child words (n,4)."""
+def decode_bvh4(nodes_bytes: np.ndarray, node_format: int = 0):
+    """Splits an exported BVH4 node array into boxes (n,4,6) and child words (n,4).
+    Quantised nodes are decoded to float boxes lo = p + qlo 2^e, hi = p + qhi 2^e
+    (float64, as the traversal decodes them in exact arithmetic)."""
+    if node_format == ILM_NODES_QUANTIZED:
+        raw = nodes_bytes.view(np.uint8).reshape(-1, 64)
+        p = raw[:, 0:12].copy().view(np.float32).reshape(-1, 3).astype(np.float64)
+        ex = raw[:, 12:15].astype(np.int64) - 127                      # (n, 3)
+        child = raw[:, 16:32].copy().view(np.uint32).reshape(-1, 4)
+        q = raw[:, 32:56].reshape(-1, 6, 4).astype(np.float64)          # qlox,qhix,qloy,qhiy,qloz,qhiz x child
+        step = np.ldexp(1.0, ex)                                       # (n, 3)
+        lo = p[:, None, :] + np.stack([q[:, 0], q[:, 2], q[:, 4]], axis=-1) * step[:, None, :]
+        hi = p[:, None, :] + np.stack([q[:, 1], q[:, 3], q[:, 5]], axis=-1) * step[:, None, :]
+        return np.concatenate([lo, hi], axis=-1), child
     raw = nodes_bytes.view(np.uint8).reshape(-1, 112)
     f = raw[:, :96].copy().view(np.float32).reshape(-1, 6, 4)   # lox,hix,loy,hiy,loz,hiz
     child = raw[:, 96:].copy().view(np.uint32).reshape(-1, 4)

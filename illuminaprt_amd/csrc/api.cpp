@@ -227,9 +227,11 @@ int ilm_bvh_build(ilm_mesh* m, const ilm_bvh_params* params) {
     if (params) p = *params;
     if (p.width == 0) p.width = 4;
     if (p.width != 4) return fail(ILM_ERR_INVALID, "this build traverses BVH4 only");
+    if (p.node_format != ILM_NODES_FLOAT && p.node_format != ILM_NODES_QUANTIZED)
+        return fail(ILM_ERR_INVALID, "unknown node format");
     if (p.builder == ILM_BUILDER_LBVH) {
         if (!m->ctx) return fail(ILM_ERR_STATE, "the GPU builder needs a device context");
-        return ilm::build_lbvh(m, p.max_leaf > 0 ? p.max_leaf : 4);
+        return ilm::build_lbvh(m, p.max_leaf > 0 ? p.max_leaf : 4, p.node_format);
     }
     auto t0 = std::chrono::steady_clock::now();
     ilm::WideBvh w;
@@ -257,6 +259,17 @@ int ilm_bvh_build(ilm_mesh* m, const ilm_bvh_params* params) {
     }
     std::vector<float> tris;
     ilm::pack_triangles(m->pos.data(), m->idx.data(), w.order, tris);
+    if (p.node_format == ILM_NODES_QUANTIZED) {
+        std::vector<uint8_t> q((size_t)w.nnodes * sizeof(ilm::QNode4));
+        const auto* src = reinterpret_cast<const ilm::WideNode<4>*>(w.nodes.data());
+        auto* dst = reinterpret_cast<ilm::QNode4*>(q.data());
+        for (int64_t i = 0; i < w.nnodes; ++i) ilm::quantize_node(src[i], dst[i]);
+        w.nodes.swap(q);
+    }
+    m->node_format = p.node_format;
+    m->info.node_format = p.node_format;
+    m->info.node_stride = p.node_format == ILM_NODES_QUANTIZED ? (int32_t)sizeof(ilm::QNode4)
+                                                               : (int32_t)sizeof(ilm::WideNode<4>);
     auto t1 = std::chrono::steady_clock::now();
     if (!m->ctx) {  // host-only mesh: keep the flattened tree for export
         m->width = w.width;

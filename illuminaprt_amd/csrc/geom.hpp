@@ -8,6 +8,7 @@
 
 #include <cfloat>
 #include <cmath>
+#include <cstring>
 
 #define ILM_HD __host__ __device__ __forceinline__
 
@@ -52,6 +53,86 @@ ILM_HD bool is_leaf(uint32_t c) { return (c & kLeafBit) != 0; }
 ILM_HD uint32_t leaf_first(uint32_t c) { return (c & ~kLeafBit) >> kLeafCountBits; }
 ILM_HD uint32_t leaf_count(uint32_t c) { return c & ((1u << kLeafCountBits) - 1u); }
 ILM_HD uint32_t make_leaf(uint32_t first, uint32_t count) { return kLeafBit | (first << kLeafCountBits) | count; }
+constexpr uint32_t kEmptySlot = kLeafBit;  // make_leaf(0, 0)
+
+// Quantised BVH4 node, 64 bytes (four 16-byte loads, half a 128-byte line).
+// Child boxes are stored as bytes relative to an origin p (the minimum corner
+// of the children) in steps of 2^e per axis:
+//   lo = p + qlo * 2^e,  hi = p + qhi * 2^e,
+// rounded outward so that every decoded box contains its child. Bytes k of
+// qlo*/qhi* belong to child k; exps holds e + 127 for x, y, z in bytes 0-2
+// (the bit pattern of 2^e as a float is that byte shifted left by 23).
+// Empty slots have child word kEmptySlot and are skipped by the traversal.
+struct alignas(16) QNode4 {
+    float px, py, pz;
+    uint32_t exps;
+    uint32_t child[4];
+    uint32_t qlox, qhix, qloy, qhiy;
+    uint32_t qloz, qhiz, pad0, pad1;
+};
+static_assert(sizeof(QNode4) == 64, "quantised BVH4 node is 64 bytes");
+
+ILM_HD float pow2_from_biased(uint32_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __uint_as_float(b << 23);
+#else
+    uint32_t u = b << 23;
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+#endif
+}
+
+// Quantises one float BVH4 node. The step per axis is the smallest power of
+// two with (max hi - p) / step <= 254; each byte is rounded outward and then
+// checked in float (p + q * step) so that the decoded box contains the child.
+ILM_HD void quantize_node(const WideNode<4>& n, QNode4& q) {
+    float p[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, top[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    bool any = false;
+    for (int k = 0; k < 4; ++k) {
+        if (n.child[k] == kEmptySlot) continue;
+        any = true;
+        const float lo[3] = {n.lox[k], n.loy[k], n.loz[k]}, hi[3] = {n.hix[k], n.hiy[k], n.hiz[k]};
+        for (int a = 0; a < 3; ++a) { p[a] = fminf(p[a], lo[a]); top[a] = fmaxf(top[a], hi[a]); }
+    }
+    if (!any)
+        for (int a = 0; a < 3; ++a) p[a] = top[a] = 0.0f;
+    float step[3];
+    uint32_t exps = 0;
+    for (int a = 0; a < 3; ++a) {
+        float ext = top[a] - p[a];
+        int e = -126;
+        if (ext > 0.0f) {
+            int x;
+            frexpf(ext / 254.0f, &x);  // ext / 254 < 2^x
+            e = x < -126 ? -126 : (x > 127 ? 127 : x);
+        }
+        step[a] = ldexpf(1.0f, e);
+        exps |= (uint32_t)(e + 127) << (8 * a);
+    }
+    q.px = p[0]; q.py = p[1]; q.pz = p[2];
+    q.exps = exps;
+    q.pad0 = q.pad1 = 0;
+    uint32_t ql[3] = {0, 0, 0}, qh[3] = {0, 0, 0};
+    for (int k = 0; k < 4; ++k) {
+        q.child[k] = n.child[k];
+        if (n.child[k] == kEmptySlot) continue;
+        const float lo[3] = {n.lox[k], n.loy[k], n.loz[k]}, hi[3] = {n.hix[k], n.hiy[k], n.hiz[k]};
+        for (int a = 0; a < 3; ++a) {
+            float fl = floorf((lo[a] - p[a]) / step[a]);
+            int l = fl < 0.0f ? 0 : (fl > 255.0f ? 255 : (int)fl);
+            while (l > 0 && p[a] + (float)l * step[a] > lo[a]) --l;
+            float fh = ceilf((hi[a] - p[a]) / step[a]);
+            int h = fh < 0.0f ? 0 : (fh > 255.0f ? 255 : (int)fh);
+            while (h < 255 && p[a] + (float)h * step[a] < hi[a]) ++h;
+            ql[a] |= (uint32_t)l << (8 * k);
+            qh[a] |= (uint32_t)h << (8 * k);
+        }
+    }
+    q.qlox = ql[0]; q.qhix = qh[0];
+    q.qloy = ql[1]; q.qhiy = qh[1];
+    q.qloz = ql[2]; q.qhiz = qh[2];
+}
 
 // ------------------------------------------------------------------ t window
 // The reference accepts a triangle hit when 1e-5 <= t <= Max - 1e-5, both

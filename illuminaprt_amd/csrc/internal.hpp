@@ -51,6 +51,7 @@ struct ilm_mesh {
     float* d_tris = nullptr;  // 12 floats per triangle, leaf order
     int32_t* d_slot_of = nullptr;  // original triangle index -> slot (built on demand)
     int width = 0;
+    int node_format = ILM_NODES_FLOAT;
     ilm_bvh_info info{};
     std::vector<uint8_t> h_nodes;
     std::vector<int32_t> h_order;
@@ -69,7 +70,7 @@ int trace_queue(ilm_mesh* m, const ilm_ray* rays, const unsigned long long* n_de
                 uint8_t* occ, hipStream_t stream);
 // Builds m's BVH4 on its device (linear BVH + collapse); fills the device
 // arrays, info and host copies as ilm_bvh_build does.
-int build_lbvh(ilm_mesh* m, int max_leaf);
+int build_lbvh(ilm_mesh* m, int max_leaf, int node_format);
 // Cache inference over a query queue whose length lives in device memory.
 int nrc_infer_queue(ilm_nrc* r, const float* queries, const unsigned long long* n_dev, int64_t capacity, float* out,
                     hipStream_t stream);

@@ -264,6 +264,16 @@ __global__ void __launch_bounds__(kBlock) k_pack(const float* __restrict__ pos, 
     }
 }
 
+// float BVH4 nodes -> quantised nodes (same quantiser as the host builders)
+__global__ void __launch_bounds__(kBlock) k_quantize(const WideNode<4>* __restrict__ in, int64_t n,
+                                                     QNode4* __restrict__ out) {
+    int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    QNode4 q;
+    quantize_node(in[i], q);
+    out[i] = q;
+}
+
 __global__ void k_init_u32(uint32_t* p, int64_t n, uint32_t v) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -275,7 +285,7 @@ inline unsigned blocks_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBl
 
 // Builds the BVH4 of m on its context's device. On success m->d_nodes,
 // m->d_tris, m->info, m->h_nodes and m->h_order hold the result.
-int build_lbvh(ilm_mesh* m, int max_leaf) {
+int build_lbvh(ilm_mesh* m, int max_leaf, int node_format) {
     ilm_ctx* ctx = m->ctx;
     const int64_t nt = m->nt, nv = m->nv;
     if (nt > (1 << 26) - 1) return fail(ILM_ERR_INVALID, "more than 2^26 - 1 triangles");
@@ -447,6 +457,20 @@ int build_lbvh(ilm_mesh* m, int max_leaf) {
     hipLaunchKernelGGL(k_pack, dim3(blocks_for(n)), dim3(kBlock), 0, st, d_pos, d_idx, ids, n,
                        reinterpret_cast<float4*>(d_tris));
     LBVH_HIP(hipGetLastError());
+    const int64_t nwide = h_counts[2];
+    size_t stride = sizeof(WideNode<4>);
+    if (node_format == ILM_NODES_QUANTIZED) {
+        void* qn = nullptr;
+        if (alloc((size_t)nwide * sizeof(QNode4), &qn) != ILM_OK) return fail_all(ILM_ERR_NOMEM, "quantised nodes");
+        hipLaunchKernelGGL(k_quantize, dim3(blocks_for(nwide)), dim3(kBlock), 0, st,
+                           reinterpret_cast<const WideNode<4>*>(d_nodes), nwide, reinterpret_cast<QNode4*>(qn));
+        LBVH_HIP(hipGetLastError());
+        // the float nodes become scratch and the quantised ones the output
+        tmp.erase(std::find(tmp.begin(), tmp.end(), qn));
+        tmp.push_back(d_nodes);
+        d_nodes = qn;
+        stride = sizeof(QNode4);
+    }
     LBVH_HIP(hipEventRecord(e1, st));
     LBVH_HIP(hipStreamSynchronize(st));
     float gpu_ms = 0;
@@ -455,8 +479,7 @@ int build_lbvh(ilm_mesh* m, int max_leaf) {
     (void)hipEventDestroy(e1);
 
     // host copies for export and for hit-slot lookups
-    const int64_t nwide = h_counts[2];
-    m->h_nodes.resize((size_t)nwide * sizeof(WideNode<4>));
+    m->h_nodes.resize((size_t)nwide * stride);
     m->h_order.resize(nt);
     LBVH_HIP(hipMemcpy(m->h_nodes.data(), d_nodes, m->h_nodes.size(), hipMemcpyDeviceToHost));
     LBVH_HIP(hipMemcpy(m->h_order.data(), ids, nt * sizeof(int32_t), hipMemcpyDeviceToHost));
@@ -469,8 +492,11 @@ int build_lbvh(ilm_mesh* m, int max_leaf) {
     m->d_nodes = d_nodes;
     m->d_tris = d_tris;
     m->width = 4;
+    m->node_format = node_format;
+    m->info.node_format = node_format;
+    m->info.node_stride = (int32_t)stride;
     m->info.nodes = nwide;
-    m->info.node_bytes = nwide * (int64_t)sizeof(WideNode<4>);
+    m->info.node_bytes = nwide * (int64_t)stride;
     m->info.tri_bytes = nt * 12 * (int64_t)sizeof(float);
     m->info.width = 4;
     m->info.depth = levels;

@@ -55,7 +55,52 @@ __device__ __forceinline__ float slab1(float lox, float hix, float loy, float hi
         uint32_t tc = c##a; c##a = c##b; c##b = tc;       \
     }
 
-template <bool ANY, bool STATS>
+// Child entry distances of a node: float boxes (7 x 16 B) or quantised boxes
+// (4 x 16 B, QNode4); returns the child words in c*.
+template <bool QUANT>
+__device__ __forceinline__ void node_hits(const float4* __restrict__ nodes, uint32_t cur, const RayState& r,
+                                          float& d0, float& d1, float& d2, float& d3, uint32_t& c0, uint32_t& c1,
+                                          uint32_t& c2, uint32_t& c3) {
+    if (!QUANT) {
+        const float4* np = nodes + (int64_t)cur * 7;
+        float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
+        uint4 ch = reinterpret_cast<const uint4*>(np)[6];
+        d0 = slab1(lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, r);
+        d1 = slab1(lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, r);
+        d2 = slab1(lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, r);
+        d3 = slab1(lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, r);
+        c0 = ch.x; c1 = ch.y; c2 = ch.z; c3 = ch.w;
+    } else {
+        const float4* np = nodes + (int64_t)cur * 4;
+        float4 o = np[0];
+        uint4 ch = reinterpret_cast<const uint4*>(np)[1];
+        uint4 qa = reinterpret_cast<const uint4*>(np)[2];
+        uint4 qb = reinterpret_cast<const uint4*>(np)[3];
+        const uint32_t ex = __float_as_uint(o.w);
+        // child plane t = (p + q 2^e - o) / d = q * (2^e / d) + (p - o) / d
+        const float Sx = __uint_as_float((ex & 0xFFu) << 23) * r.inv.x, Bx = fmaf(o.x, r.inv.x, -r.oinv.x);
+        const float Sy = __uint_as_float(((ex >> 8) & 0xFFu) << 23) * r.inv.y, By = fmaf(o.y, r.inv.y, -r.oinv.y);
+        const float Sz = __uint_as_float(((ex >> 16) & 0xFFu) << 23) * r.inv.z, Bz = fmaf(o.z, r.inv.z, -r.oinv.z);
+        float d[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int sh = 8 * k;
+            float tx0 = fmaf((float)((qa.x >> sh) & 0xFFu), Sx, Bx), tx1 = fmaf((float)((qa.y >> sh) & 0xFFu), Sx, Bx);
+            float ty0 = fmaf((float)((qa.z >> sh) & 0xFFu), Sy, By), ty1 = fmaf((float)((qa.w >> sh) & 0xFFu), Sy, By);
+            float tz0 = fmaf((float)((qb.x >> sh) & 0xFFu), Sz, Bz), tz1 = fmaf((float)((qb.y >> sh) & 0xFFu), Sz, Bz);
+            float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), r.tlo));
+            float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), r.thi));
+            d[k] = (tn <= tf * 1.00000048f) ? tn : INFINITY;
+        }
+        c0 = ch.x; c1 = ch.y; c2 = ch.z; c3 = ch.w;
+        d0 = c0 == kEmptySlot ? INFINITY : d[0];
+        d1 = c1 == kEmptySlot ? INFINITY : d[1];
+        d2 = c2 == kEmptySlot ? INFINITY : d[2];
+        d3 = c3 == kEmptySlot ? INFINITY : d[3];
+    }
+}
+
+template <bool ANY, bool STATS, bool QUANT>
 __global__ void __launch_bounds__(kBlock) k_trace4(TraceArgs a) {
     __shared__ uint32_t lds[kLds * kBlock];
     const int tid = threadIdx.x;
@@ -138,15 +183,10 @@ __global__ void __launch_bounds__(kBlock) k_trace4(TraceArgs a) {
 
         // inner nodes
         while (active && !is_leaf(cur)) {
-            const float4* np = a.nodes + (int64_t)cur * 7;
-            float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
-            uint4 ch = reinterpret_cast<const uint4*>(np)[6];
             if (STATS) ++n_nodes;
-            float d0 = slab1(lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, r);
-            float d1 = slab1(lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, r);
-            float d2 = slab1(lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, r);
-            float d3 = slab1(lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, r);
-            uint32_t c0 = ch.x, c1 = ch.y, c2 = ch.z, c3 = ch.w;
+            float d0, d1, d2, d3;
+            uint32_t c0, c1, c2, c3;
+            node_hits<QUANT>(a.nodes, cur, r, d0, d1, d2, d3, c0, c1, c2, c3);
             ILM_CSWAP(0, 1) ILM_CSWAP(2, 3) ILM_CSWAP(0, 2) ILM_CSWAP(1, 3) ILM_CSWAP(1, 2)
             if (d3 < INFINITY) push(c3);
             if (d2 < INFINITY) push(c2);
@@ -206,7 +246,10 @@ static int launch_trace(ilm_mesh* m, const ilm_ray* rays, int64_t n, ilm_hit* hi
     ilm_ctx* ctx = m->ctx;
     ILM_HIP(hipSetDevice(ctx->device));
     int per_cu = 0;
-    ILM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace4<ANY, STATS>, kBlock, 0));
+    if (m->node_format == ILM_NODES_QUANTIZED)
+        ILM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace4<ANY, STATS, true>, kBlock, 0));
+    else
+        ILM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace4<ANY, STATS, false>, kBlock, 0));
     per_cu = std::max(per_cu, 1);
     int64_t blocks = (int64_t)per_cu * ctx->cu_count;
     blocks = std::min<int64_t>(blocks, (n + kBlock - 1) / kBlock);
@@ -227,7 +270,12 @@ static int launch_trace(ilm_mesh* m, const ilm_ray* rays, int64_t n, ilm_hit* hi
     a.spill = ctx->spill;
     a.counters = ctx->counters;
     a.total_waves = (uint32_t)(blocks * (kBlock / 64));
-    hipLaunchKernelGGL((k_trace4<ANY, STATS>), dim3((unsigned)blocks), dim3(kBlock), 0, pick_stream(ctx, stream), a);
+    if (m->node_format == ILM_NODES_QUANTIZED)
+        hipLaunchKernelGGL((k_trace4<ANY, STATS, true>), dim3((unsigned)blocks), dim3(kBlock), 0,
+                           pick_stream(ctx, stream), a);
+    else
+        hipLaunchKernelGGL((k_trace4<ANY, STATS, false>), dim3((unsigned)blocks), dim3(kBlock), 0,
+                           pick_stream(ctx, stream), a);
     ILM_HIP(hipGetLastError());
     return ILM_OK;
 }

@@ -124,13 +124,19 @@ int ilm_mesh_destroy(ilm_mesh* mesh);
  * the device time. Faster to build, slower to trace than ILM_BUILDER_SAH. */
 #define ILM_BUILDER_LBVH 4
 
+/* Node formats of the BVH4 the traversal kernels read. */
+#define ILM_NODES_FLOAT 0      /* 112 B: float child boxes, SoA, then four child words */
+#define ILM_NODES_QUANTIZED 1  /* 64 B: child boxes as bytes on a per-node power-of-two grid,
+                                  rounded outward (conservative) */
+
 typedef struct {
     int32_t builder;     /* ILM_BUILDER_* */
-    int32_t width;       /* 4 (default when 0); 2 and 8 also accepted */
-    int32_t max_leaf;    /* SAH: max triangles per leaf (default 4);
+    int32_t width;       /* 4 (default when 0); other widths are rejected */
+    int32_t max_leaf;    /* SAH, LBVH: max triangles per leaf (default 4);
                             REFERENCE, KDTREE*: MaximumLeafObjects (default 10) */
     int32_t max_depth;   /* REFERENCE, KDTREE: MaximumTreeDepth (default 35, KDTREE at most 40) */
     int32_t threads;     /* host build threads (0 = all) */
+    int32_t node_format; /* ILM_NODES_* */
 } ilm_bvh_params;
 
 typedef struct {
@@ -143,6 +149,8 @@ typedef struct {
     int32_t stack_need;   /* worst-case traversal stack entries */
     double build_ms;      /* host build + flatten time */
     double sah_cost;      /* SAH cost of the wide tree (Ci = 1, Ct = 1) */
+    int32_t node_format;  /* ILM_NODES_* */
+    int32_t node_stride;  /* bytes per node */
 } ilm_bvh_info;
 
 /* Builds the hierarchy on the host, flattens it to a wide BVH and uploads
@@ -334,6 +342,26 @@ typedef struct {
  * Synchronous; stats may be NULL. */
 int ilm_render(ilm_scene* scene, const ilm_render_params* params, double* accum,
                ilm_render_stats* stats);
+
+/* ---------------------------------------------------------------- scene files */
+
+/* Wavefront OBJ (+ MTL) loading with the reference loader's rules
+ * (Scene/WavefrontSceneLoader.cpp:203-561): v / vn / vt / f / usemtl /
+ * mtllib, newmtl / Kd / Ks / Ke / Ns; faces of 3 or 4 corners (quads split
+ * (0,1,2),(0,2,3), larger polygons skipped); corners "p", "p/t/n", "p//n"
+ * (a two-field corner is position/normal); equal (p, t, n) corners share a
+ * vertex. Vertices without a normal get area-weighted normals. Materials
+ * become 8-float records (kd, ks, Ns, type 1 if ks is not black); a material
+ * with non-black Ke is also a light with Le = Ke. The arrays feed
+ * ilm_mesh_create / ilm_space_add / ilm_scene_create directly. */
+typedef struct ilm_obj ilm_obj;
+int ilm_obj_load(const char* path, ilm_obj** out);
+int ilm_obj_counts(ilm_obj* obj, int64_t* nv, int64_t* nt, int32_t* nmat, int32_t* nlight);
+/* Host copies: pos, nrm (nv*3), idx (nt*3), tri_mat, tri_light (nt),
+ * mats (nmat*8), lights (nlight*4). Any pointer may be NULL. */
+int ilm_obj_arrays(ilm_obj* obj, float* pos, float* nrm, int32_t* idx, int32_t* tri_mat, int32_t* tri_light,
+                   float* mats, float* lights);
+int ilm_obj_destroy(ilm_obj* obj);
 
 /* ---------------------------------------------------------------- frames and images */
 

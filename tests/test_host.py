@@ -174,6 +174,33 @@ def test_kdtree_builders_match_oracle(ilm, orc, kind, builder):
     m.close()
 
 
+@pytest.mark.parametrize("builder", [0, 1, 2])
+def test_quantized_nodes_contain_float_boxes(ilm, builder):
+    # The 64-byte nodes hold every child box of the 112-byte nodes, rounded
+    # outward, with the same child words; and the quantisation is tight
+    # (each side within two grid steps of the float box).
+    sp, si = meshgen.icosphere(3)
+    tp, ti = meshgen.random_triangles(2000, seed=4, extent=1.5)
+    pos, idx = meshgen.merge([(sp, si), (tp, ti)])
+    mf = ilm.Mesh(None, pos, idx)
+    fi = mf.build(builder=builder)
+    mq = ilm.Mesh(None, pos, idx)
+    qi = mq.build(builder=builder, node_format=ilm.ILM_NODES_QUANTIZED)
+    assert qi["node_stride"] == 64 and fi["node_stride"] == 112 and qi["nodes"] == fi["nodes"]
+    fb, fc = ilm.decode_bvh4(mf.export()[0])
+    qb, qc = ilm.decode_bvh4(mq.export()[0], ilm.ILM_NODES_QUANTIZED)
+    assert np.array_equal(fc, qc)
+    used = qc != 0x80000000
+    f64 = fb.astype(np.float64)
+    assert (qb[..., :3][used] <= f64[..., :3][used]).all() and (qb[..., 3:][used] >= f64[..., 3:][used]).all()
+    raw = mq.export()[0].view(np.uint8).reshape(-1, 64)
+    step = np.ldexp(1.0, raw[:, 12:15].astype(np.int64) - 127)[:, None, :].repeat(4, axis=1)
+    slack = np.concatenate([f64[..., :3] - qb[..., :3], qb[..., 3:] - f64[..., 3:]], axis=-1)
+    assert (slack[used] <= 2 * np.concatenate([step, step], axis=-1)[used]).all()
+    mf.close()
+    mq.close()
+
+
 def test_space_flattening(ilm):
     # ilm_space_build applies Transformation::Apply to every vertex and
     # Transformation::Rotate (then normalisation) to every normal.
