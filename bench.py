@@ -451,7 +451,8 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic_from_profiles(),
-            "kernel": "ilm::k_trace4<false,false>",
+            "kernel": f"ilm::k_trace4<false, false, {'true' if args.nodes == 'quantized' else 'false'}>",
+            "node_bytes": info["node_stride"],
             "bytes_per_ray": round(inc["bytes_per_ray"], 1),
         },
         "cpu_baseline": cpu,
