@@ -149,6 +149,7 @@ SIGNATURES = {
     "ilm_nrc_get_gradients": (C.c_int, [_vp, _vp]),
     "ilm_nrc_infer": (C.c_int, [_vp, _vp, _i64, _vp, _vp]),
     "ilm_nrc_train_step": (C.c_int, [_vp, _vp, _vp, _i64, _vp, _vp]),
+    "ilm_scene_set_graph": (C.c_int, [_vp, C.c_int]),
     "ilm_render_nrc": (C.c_int, [_vp, _vp, C.POINTER(RenderParams), C.POINTER(NrcRenderParams), _vp,
                                  C.POINTER(RenderStats)]),
 }
@@ -415,6 +416,9 @@ class Scene:
         st = RenderStats()
         check(load().ilm_render(self.h, C.byref(p), accum.ptr, C.byref(st)))
         return {k: getattr(st, k) for k, _ in st._fields_}
+
+    def set_graph(self, enable: bool):
+        check(load().ilm_scene_set_graph(self.h, int(enable)))
 
     def render_nrc(self, nrc: "Nrc", cam: Camera, width, height, spp, accum: DeviceBuffer | None, seed=1234,
                    spp_offset=0, rr_depth=4, eps=1e-4, prefix_depth=2, suffix_depth=8, train_fraction=1 / 16,

@@ -72,6 +72,16 @@ struct alignas(16) QNode4 {
 };
 static_assert(sizeof(QNode4) == 64, "quantised BVH4 node is 64 bytes");
 
+ILM_HD uint32_t float_bits(float f) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __float_as_uint(f);
+#else
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    return u;
+#endif
+}
+
 ILM_HD float pow2_from_biased(uint32_t b) {
 #if defined(__HIP_DEVICE_COMPILE__)
     return __uint_as_float(b << 23);

@@ -117,8 +117,10 @@ struct PtArgs {
     CamK cam;
     int W, H;
     int64_t pix0;
-    int spp, spp_offset;
-    uint32_t seed;
+    int spp;
+    // per-frame constants in device memory, so that a captured frame can be
+    // replayed with a new seed and sample offset: [0] seed, [1] spp_offset
+    const int32_t* frame;
     int max_depth, rr_depth;
     float eps;
     // path state
@@ -154,7 +156,7 @@ struct PtArgs {
 
 __device__ __forceinline__ void path_pixel(const PtArgs& a, int64_t p, int64_t& pixel, uint32_t& sample) {
     pixel = a.pix0 + p / a.spp;
-    sample = (uint32_t)(a.spp_offset + (int)(p % a.spp));
+    sample = (uint32_t)(a.frame[1] + (int)(p % a.spp));
 }
 
 __global__ void k_generate(PtArgs a) {
@@ -168,7 +170,7 @@ __global__ void k_generate(PtArgs a) {
     int64_t pixel;
     uint32_t sample;
     path_pixel(a, p, pixel, sample);
-    Block8 b = sample_block(a.seed, (uint64_t)pixel, sample, 0, 0);
+    Block8 b = sample_block((uint32_t)a.frame[0], (uint64_t)pixel, sample, 0, 0);
     int x = (int)(pixel % a.W), y = (int)(pixel / a.W);
     // PerspectiveCamera::GetRay (Camera/PerspectiveCamera.cpp:53-60) as fed by
     // RenderRegion with the box filter's -0.5 shift (BaseRenderer.cpp:106-118)
@@ -301,7 +303,7 @@ __global__ void __launch_bounds__(kPtBlock) k_shade(PtArgs a, int bounce, int cu
             ilm_ray sray;
             f3 contrib = mk(0, 0, 0), contrib2 = mk(0, 0, 0);
             if (live && l != light_id) {
-                Block8 lb = sample_block(a.seed, (uint64_t)pixel, sample, (uint32_t)bounce, 1u + (uint32_t)l);
+                Block8 lb = sample_block((uint32_t)a.frame[0], (uint64_t)pixel, sample, (uint32_t)bounce, 1u + (uint32_t)l);
                 int2 rg = a.lrange[l];
                 float4 le = a.lights[l];
                 int kk = min((int)((float)rg.y * lb.v[2]), rg.y - 1);
@@ -346,7 +348,7 @@ __global__ void __launch_bounds__(kPtBlock) k_shade(PtArgs a, int bounce, int cu
         bool cont = false;
         ilm_ray nray;
         if (live) {
-            Block8 b = sample_block(a.seed, (uint64_t)pixel, sample, (uint32_t)bounce, 0);
+            Block8 b = sample_block((uint32_t)a.frame[0], (uint64_t)pixel, sample, (uint32_t)bounce, 0);
             Basis sb = basis_from_w(Ns);
             f3 wiL;
             float pdf = 0.f;
@@ -438,6 +440,10 @@ __global__ void k_shadow_accum(PtArgs a) {
 
 // Bookkeeping between bounces (a one-thread kernel, so the bounce loop needs
 // no host synchronisation and no memset nodes).
+__global__ void k_clear_counts(unsigned long long* counts) {
+    if (threadIdx.x < 8) counts[threadIdx.x] = 0ull;
+}
+
 __global__ void k_end_bounce(unsigned long long* counts, int cur) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         counts[3] += counts[cur];
@@ -482,11 +488,26 @@ struct ilm_scene {
     int64_t train_cap = 0;
     std::vector<void*> bufs;
     ilm::PtArgs a{};
+    // frame constants (device) and their host copy
+    int32_t* d_frame = nullptr;
+    int32_t h_frame[2] = {0, 0};
+    // captured frame: replayed while the key (everything baked into the
+    // launches) is unchanged
+    bool use_graph = false;
+    hipGraphExec_t graph = nullptr;
+    std::vector<int64_t> graph_key;
 };
 
 namespace {
 
+void drop_graph(ilm_scene* s) {
+    if (s->graph) (void)hipGraphExecDestroy(s->graph);
+    s->graph = nullptr;
+    s->graph_key.clear();
+}
+
 void scene_free_state(ilm_scene* s) {
+    drop_graph(s);
     for (void* p : s->bufs) ilm_free(s->ctx, p);
     s->bufs.clear();
     s->cap = 0;
@@ -611,6 +632,7 @@ int ilm_scene_create(ilm_ctx* ctx, ilm_mesh* mesh, const float* nrm, const int32
     rc = rc ? rc : scene_alloc(s, lt.size() * 4, (void**)&s->d_ltris);
     rc = rc ? rc : scene_alloc(s, lrec.size() * 4, (void**)&s->d_lights);
     rc = rc ? rc : scene_alloc(s, lrange.size() * 4, (void**)&s->d_lrange);
+    rc = rc ? rc : scene_alloc(s, 4 * sizeof(int32_t), (void**)&s->d_frame);
     rc = rc ? rc : ilm_upload(ctx, s->d_shade, shade.data(), shade.size() * 4);
     rc = rc ? rc : ilm_upload(ctx, s->d_mats, mats, (size_t)nmat * 8 * 4);
     rc = rc ? rc : ilm_upload(ctx, s->d_ltris, lt.data(), lt.size() * 4);
@@ -631,7 +653,8 @@ int ilm_scene_create(ilm_ctx* ctx, ilm_mesh* mesh, const float* nrm, const int32
 int ilm_scene_destroy(ilm_scene* s) {
     if (!s) return ILM_OK;
     scene_free_state(s);
-    for (void* p : {(void*)s->d_shade, (void*)s->d_mats, (void*)s->d_ltris, (void*)s->d_lights, (void*)s->d_lrange})
+    for (void* p : {(void*)s->d_shade, (void*)s->d_mats, (void*)s->d_ltris, (void*)s->d_lights, (void*)s->d_lrange,
+                    (void*)s->d_frame})
         if (p) ilm_free(s->ctx, p);
     delete s;
     return ILM_OK;
@@ -693,49 +716,93 @@ int render_impl(ilm_scene* s, ilm_nrc* nrc, const ilm_render_params* prm, const 
     a.W = p.width;
     a.H = p.height;
     a.spp = p.spp;
-    a.spp_offset = p.spp_offset;
-    a.seed = p.seed;
+
     a.max_depth = p.max_depth;
     a.rr_depth = p.rr_depth;
     a.eps = p.eps;
     a.accum = accum;
+    a.frame = s->d_frame;
     hipStream_t st = ctx->stream;
     int ev0 = 0, ev1 = 0;
     rc = ilm_timer_mark(ctx, &ev0);
     if (rc) return rc;
-    ILM_HIP(hipMemsetAsync(a.counts, 0, 8 * sizeof(unsigned long long), st));
+    // per-frame constants: the only inputs that change between replays
+    s->h_frame[0] = (int32_t)p.seed;
+    s->h_frame[1] = p.spp_offset;
+    ILM_HIP(hipMemcpyAsync(s->d_frame, s->h_frame, sizeof(s->h_frame), hipMemcpyHostToDevice, st));
     const int grid = ctx->cu_count * 8;
-    if (cache)
-        hipLaunchKernelGGL(ilm::k_nrc_clear, dim3((unsigned)((train_cap + 255) / 256)), dim3(256), 0, st, a);
-    for (int64_t pix0 = 0; pix0 < npix_total; pix0 += ppb) {
-        int64_t npix = std::min(ppb, npix_total - pix0);
-        a.pix0 = pix0;
-        a.npaths = npix * p.spp;
-        hipLaunchKernelGGL(ilm::k_generate, dim3((unsigned)((a.npaths + 255) / 256)), dim3(256), 0, st, a);
-        int cur = 0;
-        for (int b = 0; b < max_bounces; ++b) {
-            rc = ilm::trace_queue(s->mesh, a.rays[cur], a.counts + cur, cap, a.hits, nullptr, st);
-            if (rc) return rc;
-            hipLaunchKernelGGL(ilm::k_shade, dim3(grid), dim3(ilm::kPtBlock), 0, st, a, b, cur);
-            if (s->nlights > 0) {
-                rc = ilm::trace_queue(s->mesh, a.srays, a.counts + 2, cap * s->nlights, nullptr, a.socc, st);
-                if (rc) return rc;
-                hipLaunchKernelGGL(ilm::k_shadow_accum, dim3(grid), dim3(256), 0, st, a);
+    // The frame's launch sequence (no host synchronisation inside, counters
+    // cleared by a kernel rather than a memset node, so it can be captured).
+    auto launch_frame = [&]() -> int {
+        hipLaunchKernelGGL(ilm::k_clear_counts, dim3(1), dim3(64), 0, st, a.counts);
+        if (cache)
+            hipLaunchKernelGGL(ilm::k_nrc_clear, dim3((unsigned)((train_cap + 255) / 256)), dim3(256), 0, st, a);
+        for (int64_t pix0 = 0; pix0 < npix_total; pix0 += ppb) {
+            int64_t npix = std::min(ppb, npix_total - pix0);
+            a.pix0 = pix0;
+            a.npaths = npix * p.spp;
+            hipLaunchKernelGGL(ilm::k_generate, dim3((unsigned)((a.npaths + 255) / 256)), dim3(256), 0, st, a);
+            int cur = 0;
+            for (int b = 0; b < max_bounces; ++b) {
+                int r2 = ilm::trace_queue(s->mesh, a.rays[cur], a.counts + cur, cap, a.hits, nullptr, st);
+                if (r2) return r2;
+                hipLaunchKernelGGL(ilm::k_shade, dim3(grid), dim3(ilm::kPtBlock), 0, st, a, b, cur);
+                if (s->nlights > 0) {
+                    r2 = ilm::trace_queue(s->mesh, a.srays, a.counts + 2, cap * s->nlights, nullptr, a.socc, st);
+                    if (r2) return r2;
+                    hipLaunchKernelGGL(ilm::k_shadow_accum, dim3(grid), dim3(256), 0, st, a);
+                }
+                hipLaunchKernelGGL(ilm::k_end_bounce, dim3(1), dim3(1), 0, st, a.counts, cur);
+                cur ^= 1;
             }
-            hipLaunchKernelGGL(ilm::k_end_bounce, dim3(1), dim3(1), 0, st, a.counts, cur);
-            cur ^= 1;
+            if (cache) {
+                // batched cache inference at the query vertices, then training targets
+                int r2 = ilm::nrc_infer_queue(nrc, a.queries, a.counts + 5, cap, reinterpret_cast<float*>(a.qout), st);
+                if (r2) return r2;
+                hipLaunchKernelGGL(ilm::k_nrc_resolve, dim3(grid), dim3(256), 0, st, a);
+                hipLaunchKernelGGL(ilm::k_nrc_collect, dim3((unsigned)((a.npaths + 255) / 256)), dim3(256), 0, st, a);
+                hipLaunchKernelGGL(ilm::k_nrc_reset_queries, dim3(1), dim3(1), 0, st, a.counts);
+            }
+            if (do_accum)
+                hipLaunchKernelGGL(ilm::k_accumulate, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, st, a, npix);
+            ILM_HIP(hipGetLastError());
         }
-        if (cache) {
-            // batched cache inference at the query vertices, then training targets
-            rc = ilm::nrc_infer_queue(nrc, a.queries, a.counts + 5, cap, reinterpret_cast<float*>(a.qout), st);
+        return ILM_OK;
+    };
+    if (!s->use_graph) {
+        rc = launch_frame();
+        if (rc) return rc;
+    } else {
+        // everything the captured launches bake in
+        const std::vector<int64_t> key = {
+            (int64_t)(uintptr_t)nrc, (int64_t)(uintptr_t)accum, p.width, p.height, p.spp, p.max_depth, p.rr_depth,
+            (int64_t)ilm::float_bits(p.eps), cap, do_accum, prefix, max_depth_train, (int64_t)ilm::float_bits(train_frac),
+            train_cap, (int64_t)(uintptr_t)s->mesh->d_nodes, s->mesh->node_format};
+        if (s->graph && s->graph_key == key) {
+            ILM_HIP(hipGraphLaunch(s->graph, st));
+        } else {
+            // first frame with these settings: run it, then capture the same
+            // sequence for the next frames
+            drop_graph(s);
+            rc = launch_frame();
             if (rc) return rc;
-            hipLaunchKernelGGL(ilm::k_nrc_resolve, dim3(grid), dim3(256), 0, st, a);
-            hipLaunchKernelGGL(ilm::k_nrc_collect, dim3((unsigned)((a.npaths + 255) / 256)), dim3(256), 0, st, a);
-            hipLaunchKernelGGL(ilm::k_nrc_reset_queries, dim3(1), dim3(1), 0, st, a.counts);
+            ILM_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+            int crc = launch_frame();
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(st, &g);
+            if (crc) {
+                if (g) (void)hipGraphDestroy(g);
+                return crc;
+            }
+            if (e != hipSuccess) return ilm::fail_hip(e, "frame capture");
+            e = hipGraphInstantiate(&s->graph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) {
+                s->graph = nullptr;
+                return ilm::fail_hip(e, "frame graph instantiation");
+            }
+            s->graph_key = key;
         }
-        if (do_accum)
-            hipLaunchKernelGGL(ilm::k_accumulate, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, st, a, npix);
-        ILM_HIP(hipGetLastError());
     }
     for (int k = 0; k < train_steps; ++k) {
         rc = ilm_nrc_train_step(nrc, a.tq, reinterpret_cast<const float*>(a.tt), train_cap, nullptr, st);
@@ -760,6 +827,13 @@ int render_impl(ilm_scene* s, ilm_nrc* nrc, const ilm_render_params* prm, const 
 }  // namespace
 
 extern "C" {
+
+int ilm_scene_set_graph(ilm_scene* s, int enable) {
+    if (!s) return fail(ILM_ERR_INVALID, "null scene");
+    s->use_graph = enable != 0;
+    if (!s->use_graph) drop_graph(s);
+    return ILM_OK;
+}
 
 int ilm_render(ilm_scene* s, const ilm_render_params* prm, double* accum, ilm_render_stats* stats) {
     return render_impl(s, nullptr, prm, nullptr, accum, stats);

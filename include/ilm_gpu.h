@@ -343,6 +343,16 @@ typedef struct {
 int ilm_render(ilm_scene* scene, const ilm_render_params* params, double* accum,
                ilm_render_stats* stats);
 
+/* enable = 1: ilm_render / ilm_render_nrc capture a frame's launch sequence
+ * (generate, the bounce loop's trace / shade / shadow launches, cache
+ * inference, accumulation) in a HIP graph on the first frame with given
+ * settings, and replay it for later frames with the same settings (image
+ * size, spp, depths, accumulator, cache and tree); only the seed and sample
+ * offset may change between replays. The cache's training steps run
+ * outside the graph. No reference counterpart: it replaces the per-tile
+ * thread loop of RenderThread (Sandbox/Multithreaded.h:207-263). */
+int ilm_scene_set_graph(ilm_scene* scene, int enable);
+
 /* ---------------------------------------------------------------- scene files */
 
 /* Wavefront OBJ (+ MTL) loading with the reference loader's rules

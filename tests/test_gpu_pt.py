@@ -83,6 +83,59 @@ def test_room_batches_and_determinism(gpu_ctx, ilm):
     assert np.array_equal(a, b)
 
 
+def test_graph_replays_match_eager(gpu_ctx, ilm):
+    """A captured frame replayed with new seeds equals the eager frame with the
+    same seed, on every replay, with eager work and synchronisation between
+    replays; also for the cache renderer with training off."""
+    from illuminaprt_amd import workloads as w
+    pos, nrm, idx, tm, tl, mats, lights = w.box_room()
+    cam = w.room_camera()
+    W = H = 64
+    m = ilm.Mesh(gpu_ctx, pos, idx)
+    m.build()
+    sc = ilm.Scene(gpu_ctx, m, nrm, tm, tl, mats, lights)
+    sc2 = ilm.Scene(gpu_ctx, m, nrm, tm, tl, mats, lights)   # eager work between replays
+    acc = gpu_ctx.buffer(W * H * 3 * 8)
+    other = gpu_ctx.buffer(W * H * 3 * 8)
+    seeds = [11, 12, 13, 14]
+    eager = {}
+    for sd in seeds:
+        acc.zero()
+        sc.render(cam, W, H, 4, acc, seed=sd, spp_offset=sd)
+        eager[sd] = acc.download(np.float64, W * H * 3)
+    sc.set_graph(True)
+    for rep in range(2):
+        for sd in seeds:
+            acc.zero()
+            sc.render(cam, W, H, 4, acc, seed=sd, spp_offset=sd)
+            got = acc.download(np.float64, W * H * 3)
+            assert np.array_equal(got, eager[sd]), (rep, sd)
+            other.zero()
+            sc2.render(cam, W, H, 2, other, seed=99 + rep)
+            gpu_ctx.sync()
+    # cache renderer, no training: replays equal eager frames
+    net = ilm.Nrc(gpu_ctx, (0, 0, 0), (1, 1, 1), seed=3)
+    ne = {}
+    sc.set_graph(False)
+    for sd in seeds[:2]:
+        acc.zero()
+        sc.render_nrc(net, cam, W, H, 2, acc, seed=sd, train_steps=0)
+        ne[sd] = acc.download(np.float64, W * H * 3)
+    sc.set_graph(True)
+    for rep in range(3):
+        for sd in seeds[:2]:
+            acc.zero()
+            sc.render_nrc(net, cam, W, H, 2, acc, seed=sd, train_steps=0)
+            assert np.array_equal(acc.download(np.float64, W * H * 3), ne[sd]), (rep, sd)
+    sc.set_graph(False)
+    net.close()
+    for b in (acc, other):
+        b.free()
+    sc2.close()
+    sc.close()
+    m.close()
+
+
 def _closed_box(albedo, le):
     c = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]], np.float32)
     faces = [(0, 1, 2, 3), (4, 7, 6, 5), (0, 4, 5, 1), (3, 2, 6, 7), (0, 3, 7, 4), (1, 5, 6, 2)]
