@@ -73,6 +73,22 @@ class RenderParams(C.Structure):
                 ("eps", C.c_float), ("seed", C.c_uint32), ("batch_paths", C.c_int32)]
 
 
+class BatchView(C.Structure):
+    _fields_ = [("scene", C.c_int32), ("cam", Camera), ("width", C.c_int32), ("height", C.c_int32),
+                ("spp", C.c_int32)]
+
+
+class BatchTuning(C.Structure):
+    _fields_ = [("node_format", C.c_int32), ("batch_paths", C.c_int64), ("best_ms", C.c_double),
+                ("candidates", C.c_int32), ("candidate_format", C.c_int32 * 8), ("candidate_batch", C.c_int64 * 8),
+                ("candidate_ms", C.c_double * 8)]
+
+    def as_dict(self):
+        n = self.candidates
+        return {"node_format": self.node_format, "batch_paths": self.batch_paths, "best_ms": self.best_ms,
+                "table": [(self.candidate_format[i], self.candidate_batch[i], self.candidate_ms[i]) for i in range(n)]}
+
+
 class NrcParams(C.Structure):
     _fields_ = [("bounds_lo", C.c_float * 3), ("bounds_hi", C.c_float * 3), ("lr", C.c_float),
                 ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float), ("seed", C.c_uint32)]
@@ -150,6 +166,14 @@ SIGNATURES = {
     "ilm_nrc_infer": (C.c_int, [_vp, _vp, _i64, _vp, _vp]),
     "ilm_nrc_train_step": (C.c_int, [_vp, _vp, _vp, _i64, _vp, _vp]),
     "ilm_scene_set_graph": (C.c_int, [_vp, C.c_int]),
+    "ilm_batch_create": (C.c_int, [_vp, C.POINTER(_vp), C.c_int, C.POINTER(_vp)]),
+    "ilm_batch_destroy": (C.c_int, [_vp]),
+    "ilm_render_batch": (C.c_int, [_vp, C.POINTER(BatchView), C.c_int, C.POINTER(RenderParams), _vp,
+                                   C.POINTER(RenderStats)]),
+    "ilm_batch_set_graph": (C.c_int, [_vp, C.c_int]),
+    "ilm_batch_configure": (C.c_int, [_vp, C.c_int, _i64]),
+    "ilm_batch_autotune": (C.c_int, [_vp, C.POINTER(BatchView), C.c_int, C.POINTER(RenderParams),
+                                     C.POINTER(BatchTuning)]),
     "ilm_render_nrc": (C.c_int, [_vp, _vp, C.POINTER(RenderParams), C.POINTER(NrcRenderParams), _vp,
                                  C.POINTER(RenderStats)]),
 }
@@ -334,6 +358,51 @@ class Mesh:
         if self.h and not getattr(self, "borrowed", False):
             check(load().ilm_mesh_destroy(self.h))
         self.h = None
+
+
+class Batch:
+    """Several scenes rendered by one frame's launch sequence."""
+
+    def __init__(self, ctx: Context, scenes):
+        self.ctx = ctx
+        arr = (C.c_void_p * len(scenes))(*[sc.h for sc in scenes])
+        h = C.c_void_p()
+        check(load().ilm_batch_create(ctx.h, arr, len(scenes), C.byref(h)))
+        self.h = h.value
+
+    @staticmethod
+    def _views(views):
+        arr = (BatchView * len(views))()
+        for i, (scene, cam, w, h, spp) in enumerate(views):
+            arr[i].scene, arr[i].cam, arr[i].width, arr[i].height, arr[i].spp = scene, cam, w, h, spp
+        return arr
+
+    def render(self, views, accum: DeviceBuffer, max_depth=5, rr_depth=4, eps=1e-4, seed=1234, spp_offset=0,
+               batch_paths=0) -> dict:
+        """views: list of (scene index, Camera, width, height, spp)."""
+        arr = self._views(views)
+        p = RenderParams(Camera(), 0, 0, 0, spp_offset, max_depth, rr_depth, eps, seed, batch_paths)
+        st = RenderStats()
+        check(load().ilm_render_batch(self.h, arr, len(views), C.byref(p), accum.ptr, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in st._fields_}
+
+    def autotune(self, views, max_depth=5, rr_depth=4, eps=1e-4, seed=1234) -> dict:
+        arr = self._views(views)
+        p = RenderParams(Camera(), 0, 0, 0, 0, max_depth, rr_depth, eps, seed, 0)
+        t = BatchTuning()
+        check(load().ilm_batch_autotune(self.h, arr, len(views), C.byref(p), C.byref(t)))
+        return t.as_dict()
+
+    def configure(self, node_format=ILM_NODES_FLOAT, batch_paths=0):
+        check(load().ilm_batch_configure(self.h, node_format, batch_paths))
+
+    def set_graph(self, enable: bool):
+        check(load().ilm_batch_set_graph(self.h, int(enable)))
+
+    def close(self):
+        if self.h:
+            load().ilm_batch_destroy(self.h)
+            self.h = None
 
 
 class Space:

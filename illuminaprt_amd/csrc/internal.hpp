@@ -66,8 +66,10 @@ hipStream_t pick_stream(ilm_ctx* ctx, void* stream);
 void camera_frame(const ilm_camera& c, float out[12]);
 // Closest hit (occ == null) or any hit over a ray queue whose length lives in
 // device memory (*n_dev, at most `capacity`), without a host round trip.
+// rooted: each ray's tmin field holds (as bits) the BVH node to start from,
+// so one launch can trace rays of several BVHs packed in one node array.
 int trace_queue(ilm_mesh* m, const ilm_ray* rays, const unsigned long long* n_dev, int64_t capacity, ilm_hit* hits,
-                uint8_t* occ, hipStream_t stream);
+                uint8_t* occ, hipStream_t stream, bool rooted = false);
 // Builds m's BVH4 on its device (linear BVH + collapse); fills the device
 // arrays, info and host copies as ilm_bvh_build does.
 int build_lbvh(ilm_mesh* m, int max_leaf, int node_format);

@@ -17,6 +17,7 @@
 #include <string>
 #include <vector>
 
+#include "bvh_build.hpp"
 #include "geom.hpp"
 #include "internal.hpp"
 
@@ -105,6 +106,19 @@ struct CamK {
     f3 centre, corner, across, up;
 };
 
+// One image of a frame: its camera, size and samples, the lights and BVH root
+// of its scene, and where its paths and pixels start in the frame's global
+// path and pixel numbering (views are stored in that order). A single-scene
+// frame is one view with root 0 and offsets 0.
+struct View {
+    CamK cam;
+    int W, H, spp;
+    int l0, l1;       // lights [l0, l1) of the view's scene
+    uint32_t root;    // BVH root node of the view's scene
+    int64_t path0;    // first global path
+    int64_t pix0;     // first global pixel (accumulator offset / 3)
+};
+
 struct PtArgs {
     // scene
     const float4* shade;     // 6 float4 per triangle: (v0, mat), (v1, light), (v2, 0), n0, n1, n2
@@ -112,12 +126,11 @@ struct PtArgs {
     const float4* ltris;     // emitter triangles, 3 float4 each
     const float4* lights;    // (Le rgb, area)
     const int2* lrange;      // (first emitter triangle, count)
-    int nlights;
-    // camera and pixels
-    CamK cam;
-    int W, H;
-    int64_t pix0;
-    int spp;
+    int nlights;             // lights of all views' scenes
+    // views, and the batch of global paths / pixels this launch covers
+    const View* views;
+    int nviews;
+    int64_t bpath0, bpix0;
     // per-frame constants in device memory, so that a captured frame can be
     // replayed with a new seed and sample offset: [0] seed, [1] spp_offset
     const int32_t* frame;
@@ -154,9 +167,25 @@ struct PtArgs {
     float4* tt;              // training targets (train_cap x (rgb, weight))
 };
 
-__device__ __forceinline__ void path_pixel(const PtArgs& a, int64_t p, int64_t& pixel, uint32_t& sample) {
-    pixel = a.pix0 + p / a.spp;
-    sample = (uint32_t)(a.frame[1] + (int)(p % a.spp));
+// last view whose first path (or pixel) is <= g
+__device__ __forceinline__ int find_view(const View* v, int n, int64_t g, bool by_pixel) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if ((by_pixel ? v[mid].pix0 : v[mid].path0) <= g) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Path p of the batch: its view, global pixel (the RNG's pixel counter and
+// the accumulator index) and sample index.
+__device__ __forceinline__ void path_pixel(const PtArgs& a, int64_t p, int& vi, int64_t& pixel, uint32_t& sample) {
+    const int64_t g = a.bpath0 + p;
+    vi = find_view(a.views, a.nviews, g, false);
+    const int64_t local = g - a.views[vi].path0;
+    const int spp = a.views[vi].spp;
+    pixel = a.views[vi].pix0 + local / spp;
+    sample = (uint32_t)(a.frame[1] + (int)(local % spp));
 }
 
 __global__ void k_generate(PtArgs a) {
@@ -167,20 +196,25 @@ __global__ void k_generate(PtArgs a) {
         a.counts[2] = 0ull;
     }
     if (p >= a.npaths) return;
+    int vi;
     int64_t pixel;
     uint32_t sample;
-    path_pixel(a, p, pixel, sample);
+    path_pixel(a, p, vi, pixel, sample);
+    const View& v = a.views[vi];
     Block8 b = sample_block((uint32_t)a.frame[0], (uint64_t)pixel, sample, 0, 0);
-    int x = (int)(pixel % a.W), y = (int)(pixel / a.W);
+    const int64_t lp = pixel - v.pix0;
+    int x = (int)(lp % v.W), y = (int)(lp / v.W);
     // PerspectiveCamera::GetRay (Camera/PerspectiveCamera.cpp:53-60) as fed by
     // RenderRegion with the box filter's -0.5 shift (BaseRenderer.cpp:106-118)
-    float rcpW = 1.f / (float)a.W, rcpH = 1.f / (float)a.H;
+    float rcpW = 1.f / (float)v.W, rcpH = 1.f / (float)v.H;
     float nx = (float)x * rcpW, ny = (float)y * rcpH;
     float xi1 = (b.v[0] - 0.5f) * rcpW, xi2 = (b.v[1] - 0.5f) * rcpH;
-    f3 t = a.cam.corner + a.cam.across * (nx + xi1) + a.cam.up * (ny + xi2);
-    f3 d = normalize(t - a.cam.centre);
+    f3 t = v.cam.corner + v.cam.across * (nx + xi1) + v.cam.up * (ny + xi2);
+    f3 d = normalize(t - v.cam.centre);
+    // tmin carries the BVH root of the view's scene (the traversal's t window
+    // starts at 1e-5 regardless of tmin, as the reference's triangle test does)
     ilm_ray r;
-    r.ox = a.cam.centre.x; r.oy = a.cam.centre.y; r.oz = a.cam.centre.z; r.tmin = 0.f;
+    r.ox = v.cam.centre.x; r.oy = v.cam.centre.y; r.oz = v.cam.centre.z; r.tmin = __uint_as_float(v.root);
     r.dx = d.x; r.dy = d.y; r.dz = d.z; r.tmax = FLT_MAX;
     a.rays[0][p] = r;
     a.qpath[0][p] = (int32_t)p;
@@ -218,6 +252,7 @@ __global__ void __launch_bounds__(kPtBlock) k_shade(PtArgs a, int bounce, int cu
         bool train = false;
         int64_t pixel = 0;
         uint32_t sample = 0;
+        int vi = 0;
         if (live) {
             const float4* sr = a.shade + 6 * (int64_t)h.prim;
             float4 A = sr[0], B = sr[1], C = sr[2], N0 = sr[3], N1 = sr[4], N2 = sr[5];
@@ -237,7 +272,7 @@ __global__ void __launch_bounds__(kPtBlock) k_shade(PtArgs a, int bounce, int cu
                 float4 t4 = a.tbeta[p];
                 tbeta = mk(t4.x, t4.y, t4.z);
             }
-            path_pixel(a, p, pixel, sample);
+            path_pixel(a, p, vi, pixel, sample);
             if (mat_id < 0) live = false;  // no material: the path ends (PathIntegrator.cpp:110-111)
         }
         Material mat;
@@ -298,12 +333,19 @@ __global__ void __launch_bounds__(kPtBlock) k_shade(PtArgs a, int bounce, int cu
         // next-event estimation over every light except the one hit
         // (SampleAllLights / EstimateDirectLighting, Integrator.cpp:114-185)
         Basis gb = basis_from_w(Ng);
-        for (int l = 0; l < a.nlights; ++l) {
+        // the view's scene's lights (uniform bounds: the loop's wave appends
+        // must be reached by every lane together)
+        const int lbeg = 0, lend = a.nlights;
+        const int vl0 = a.views[vi].l0, vl1 = a.views[vi].l1;
+        const uint32_t root = a.views[vi].root;
+        for (int l = lbeg; l < lend; ++l) {
+            if (__ballot(live && l >= vl0 && l < vl1) == 0ull) continue;
             bool want = false;
             ilm_ray sray;
             f3 contrib = mk(0, 0, 0), contrib2 = mk(0, 0, 0);
-            if (live && l != light_id) {
-                Block8 lb = sample_block((uint32_t)a.frame[0], (uint64_t)pixel, sample, (uint32_t)bounce, 1u + (uint32_t)l);
+            if (live && l != light_id && l >= vl0 && l < vl1) {
+                Block8 lb = sample_block((uint32_t)a.frame[0], (uint64_t)pixel, sample, (uint32_t)bounce,
+                                         1u + (uint32_t)(l - vl0));
                 int2 rg = a.lrange[l];
                 float4 le = a.lights[l];
                 int kk = min((int)((float)rg.y * lb.v[2]), rg.y - 1);
@@ -332,7 +374,8 @@ __global__ void __launch_bounds__(kPtBlock) k_shade(PtArgs a, int bounce, int cu
                     if (!is_black(contrib) || !is_black(contrib2)) {
                         want = true;
                         f3 sd = Q - P;  // VisibilityQuery::SetSegment (Visibility.cpp:50-53)
-                        sray.ox = P.x; sray.oy = P.y; sray.oz = P.z; sray.tmin = 1e-4f;
+                        // tmin (1e-4 in the reference; the t window is fixed) carries the root
+                        sray.ox = P.x; sray.oy = P.y; sray.oz = P.z; sray.tmin = __uint_as_float(root);
                         sray.dx = sd.x; sray.dy = sd.y; sray.dz = sd.z; sray.tmax = 1.0f - 1e-4f;
                     }
                 }
@@ -370,7 +413,7 @@ __global__ void __launch_bounds__(kPtBlock) k_shade(PtArgs a, int bounce, int cu
                 if (last) cont = false;  // the last vertex's continuation ray is never used
                 if (cont) {
                     f3 org = P + Ng * a.eps;  // offset along the unflipped geometric normal
-                    nray.ox = org.x; nray.oy = org.y; nray.oz = org.z; nray.tmin = a.eps;
+                    nray.ox = org.x; nray.oy = org.y; nray.oz = org.z; nray.tmin = __uint_as_float(root);
                     nray.dx = wIn.x; nray.dy = wIn.y; nray.dz = wIn.z; nray.tmax = FLT_MAX;
                 }
             }
@@ -457,14 +500,17 @@ __global__ void k_end_bounce(unsigned long long* counts, int cur) {
 __global__ void k_accumulate(PtArgs a, int64_t npix) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= npix) return;
+    const int64_t gp = a.bpix0 + i;
+    const View& v = a.views[find_view(a.views, a.nviews, gp, true)];
+    const int64_t p0 = v.path0 + (gp - v.pix0) * v.spp - a.bpath0;  // the pixel's first path in the batch
     double s0 = 0, s1 = 0, s2 = 0;
-    for (int s = 0; s < a.spp; ++s) {
-        int64_t p = i * a.spp + s;
+    for (int s = 0; s < v.spp; ++s) {
+        int64_t p = p0 + s;
         s0 += (double)a.L[3 * p];
         s1 += (double)a.L[3 * p + 1];
         s2 += (double)a.L[3 * p + 2];
     }
-    double* o = a.accum + 3 * (a.pix0 + i);
+    double* o = a.accum + 3 * gp;
     o[0] += s0;
     o[1] += s1;
     o[2] += s2;
@@ -484,6 +530,7 @@ struct ilm_scene {
     int nlights = 0, nmat = 0;
     // path-state buffers (grown on demand)
     int64_t cap = 0;
+    int state_nl = 0;
     bool cache_ready = false;  // radiance-cache buffers allocated for `cap` paths
     int64_t train_cap = 0;
     std::vector<void*> bufs;
@@ -491,11 +538,19 @@ struct ilm_scene {
     // frame constants (device) and their host copy
     int32_t* d_frame = nullptr;
     int32_t h_frame[2] = {0, 0};
+    // the frame's views (device; reallocated when more are needed) and host copy
+    ilm::View* d_views = nullptr;
+    int views_cap = 0;
+    std::vector<ilm::View> h_views;
     // captured frame: replayed while the key (everything baked into the
     // launches) is unchanged
     bool use_graph = false;
     hipGraphExec_t graph = nullptr;
     std::vector<int64_t> graph_key;
+    // host copies of the shading arrays (merged by batches)
+    std::vector<float> h_shade, h_mats, h_lt, h_lrec;
+    std::vector<int32_t> h_lrange;
+    bool owns_mesh = false;  // a batch's merged scene owns its packed mesh
 };
 
 namespace {
@@ -511,6 +566,7 @@ void scene_free_state(ilm_scene* s) {
     for (void* p : s->bufs) ilm_free(s->ctx, p);
     s->bufs.clear();
     s->cap = 0;
+    s->state_nl = 0;
     s->cache_ready = false;
     s->train_cap = 0;
 }
@@ -521,11 +577,40 @@ int scene_alloc(ilm_scene* s, size_t bytes, void** out) {
     return rc;
 }
 
-int ensure_state(ilm_scene* s, int64_t cap, bool cache = false, int64_t train_cap = 0) {
-    if (s->cap >= cap && (!cache || (s->cache_ready && s->train_cap == train_cap))) return ILM_OK;
+// Path state for `cap` paths, each sending at most `nl` shadow rays per bounce.
+// Uploads the scene's shading arrays (its host copies) to the device.
+int scene_upload(ilm_scene* s) {
+    ilm_ctx* ctx = s->ctx;
+    int rc = 0;
+    rc = rc ? rc : scene_alloc(s, s->h_shade.size() * 4, (void**)&s->d_shade);
+    rc = rc ? rc : scene_alloc(s, s->h_mats.size() * 4, (void**)&s->d_mats);
+    rc = rc ? rc : scene_alloc(s, s->h_lt.size() * 4, (void**)&s->d_ltris);
+    rc = rc ? rc : scene_alloc(s, s->h_lrec.size() * 4, (void**)&s->d_lights);
+    rc = rc ? rc : scene_alloc(s, s->h_lrange.size() * 4, (void**)&s->d_lrange);
+    rc = rc ? rc : scene_alloc(s, 4 * sizeof(int32_t), (void**)&s->d_frame);
+    rc = rc ? rc : ilm_upload(ctx, s->d_shade, s->h_shade.data(), s->h_shade.size() * 4);
+    rc = rc ? rc : ilm_upload(ctx, s->d_mats, s->h_mats.data(), s->h_mats.size() * 4);
+    rc = rc ? rc : ilm_upload(ctx, s->d_ltris, s->h_lt.data(), s->h_lt.size() * 4);
+    rc = rc ? rc : ilm_upload(ctx, s->d_lights, s->h_lrec.data(), s->h_lrec.size() * 4);
+    rc = rc ? rc : ilm_upload(ctx, s->d_lrange, s->h_lrange.data(), s->h_lrange.size() * 4);
+    if (rc) {
+        std::string msg = ilm_last_error();
+        for (void* p : s->bufs) ilm_free(ctx, p);
+        s->bufs.clear();
+        return ilm::fail(rc, msg);
+    }
+    // scene buffers are not path state: keep them out of the state list
+    s->bufs.clear();
+    return ILM_OK;
+}
+
+int ensure_state(ilm_scene* s, int64_t cap, int nl, bool cache = false, int64_t train_cap = 0) {
+    nl = std::max(1, nl);
+    if (s->cap >= cap && s->state_nl >= nl && (!cache || (s->cache_ready && s->train_cap == train_cap)))
+        return ILM_OK;
     cap = std::max(cap, s->cap);
+    nl = std::max(nl, s->state_nl);
     scene_free_state(s);
-    int nl = std::max(1, s->nlights);
     ilm::PtArgs& a = s->a;
     int rc = 0;
     if (cache) {
@@ -557,6 +642,7 @@ int ensure_state(ilm_scene* s, int64_t cap, bool cache = false, int64_t train_ca
         return ILM_ERR_NOMEM;
     }
     s->cap = cap;
+    s->state_nl = nl;
     return ILM_OK;
 }
 
@@ -626,26 +712,16 @@ int ilm_scene_create(ilm_ctx* ctx, ilm_mesh* mesh, const float* nrm, const int32
     s->mesh = mesh;
     s->nlights = nlight;
     s->nmat = nmat;
-    int rc = 0;
-    rc = rc ? rc : scene_alloc(s, shade.size() * 4, (void**)&s->d_shade);
-    rc = rc ? rc : scene_alloc(s, (size_t)nmat * 8 * 4, (void**)&s->d_mats);
-    rc = rc ? rc : scene_alloc(s, lt.size() * 4, (void**)&s->d_ltris);
-    rc = rc ? rc : scene_alloc(s, lrec.size() * 4, (void**)&s->d_lights);
-    rc = rc ? rc : scene_alloc(s, lrange.size() * 4, (void**)&s->d_lrange);
-    rc = rc ? rc : scene_alloc(s, 4 * sizeof(int32_t), (void**)&s->d_frame);
-    rc = rc ? rc : ilm_upload(ctx, s->d_shade, shade.data(), shade.size() * 4);
-    rc = rc ? rc : ilm_upload(ctx, s->d_mats, mats, (size_t)nmat * 8 * 4);
-    rc = rc ? rc : ilm_upload(ctx, s->d_ltris, lt.data(), lt.size() * 4);
-    rc = rc ? rc : ilm_upload(ctx, s->d_lights, lrec.data(), lrec.size() * 4);
-    rc = rc ? rc : ilm_upload(ctx, s->d_lrange, lrange.data(), lrange.size() * 4);
+    s->h_shade.swap(shade);
+    s->h_mats.assign(mats, mats + (size_t)nmat * 8);
+    s->h_lt.swap(lt);
+    s->h_lrec.swap(lrec);
+    s->h_lrange.swap(lrange);
+    int rc = scene_upload(s);
     if (rc) {
-        std::string msg = ilm_last_error();
-        for (void* p : s->bufs) ilm_free(ctx, p);
         delete s;
-        return fail(rc, msg);
+        return rc;
     }
-    // scene buffers are not path state: keep them out of the state list
-    s->bufs.clear();
     *out = s;
     return ILM_OK;
 }
@@ -654,8 +730,9 @@ int ilm_scene_destroy(ilm_scene* s) {
     if (!s) return ILM_OK;
     scene_free_state(s);
     for (void* p : {(void*)s->d_shade, (void*)s->d_mats, (void*)s->d_ltris, (void*)s->d_lights, (void*)s->d_lrange,
-                    (void*)s->d_frame})
+                    (void*)s->d_frame, (void*)s->d_views})
         if (p) ilm_free(s->ctx, p);
+    if (s->owns_mesh) ilm_mesh_destroy(s->mesh);
     delete s;
     return ILM_OK;
 }
@@ -664,12 +741,13 @@ int ilm_scene_destroy(ilm_scene* s) {
 
 namespace {
 
-int render_impl(ilm_scene* s, ilm_nrc* nrc, const ilm_render_params* prm, const ilm_nrc_render_params* np,
-                double* accum, ilm_render_stats* stats) {
-    if (!s || !prm || (!accum && !(np && !np->accumulate))) return fail(ILM_ERR_INVALID, "null argument");
-    const ilm_render_params& p = *prm;
-    if (p.width <= 0 || p.height <= 0 || p.spp <= 0 || p.max_depth <= 0)
-        return fail(ILM_ERR_INVALID, "width, height, spp and max_depth must be positive");
+// One frame over `views` (all of the scene `s`, whose lights and BVH roots the
+// views reference). Paths are numbered view by view, pixel-major, spp per
+// pixel; batches of at most batch_paths paths end at pixel boundaries.
+int render_views(ilm_scene* s, ilm_nrc* nrc, const std::vector<ilm::View>& views_in, const ilm_render_params& p,
+                 const ilm_nrc_render_params* np, double* accum, ilm_render_stats* stats) {
+    if (!accum && !(np && !np->accumulate)) return fail(ILM_ERR_INVALID, "null accumulator");
+    if (views_in.empty() || p.max_depth <= 0) return fail(ILM_ERR_INVALID, "no views or max_depth not positive");
     const bool cache = nrc != nullptr;
     int prefix = -1, max_depth_train = p.max_depth, train_steps = 0;
     int64_t train_cap = 0;
@@ -684,14 +762,57 @@ int render_impl(ilm_scene* s, ilm_nrc* nrc, const ilm_render_params* prm, const 
     }
     ilm_ctx* ctx = s->ctx;
     ILM_HIP(hipSetDevice(ctx->device));
-    const int64_t npix_total = (int64_t)p.width * p.height;
-    int64_t want = p.batch_paths > 0 ? p.batch_paths : (int64_t(1) << 22);
-    int64_t ppb = std::max<int64_t>(1, want / p.spp);          // pixels per batch
-    ppb = std::min(ppb, npix_total);
-    const int64_t cap = ppb * p.spp;
+    // global path / pixel offsets of the views
+    std::vector<ilm::View>& views = s->h_views;
+    views = views_in;
+    int64_t total_paths = 0, total_pix = 0;
+    int max_spp = 1, max_vl = 1;
+    for (ilm::View& v : views) {
+        if (v.W <= 0 || v.H <= 0 || v.spp <= 0) return fail(ILM_ERR_INVALID, "width, height and spp must be positive");
+        v.path0 = total_paths;
+        v.pix0 = total_pix;
+        total_pix += (int64_t)v.W * v.H;
+        total_paths += (int64_t)v.W * v.H * v.spp;
+        max_spp = std::max(max_spp, v.spp);
+        max_vl = std::max(max_vl, v.l1 - v.l0);
+    }
+    const int64_t want = std::max<int64_t>(p.batch_paths > 0 ? p.batch_paths : (int64_t(1) << 22), max_spp);
+    struct Batch { int64_t pix0, npix, path0, npaths; };
+    std::vector<Batch> batches;
+    {
+        Batch b{0, 0, 0, 0};
+        for (const ilm::View& v : views) {
+            int64_t left = (int64_t)v.W * v.H, gp = v.pix0;
+            while (left > 0) {
+                int64_t k = std::min(left, (want - b.npaths) / v.spp);
+                if (k == 0) {
+                    batches.push_back(b);
+                    b = Batch{gp, 0, v.path0 + (gp - v.pix0) * v.spp, 0};
+                    continue;
+                }
+                if (b.npix == 0) b = Batch{gp, 0, v.path0 + (gp - v.pix0) * v.spp, 0};
+                b.npix += k;
+                b.npaths += k * v.spp;
+                gp += k;
+                left -= k;
+            }
+        }
+        if (b.npix > 0) batches.push_back(b);
+    }
+    int64_t cap = 0;
+    for (const Batch& b : batches) cap = std::max(cap, b.npaths);
     if (cap > (int64_t(1) << 31) - 1) return fail(ILM_ERR_INVALID, "batch too large");
-    int rc = ensure_state(s, cap, cache, train_cap);
+    int rc = ensure_state(s, cap, max_vl, cache, train_cap);
     if (rc) return fail(rc, "path state allocation");
+    if (s->views_cap < (int)views.size()) {
+        drop_graph(s);
+        if (s->d_views) ilm_free(ctx, s->d_views);
+        s->d_views = nullptr;
+        s->views_cap = 0;
+        rc = ilm_alloc(ctx, views.size() * sizeof(ilm::View), (void**)&s->d_views);
+        if (rc) return rc;
+        s->views_cap = (int)views.size();
+    }
     ilm::PtArgs a = s->a;
     a.nrc_prefix = prefix;
     a.max_depth_train = max_depth_train;
@@ -705,18 +826,8 @@ int render_impl(ilm_scene* s, ilm_nrc* nrc, const ilm_render_params* prm, const 
     a.lights = s->d_lights;
     a.lrange = s->d_lrange;
     a.nlights = s->nlights;
-    {
-        float fr[12];
-        ilm::camera_frame(p.cam, fr);  // same frame as ilm_raygen_camera
-        a.cam.centre = ilm::mk(fr[0], fr[1], fr[2]);
-        a.cam.corner = ilm::mk(fr[3], fr[4], fr[5]);
-        a.cam.across = ilm::mk(fr[6], fr[7], fr[8]);
-        a.cam.up = ilm::mk(fr[9], fr[10], fr[11]);
-    }
-    a.W = p.width;
-    a.H = p.height;
-    a.spp = p.spp;
-
+    a.views = s->d_views;
+    a.nviews = (int)views.size();
     a.max_depth = p.max_depth;
     a.rr_depth = p.rr_depth;
     a.eps = p.eps;
@@ -726,10 +837,11 @@ int render_impl(ilm_scene* s, ilm_nrc* nrc, const ilm_render_params* prm, const 
     int ev0 = 0, ev1 = 0;
     rc = ilm_timer_mark(ctx, &ev0);
     if (rc) return rc;
-    // per-frame constants: the only inputs that change between replays
+    // per-frame inputs: the only ones that may change between replays
     s->h_frame[0] = (int32_t)p.seed;
     s->h_frame[1] = p.spp_offset;
     ILM_HIP(hipMemcpyAsync(s->d_frame, s->h_frame, sizeof(s->h_frame), hipMemcpyHostToDevice, st));
+    ILM_HIP(hipMemcpyAsync(s->d_views, views.data(), views.size() * sizeof(ilm::View), hipMemcpyHostToDevice, st));
     const int grid = ctx->cu_count * 8;
     // The frame's launch sequence (no host synchronisation inside, counters
     // cleared by a kernel rather than a memset node, so it can be captured).
@@ -737,18 +849,19 @@ int render_impl(ilm_scene* s, ilm_nrc* nrc, const ilm_render_params* prm, const 
         hipLaunchKernelGGL(ilm::k_clear_counts, dim3(1), dim3(64), 0, st, a.counts);
         if (cache)
             hipLaunchKernelGGL(ilm::k_nrc_clear, dim3((unsigned)((train_cap + 255) / 256)), dim3(256), 0, st, a);
-        for (int64_t pix0 = 0; pix0 < npix_total; pix0 += ppb) {
-            int64_t npix = std::min(ppb, npix_total - pix0);
-            a.pix0 = pix0;
-            a.npaths = npix * p.spp;
+        for (const Batch& bt : batches) {
+            const int64_t npix = bt.npix;
+            a.bpix0 = bt.pix0;
+            a.bpath0 = bt.path0;
+            a.npaths = bt.npaths;
             hipLaunchKernelGGL(ilm::k_generate, dim3((unsigned)((a.npaths + 255) / 256)), dim3(256), 0, st, a);
             int cur = 0;
             for (int b = 0; b < max_bounces; ++b) {
-                int r2 = ilm::trace_queue(s->mesh, a.rays[cur], a.counts + cur, cap, a.hits, nullptr, st);
+                int r2 = ilm::trace_queue(s->mesh, a.rays[cur], a.counts + cur, cap, a.hits, nullptr, st, true);
                 if (r2) return r2;
                 hipLaunchKernelGGL(ilm::k_shade, dim3(grid), dim3(ilm::kPtBlock), 0, st, a, b, cur);
                 if (s->nlights > 0) {
-                    r2 = ilm::trace_queue(s->mesh, a.srays, a.counts + 2, cap * s->nlights, nullptr, a.socc, st);
+                    r2 = ilm::trace_queue(s->mesh, a.srays, a.counts + 2, cap * max_vl, nullptr, a.socc, st, true);
                     if (r2) return r2;
                     hipLaunchKernelGGL(ilm::k_shadow_accum, dim3(grid), dim3(256), 0, st, a);
                 }
@@ -774,10 +887,15 @@ int render_impl(ilm_scene* s, ilm_nrc* nrc, const ilm_render_params* prm, const 
         if (rc) return rc;
     } else {
         // everything the captured launches bake in
-        const std::vector<int64_t> key = {
-            (int64_t)(uintptr_t)nrc, (int64_t)(uintptr_t)accum, p.width, p.height, p.spp, p.max_depth, p.rr_depth,
+        std::vector<int64_t> key = {
+            (int64_t)(uintptr_t)nrc, (int64_t)(uintptr_t)accum, p.max_depth, p.rr_depth,
             (int64_t)ilm::float_bits(p.eps), cap, do_accum, prefix, max_depth_train, (int64_t)ilm::float_bits(train_frac),
-            train_cap, (int64_t)(uintptr_t)s->mesh->d_nodes, s->mesh->node_format};
+            train_cap, (int64_t)(uintptr_t)s->mesh->d_nodes, s->mesh->node_format, want, (int64_t)views.size()};
+        for (const ilm::View& v : views) {
+            key.push_back(v.W);
+            key.push_back(v.H);
+            key.push_back(v.spp);
+        }
         if (s->graph && s->graph_key == key) {
             ILM_HIP(hipGraphLaunch(s->graph, st));
         } else {
@@ -816,12 +934,38 @@ int render_impl(ilm_scene* s, ilm_nrc* nrc, const ilm_render_params* prm, const 
     unsigned long long cnt[8];
     ILM_HIP(hipMemcpy(cnt, a.counts, sizeof(cnt), hipMemcpyDeviceToHost));
     if (stats) {
-        stats->camera_rays = npix_total * p.spp;
+        stats->camera_rays = total_paths;
         stats->extension_rays = (int64_t)cnt[3] - stats->camera_rays;
         stats->shadow_rays = (int64_t)cnt[4];
         stats->ms = ms;
     }
     return ilm_sync(ctx);
+}
+
+ilm::View view_of(const ilm_render_params& p, int nlights) {
+    ilm::View v{};
+    float fr[12];
+    ilm::camera_frame(p.cam, fr);  // same frame as ilm_raygen_camera
+    v.cam.centre = ilm::mk(fr[0], fr[1], fr[2]);
+    v.cam.corner = ilm::mk(fr[3], fr[4], fr[5]);
+    v.cam.across = ilm::mk(fr[6], fr[7], fr[8]);
+    v.cam.up = ilm::mk(fr[9], fr[10], fr[11]);
+    v.W = p.width;
+    v.H = p.height;
+    v.spp = p.spp;
+    v.l0 = 0;
+    v.l1 = nlights;
+    v.root = 0;
+    return v;
+}
+
+int render_impl(ilm_scene* s, ilm_nrc* nrc, const ilm_render_params* prm, const ilm_nrc_render_params* np,
+                double* accum, ilm_render_stats* stats) {
+    if (!s || !prm) return fail(ILM_ERR_INVALID, "null argument");
+    if (prm->width <= 0 || prm->height <= 0 || prm->spp <= 0 || prm->max_depth <= 0)
+        return fail(ILM_ERR_INVALID, "width, height, spp and max_depth must be positive");
+    std::vector<ilm::View> views(1, view_of(*prm, s->nlights));
+    return render_views(s, nrc, views, *prm, np, accum, stats);
 }
 
 }  // namespace
@@ -843,6 +987,323 @@ int ilm_render_nrc(ilm_scene* s, ilm_nrc* nrc, const ilm_render_params* prm, con
                    double* accum, ilm_render_stats* stats) {
     if (!nrc) return fail(ILM_ERR_INVALID, "null cache");
     return render_impl(s, nrc, prm, np, accum, stats);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- batches
+// Several scenes rendered by one frame's launch sequence: their BVHs are
+// packed into one node array (each scene's nodes keep their shape; child
+// words are rebased), their triangles into one array with global indices,
+// and their shading, materials and lights are concatenated with rebased
+// indices. A view names its scene; the traversal starts each ray at its
+// scene's root (carried in the ray's tmin bits) and a path samples only its
+// scene's lights.
+struct ilm_batch {
+    ilm_ctx* ctx = nullptr;
+    ilm_scene* merged = nullptr;
+    std::vector<uint32_t> root;
+    std::vector<int32_t> lbase, nl;
+    void* d_nodes_float = nullptr;
+    void* d_nodes_quant = nullptr;
+    int64_t nodes = 0;
+    int node_format = ILM_NODES_FLOAT;
+    int64_t batch_paths = 0;  // 0: the library default
+};
+
+namespace {
+
+// Rebases the child words of packed nodes: internal children by the scene's
+// first node, leaves by its first triangle slot.
+void rebase_children(uint32_t* child, uint32_t node_base, uint32_t slot_base) {
+    for (int k = 0; k < 4; ++k) {
+        uint32_t c = child[k];
+        if (c == ilm::kEmptySlot) continue;
+        if (ilm::is_leaf(c)) child[k] = ilm::make_leaf(ilm::leaf_first(c) + slot_base, ilm::leaf_count(c));
+        else child[k] = c + node_base;
+    }
+}
+
+void batch_use_format(ilm_batch* b, int fmt) {
+    ilm_mesh* m = b->merged->mesh;
+    m->d_nodes = fmt == ILM_NODES_QUANTIZED ? b->d_nodes_quant : b->d_nodes_float;
+    m->node_format = fmt;
+    m->info.node_format = fmt;
+    m->info.node_stride = fmt == ILM_NODES_QUANTIZED ? (int32_t)sizeof(ilm::QNode4) : (int32_t)sizeof(ilm::WideNode<4>);
+    m->info.node_bytes = b->nodes * m->info.node_stride;
+    b->node_format = fmt;
+    drop_graph(b->merged);
+}
+
+}  // namespace
+
+extern "C" {
+
+int ilm_batch_destroy(ilm_batch* b) {
+    if (!b) return ILM_OK;
+    if (b->merged) {
+        ilm_mesh* m = b->merged->mesh;
+        // the mesh frees whichever node array it points at; free the other here
+        void* other = m->d_nodes == b->d_nodes_float ? b->d_nodes_quant : b->d_nodes_float;
+        if (other) ilm_free(b->ctx, other);
+        ilm_scene_destroy(b->merged);
+    }
+    delete b;
+    return ILM_OK;
+}
+
+int ilm_batch_create(ilm_ctx* ctx, ilm_scene* const* scenes, int n, ilm_batch** out) {
+    if (!ctx || !scenes || n <= 0 || !out) return fail(ILM_ERR_INVALID, "bad arguments");
+    *out = nullptr;
+    // counts and bases
+    int64_t nv = 0, nt = 0, slots = 0, nodes = 0;
+    int nmat = 0, nlight = 0;
+    int stack_need = 0, depth = 0, max_leaf = 0;
+    double sah = 0;
+    for (int i = 0; i < n; ++i) {
+        ilm_scene* s = scenes[i];
+        if (!s || s->ctx != ctx) return fail(ILM_ERR_INVALID, "scene missing or from another context");
+        ilm_mesh* m = s->mesh;
+        if (m->node_format != ILM_NODES_FLOAT || m->width != 4)
+            return fail(ILM_ERR_INVALID, "batched scenes need meshes built with float BVH4 nodes");
+        nv += m->nv;
+        nt += m->nt;
+        slots += (int64_t)m->h_order.size();
+        nodes += m->info.nodes;
+        nmat += s->nmat;
+        nlight += s->nlights;
+        stack_need = std::max(stack_need, m->info.stack_need);
+        depth = std::max(depth, m->info.depth);
+        max_leaf = std::max(max_leaf, m->info.max_leaf);
+        sah += m->info.sah_cost;
+    }
+    if (slots >= (int64_t(1) << 26) || nodes >= (int64_t(1) << 31) || nt >= (int64_t(1) << 31))
+        return fail(ILM_ERR_INVALID, "batch exceeds 2^26 triangle slots");
+    ilm_batch* b = new ilm_batch();
+    b->ctx = ctx;
+    b->nodes = nodes;
+    std::vector<float> pos;
+    std::vector<int32_t> idx, order;
+    std::vector<ilm::WideNode<4>> fnodes;
+    pos.reserve(3 * nv);
+    idx.reserve(3 * nt);
+    order.reserve(slots);
+    fnodes.reserve(nodes);
+    ilm_scene* ms = new ilm_scene();
+    ms->ctx = ctx;
+    int64_t vbase = 0, pbase = 0, sbase = 0, nbase = 0;
+    int mbase = 0, lb = 0, ltbase = 0;
+    for (int i = 0; i < n; ++i) {
+        ilm_scene* s = scenes[i];
+        ilm_mesh* m = s->mesh;
+        b->root.push_back((uint32_t)nbase);
+        b->lbase.push_back(lb);
+        b->nl.push_back(s->nlights);
+        pos.insert(pos.end(), m->pos.begin(), m->pos.end());
+        for (int32_t v : m->idx) idx.push_back((int32_t)(v + vbase));
+        for (int32_t o : m->h_order) order.push_back((int32_t)(o + pbase));
+        const auto* src = reinterpret_cast<const ilm::WideNode<4>*>(m->h_nodes.data());
+        for (int64_t k = 0; k < m->info.nodes; ++k) {
+            ilm::WideNode<4> w = src[k];
+            rebase_children(w.child, (uint32_t)nbase, (uint32_t)sbase);
+            fnodes.push_back(w);
+        }
+        // shading: rebase material and light ids
+        for (int64_t t = 0; t < m->nt; ++t) {
+            float rec[24];
+            memcpy(rec, s->h_shade.data() + 24 * t, sizeof rec);
+            int32_t mm, ll;
+            memcpy(&mm, &rec[3], 4);
+            memcpy(&ll, &rec[7], 4);
+            if (mm >= 0) mm += mbase;
+            if (ll >= 0) ll += lb;
+            memcpy(&rec[3], &mm, 4);
+            memcpy(&rec[7], &ll, 4);
+            ms->h_shade.insert(ms->h_shade.end(), rec, rec + 24);
+        }
+        ms->h_mats.insert(ms->h_mats.end(), s->h_mats.begin(), s->h_mats.end());
+        for (int l = 0; l < s->nlights; ++l) {
+            ms->h_lrange.push_back(s->h_lrange[2 * l] + ltbase);
+            ms->h_lrange.push_back(s->h_lrange[2 * l + 1]);
+            ms->h_lrec.insert(ms->h_lrec.end(), s->h_lrec.begin() + 4 * l, s->h_lrec.begin() + 4 * l + 4);
+        }
+        if (s->nlights > 0) {
+            int64_t nlt = 0;
+            for (int l = 0; l < s->nlights; ++l) nlt += s->h_lrange[2 * l + 1];
+            ms->h_lt.insert(ms->h_lt.end(), s->h_lt.begin(), s->h_lt.begin() + 12 * nlt);
+            ltbase += (int)nlt;
+        }
+        vbase += m->nv;
+        pbase += m->nt;
+        sbase += (int64_t)m->h_order.size();
+        nbase += m->info.nodes;
+        mbase += s->nmat;
+        lb += s->nlights;
+    }
+    if (ms->h_lt.empty()) ms->h_lt.assign(12, 0.f);
+    if (ms->h_lrec.empty()) ms->h_lrec.assign(4, 0.f);
+    if (ms->h_lrange.empty()) ms->h_lrange.assign(2, 0);
+    ms->nmat = nmat;
+    ms->nlights = nlight;
+    // the packed mesh: positions and indices for export, nodes and triangles on the device
+    ilm_mesh* pm = nullptr;
+    int rc = ilm_mesh_create(ctx, pos.data(), nv, idx.data(), nt, &pm);
+    if (rc) {
+        delete ms;
+        delete b;
+        return rc;
+    }
+    std::vector<float> tris;
+    ilm::pack_triangles(pos.data(), idx.data(), order, tris);
+    std::vector<ilm::QNode4> qnodes(nodes);
+    for (int64_t k = 0; k < nodes; ++k) ilm::quantize_node(fnodes[k], qnodes[k]);
+    rc = ilm_alloc(ctx, (size_t)nodes * sizeof(ilm::WideNode<4>), &b->d_nodes_float);
+    if (!rc) rc = ilm_alloc(ctx, (size_t)nodes * sizeof(ilm::QNode4), &b->d_nodes_quant);
+    if (!rc) rc = ilm_alloc(ctx, tris.size() * sizeof(float), (void**)&pm->d_tris);
+    if (!rc) rc = ilm_upload(ctx, b->d_nodes_float, fnodes.data(), (size_t)nodes * sizeof(ilm::WideNode<4>));
+    if (!rc) rc = ilm_upload(ctx, b->d_nodes_quant, qnodes.data(), (size_t)nodes * sizeof(ilm::QNode4));
+    if (!rc) rc = ilm_upload(ctx, pm->d_tris, tris.data(), tris.size() * sizeof(float));
+    if (rc) {
+        if (b->d_nodes_float) ilm_free(ctx, b->d_nodes_float);
+        if (b->d_nodes_quant) ilm_free(ctx, b->d_nodes_quant);
+        ilm_mesh_destroy(pm);
+        delete ms;
+        delete b;
+        return rc;
+    }
+    pm->d_nodes = b->d_nodes_float;
+    pm->width = 4;
+    pm->h_nodes.assign(reinterpret_cast<const uint8_t*>(fnodes.data()),
+                       reinterpret_cast<const uint8_t*>(fnodes.data()) + (size_t)nodes * sizeof(ilm::WideNode<4>));
+    pm->h_order.swap(order);
+    pm->info.nodes = nodes;
+    pm->info.tri_bytes = (int64_t)(tris.size() * sizeof(float));
+    pm->info.width = 4;
+    pm->info.depth = depth;
+    pm->info.max_leaf = max_leaf;
+    pm->info.stack_need = stack_need;
+    pm->info.sah_cost = sah;
+    pm->built = true;
+    ms->mesh = pm;
+    ms->owns_mesh = true;
+    rc = scene_upload(ms);
+    if (rc) {
+        ilm_free(ctx, b->d_nodes_quant);
+        ilm_mesh_destroy(pm);  // frees the float nodes and the triangles
+        delete ms;
+        delete b;
+        return rc;
+    }
+    b->merged = ms;
+    batch_use_format(b, ILM_NODES_FLOAT);
+    *out = b;
+    return ILM_OK;
+}
+
+int ilm_batch_set_graph(ilm_batch* b, int enable) {
+    if (!b) return fail(ILM_ERR_INVALID, "null batch");
+    return ilm_scene_set_graph(b->merged, enable);
+}
+
+int ilm_batch_configure(ilm_batch* b, int node_format, int64_t batch_paths) {
+    if (!b || batch_paths < 0) return fail(ILM_ERR_INVALID, "bad arguments");
+    if (node_format != ILM_NODES_FLOAT && node_format != ILM_NODES_QUANTIZED)
+        return fail(ILM_ERR_INVALID, "unknown node format");
+    batch_use_format(b, node_format);
+    b->batch_paths = batch_paths;
+    return ILM_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+int batch_views(ilm_batch* b, const ilm_batch_view* bv, int nviews, std::vector<ilm::View>& views) {
+    views.clear();
+    for (int i = 0; i < nviews; ++i) {
+        const ilm_batch_view& q = bv[i];
+        if (q.scene < 0 || q.scene >= (int)b->root.size()) return fail(ILM_ERR_INVALID, "view names no scene of the batch");
+        ilm_render_params p{};
+        p.cam = q.cam;
+        p.width = q.width;
+        p.height = q.height;
+        p.spp = q.spp;
+        ilm::View v = view_of(p, 0);
+        v.l0 = b->lbase[q.scene];
+        v.l1 = b->lbase[q.scene] + b->nl[q.scene];
+        v.root = b->root[q.scene];
+        views.push_back(v);
+    }
+    return ILM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ilm_render_batch(ilm_batch* b, const ilm_batch_view* bv, int nviews, const ilm_render_params* prm, double* accum,
+                     ilm_render_stats* stats) {
+    if (!b || !bv || nviews <= 0 || !prm) return fail(ILM_ERR_INVALID, "bad arguments");
+    std::vector<ilm::View> views;
+    int rc = batch_views(b, bv, nviews, views);
+    if (rc) return rc;
+    ilm_render_params p = *prm;
+    if (p.batch_paths <= 0 && b->batch_paths > 0) p.batch_paths = (int32_t)std::min<int64_t>(b->batch_paths, INT32_MAX);
+    return render_views(b->merged, nullptr, views, p, nullptr, accum, stats);
+}
+
+int ilm_batch_autotune(ilm_batch* b, const ilm_batch_view* bv, int nviews, const ilm_render_params* prm,
+                       ilm_batch_tuning* out) {
+    if (!b || !bv || nviews <= 0 || !prm) return fail(ILM_ERR_INVALID, "bad arguments");
+    // timing frames at (at most) 2 spp per pixel
+    std::vector<ilm_batch_view> tv(bv, bv + nviews);
+    int64_t npix = 0;
+    for (ilm_batch_view& v : tv) {
+        v.spp = std::min(v.spp, 2);
+        npix += (int64_t)v.width * v.height;
+    }
+    double* acc = nullptr;
+    int rc = ilm_alloc(b->ctx, (size_t)npix * 3 * sizeof(double), (void**)&acc);
+    if (rc) return rc;
+    const int fmts[2] = {ILM_NODES_FLOAT, ILM_NODES_QUANTIZED};
+    const int64_t sizes[3] = {int64_t(1) << 20, int64_t(1) << 22, int64_t(1) << 24};
+    ilm_batch_tuning t{};
+    t.best_ms = 1e30;
+    int k = 0;
+    for (int f : fmts)
+        for (int64_t sz : sizes) {
+            batch_use_format(b, f);
+            ilm_render_params p = *prm;
+            p.batch_paths = (int32_t)sz;
+            double best = 1e30;
+            for (int rep = 0; rep < 2 && rc == ILM_OK; ++rep) {
+                ilm_render_stats st{};
+                p.seed = prm->seed + 7919u * (uint32_t)rep;
+                std::vector<ilm::View> views;
+                rc = batch_views(b, tv.data(), nviews, views);
+                if (rc) break;
+                rc = render_views(b->merged, nullptr, views, p, nullptr, acc, &st);
+                if (rc) break;
+                best = std::min(best, st.ms);
+            }
+            if (rc) break;
+            t.candidate_ms[k] = best;
+            t.candidate_format[k] = f;
+            t.candidate_batch[k] = sz;
+            ++k;
+            if (best < t.best_ms) {
+                t.best_ms = best;
+                t.node_format = f;
+                t.batch_paths = sz;
+            }
+        }
+    ilm_free(b->ctx, acc);
+    if (rc) return rc;
+    t.candidates = k;
+    batch_use_format(b, t.node_format);
+    b->batch_paths = t.batch_paths;
+    if (out) *out = t;
+    return ILM_OK;
 }
 
 }  // extern "C"

@@ -34,6 +34,7 @@ struct TraceArgs {
     int32_t* __restrict__ stats;
     uint32_t* __restrict__ spill;
     int32_t spill_depth;
+    int32_t rooted;                   // start node of each ray from its tmin bits
     int64_t grid_threads;
     unsigned long long* counters;
     uint32_t total_waves;
@@ -169,7 +170,7 @@ __global__ void __launch_bounds__(kBlock) k_trace4(TraceArgs a) {
                     const float4* rp = reinterpret_cast<const float4*>(a.rays + my);
                     float4 o4 = rp[0], d4 = rp[1];
                     ray_setup(r, mk(o4.x, o4.y, o4.z), mk(d4.x, d4.y, d4.z), d4.w);
-                    cur = 0;
+                    cur = a.rooted ? __float_as_uint(o4.w) : 0u;
                     sp = 0;
                     lcnt = 0;
                     best_prim = -1;
@@ -234,7 +235,7 @@ __global__ void __launch_bounds__(kBlock) k_trace4(TraceArgs a) {
 
 template <bool ANY, bool STATS>
 static int launch_trace(ilm_mesh* m, const ilm_ray* rays, int64_t n, ilm_hit* hits, uint8_t* occ, int32_t* stats,
-                        void* stream, const unsigned long long* n_dev = nullptr) {
+                        void* stream, const unsigned long long* n_dev = nullptr, bool rooted = false) {
     if (!m) return fail(ILM_ERR_INVALID, "null mesh");
     if (!m->built) return fail(ILM_ERR_STATE, "BVH not built");
     if (!m->ctx) return fail(ILM_ERR_STATE, "mesh was created without a device context");
@@ -264,6 +265,7 @@ static int launch_trace(ilm_mesh* m, const ilm_ray* rays, int64_t n, ilm_hit* hi
     a.occ = occ;
     a.stats = stats;
     a.grid_threads = blocks * kBlock;
+    a.rooted = rooted ? 1 : 0;
     a.spill_depth = std::max(1, m->info.stack_need + 1);
     int rc = ensure_spill(ctx, (size_t)a.spill_depth * (size_t)a.grid_threads);
     if (rc) return rc;
@@ -494,9 +496,9 @@ __global__ void k_copy(const float4* __restrict__ src, float4* __restrict__ dst,
 
 namespace ilm {
 int trace_queue(ilm_mesh* m, const ilm_ray* rays, const unsigned long long* n_dev, int64_t capacity, ilm_hit* hits,
-                uint8_t* occ, hipStream_t stream) {
-    if (occ) return launch_trace<true, false>(m, rays, capacity, nullptr, occ, nullptr, stream, n_dev);
-    return launch_trace<false, false>(m, rays, capacity, hits, nullptr, nullptr, stream, n_dev);
+                uint8_t* occ, hipStream_t stream, bool rooted) {
+    if (occ) return launch_trace<true, false>(m, rays, capacity, nullptr, occ, nullptr, stream, n_dev, rooted);
+    return launch_trace<false, false>(m, rays, capacity, hits, nullptr, nullptr, stream, n_dev, rooted);
 }
 }  // namespace ilm
 

@@ -395,6 +395,54 @@ int ilm_image_tonemap(ilm_ctx* ctx, const float* in, int width, int height, int 
 /* Writes host RGB floats (rows in the given order) to path. */
 int ilm_image_write(const char* path, const float* rgb, int width, int height, int format);
 
+/* ---------------------------------------------------------------- batches of scenes */
+
+/* Several scenes rendered by one frame's launch sequence: rays of every
+ * scene share the traversal launches (each starts at its own scene's BVH root
+ * in one packed node array) and paths of every view share the shading
+ * launches. Replaces running one RenderThread pool per scene
+ * (Sandbox/Multithreaded.h:336-534); no reference counterpart otherwise.
+ * The scenes' meshes must be built with ILM_NODES_FLOAT (the batch also keeps
+ * a quantised copy of the packed nodes). The scenes stay owned by the caller
+ * and may be destroyed after the batch is created. */
+typedef struct ilm_batch ilm_batch;
+int ilm_batch_create(ilm_ctx* ctx, ilm_scene* const* scenes, int nscenes, ilm_batch** out);
+int ilm_batch_destroy(ilm_batch* batch);
+
+typedef struct {
+    int32_t scene;        /* index into the scenes given to ilm_batch_create */
+    ilm_camera cam;
+    int32_t width, height, spp;
+} ilm_batch_view;
+
+/* Renders every view and ADDS their radiance sums into accum: the views'
+ * images one after another (sum of width*height*3 doubles). params gives
+ * max_depth, rr_depth, eps, seed, spp_offset and batch_paths (0 = the tuned or
+ * default value); its camera, width, height and spp are not used. */
+int ilm_render_batch(ilm_batch* batch, const ilm_batch_view* views, int nviews, const ilm_render_params* params,
+                     double* accum, ilm_render_stats* stats);
+/* As ilm_scene_set_graph, for the batch's frames. */
+int ilm_batch_set_graph(ilm_batch* batch, int enable);
+/* Fixes the node format (ILM_NODES_*) and paths per wavefront batch
+ * (0 = default) of later frames. */
+int ilm_batch_configure(ilm_batch* batch, int node_format, int64_t batch_paths);
+
+typedef struct {
+    int32_t node_format;     /* chosen ILM_NODES_* */
+    int64_t batch_paths;     /* chosen paths per wavefront batch */
+    double best_ms;          /* its timing frame */
+    int32_t candidates;
+    int32_t candidate_format[8];
+    int64_t candidate_batch[8];
+    double candidate_ms[8];
+} ilm_batch_tuning;
+
+/* Times frames of the given views (at most 2 spp) for each node format and
+ * batch size (2^20, 2^22, 2^24 paths), keeps the fastest for later frames and
+ * reports the table (out may be NULL). */
+int ilm_batch_autotune(ilm_batch* batch, const ilm_batch_view* views, int nviews, const ilm_render_params* params,
+                       ilm_batch_tuning* out);
+
 /* ---------------------------------------------------------------- neural radiance cache */
 
 /* A small MLP (96 -> 5 x 64 ReLU -> 16, bf16 weights and activations, FP32

@@ -120,3 +120,62 @@ def test_training_fits_a_smooth_field(gpu_ctx, ilm):
     assert losses[-1] < losses[0] / 10
     assert err < 0.1
     net.close()
+
+
+def test_trained_cache_beats_plain_pt_at_equal_time(gpu_ctx, ilm):
+    """What the learned part gains: on the radiance-cache workload at
+    960 x 540, cache rendering (trained online from a cold start) reaches a
+    lower relRMSE than plain path tracing given the same GPU time, and the
+    same renderer with training switched off is far worse (the untrained
+    network's bias). Errors are against a 256-spp plain render whose own
+    noise, estimated from two independent halves, is removed."""
+    from illuminaprt_amd import workloads as w
+    W, H, depth = 960, 540, 10
+    pos, nrm, idx, tm, tl, mats, lights = w.cache_scene()
+    m = ilm.Mesh(gpu_ctx, pos, idx)
+    m.build()
+    sc = ilm.Scene(gpu_ctx, m, nrm, tm, tl, mats, lights)
+    cam = w.cache_camera(W / H)
+    n3 = W * H * 3
+    buf = gpu_ctx.buffer(n3 * 8)
+
+    def mean_of(spp, seed, chunk=32):
+        buf.zero()
+        ms = 0.0
+        for k in range(0, spp, chunk):
+            ms += sc.render(cam, W, H, min(chunk, spp - k), buf, max_depth=depth, seed=seed, spp_offset=k)["ms"]
+        return buf.download(np.float64, n3).reshape(H, W, 3) / spp, ms
+
+    r1, _ = mean_of(128, 501)
+    r2, _ = mean_of(128, 502)
+    ref = 0.5 * (r1 + r2)
+    noise2 = float(np.mean((r1 - r2) ** 2 / (ref ** 2 + 0.01)) / 4.0)
+
+    def err(img):
+        return float(np.sqrt(max(float(np.mean((img - ref) ** 2 / (ref ** 2 + 0.01))) - noise2, 0.0)))
+
+    def cache_run(train_steps, frames=56, skip=16):
+        net = ilm.Nrc(gpu_ctx, *w.CACHE_BOUNDS, lr=1e-2, seed=5)
+        buf.zero()
+        ms = 0.0
+        for f in range(frames):
+            st = sc.render_nrc(net, cam, W, H, 1, buf if f >= skip else None, seed=7000 + f, prefix_depth=2,
+                               suffix_depth=depth - 2, train_fraction=1 / 16, train_batch=32768,
+                               train_steps=train_steps)
+            ms += st["ms"]
+        net.close()
+        return err(buf.download(np.float64, n3).reshape(H, W, 3) / (frames - skip)), ms
+
+    e_nrc, t_nrc = cache_run(1)
+    e_off, _ = cache_run(0)
+    # plain PT given the cache's time
+    spp_per_ms = 1.0 / (mean_of(4, 900)[1] / 4)
+    spp = max(1, int(t_nrc * spp_per_ms))
+    e_plain = err(mean_of(spp, 901)[0])
+    print(f"[cache gain] ref noise {np.sqrt(noise2):.4f}; cache {e_nrc:.4f} in {t_nrc:.0f} ms; "
+          f"plain {e_plain:.4f} at {spp} spp (same time); cache untrained {e_off:.4f}", flush=True)
+    assert e_nrc < 0.85 * e_plain
+    assert e_off > 2.0 * e_nrc
+    buf.free()
+    sc.close()
+    m.close()
