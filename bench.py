@@ -175,6 +175,57 @@ def large_scene_leg(ctx, ilm, workloads, args):
     return out
 
 
+def batch_leg(ctx, ilm, workloads, args):
+    """64 box rooms, triangle counts log-uniform in [1k, 2M], square cameras of
+    128..1024 pixels, depth 8, rendered by one frame's launch sequence (a
+    batch): device BVH builds, autotuned node format and batch size, then
+    timed frames at 2 spp (the 32-spp frame is 16 of them)."""
+    rng = np.random.default_rng(2024)
+    t0 = time.perf_counter()
+    scenes, meshes, views, build_ms, tris, pix = [], [], [], 0.0, 0, 0
+    for k in range(args.batch_scenes):
+        (pos, nrm, idx, tm, tl, mats, lights), cam, side, _ = workloads.batch_scene(rng)
+        m = ilm.Mesh(ctx, pos, idx)
+        build_ms += m.build(builder=ilm.ILM_BUILDER_LBVH)["build_ms"]
+        scenes.append(ilm.Scene(ctx, m, nrm, tm, tl, mats, lights))
+        meshes.append(m)
+        views.append((k, cam, side, side, args.batch_spp))
+        tris += len(idx)
+        pix += side * side
+    b = ilm.Batch(ctx, scenes)
+    for sc in scenes:
+        sc.close()
+    for m in meshes:
+        m.close()
+    setup_s = time.perf_counter() - t0
+    tune = b.autotune(views, max_depth=8)
+    log(f"batch: {len(views)} scenes, {tris} tris, {pix} pixels, setup {setup_s:.1f} s, autotune {tune}")
+    acc = ctx.buffer(pix * 24).zero()
+    frames = []
+    for graph in (False, True):
+        b.set_graph(graph)
+        for i in range(2):   # the first graph frame captures
+            b.render(views, acc, max_depth=8, seed=50 + i)
+        for i in range(max(2, args.steps // 8)):
+            frames.append((graph, b.render(views, acc, max_depth=8, seed=100 + i)))
+    b.set_graph(False)
+    acc.free()
+    b.close()
+    out = {"workload": f"{len(views)} box rooms, {tris} tris (1k-2M log-uniform per room), {pix} pixels, "
+                       f"{args.batch_spp} spp per frame, depth 8",
+           "triangles": tris, "pixels": pix, "device_build_ms": round(build_ms, 1),
+           "autotune": {"node_format": tune["node_format"], "batch_paths": tune["batch_paths"],
+                        "table_ms": [[f, bp, round(ms, 2)] for f, bp, ms in tune["table"]]}}
+    for graph in (False, True):
+        fr = [f for g, f in frames if g == graph]
+        ms = float(np.mean([f["ms"] for f in fr]))
+        rays = fr[0]["camera_rays"] + fr[0]["extension_rays"] + fr[0]["shadow_rays"]
+        out["graph" if graph else "eager"] = {"frame_ms": round(ms, 2), "rays_per_frame": int(rays),
+                                               "mrays_s": round(rays / ms / 1e3, 1)}
+    log(f"batch: {out}")
+    return out
+
+
 MLP_FLOP_PER_QUERY = 2 * (96 * 64 + 4 * 64 * 64 + 64 * 16)  # padded network as executed
 MFMA_BF16_PEAK_TFLOPS = 2500.0
 
@@ -327,6 +378,10 @@ def main():
                     help="only run the plain copy kernel over MIB mebibytes (bandwidth / counter calibration)")
     ap.add_argument("--no-cache", action="store_true", help="skip the time-to-relRMSE leg")
     ap.add_argument("--no-large", action="store_true", help="skip the 11M-triangle leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batched leg")
+    ap.add_argument("--batch-only", action="store_true", help="run only the 64-scene batched leg")
+    ap.add_argument("--batch-scenes", type=int, default=64)
+    ap.add_argument("--batch-spp", type=int, default=2)
     ap.add_argument("--nodes", choices=["float", "quantized"], default="float",
                     help="BVH4 node format of the traced trees")
     ap.add_argument("--cache-only", action="store_true", help="run only the time-to-relRMSE leg")
@@ -358,6 +413,10 @@ def main():
         _, per = time_launches(ctx, lambda: ctx.copy(dst, src, nbytes), args.steps)
         ms = float(np.mean(per))
         print(json.dumps({"copy_bytes": nbytes, "kernel_ms": ms, "gbs_read_plus_write": 2 * nbytes / ms / 1e6}))
+        return
+    if args.batch_only:
+        print(json.dumps({"batched_scenes": batch_leg(ctx, ilm, workloads, args)}), flush=True)
+        ctx.close()
         return
     if args.cache_only:
         print(json.dumps({"time_to_rel_rmse": cache_convergence(ctx, ilm, workloads, args)}), flush=True)
@@ -420,6 +479,7 @@ def main():
     room = None if args.only else path_tracing_room(ctx, ilm, workloads, args)
     cache = None if (args.only or args.no_cache) else cache_convergence(ctx, ilm, workloads, args)
     large = None if (args.only or args.no_large) else large_scene_leg(ctx, ilm, workloads, args)
+    batch = None if (args.only or args.no_batch) else batch_leg(ctx, ilm, workloads, args)
     out = {
         "metric": METRIC,
         "value": round(inc["wall_mrays_s"], 3),
@@ -460,6 +520,7 @@ def main():
         "time_to_rel_rmse": cache,
         "bvh_build": build,
         "large_scene": large,
+        "batched_scenes": batch,
     }
     print(json.dumps(out), flush=True)
     for b in (hits, occ, visits):

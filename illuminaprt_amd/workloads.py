@@ -247,3 +247,52 @@ CACHE_BOUNDS = ((-3.0, -1.2, -3.0), (3.0, 3.0, 3.0))
 
 def cache_camera(aspect=16 / 9):
     return Camera.make((0.0, 0.6, -2.8), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 60.0, aspect)
+
+
+# ---------------------------------------------------------------- batched scenes
+# Batched path-tracing workload: n box rooms (the plain path-tracing room),
+# each holding a displaced icosphere (levels 3-8) and small boxes so that its
+# triangle count is log-uniform in [1k, 2M]; object materials are 70%
+# Lambertian and 30% Lambertian + Phong with exponent log-uniform in
+# [10, 1000]; each room is seen by its own camera at a square resolution
+# uniform in [128, 1024]; 32 spp, depth 8.
+def batch_scene(rng):
+    """One room of the batched workload: (arrays for Scene, camera, side, target triangles)."""
+    from . import vertex_normals
+    target = int(np.exp(rng.uniform(np.log(1e3), np.log(2e6))))
+    pos, nrm, idx, tm, tl, mats, lights = box_room()
+    nbase = len(ROOM_MATS)
+    nobj = 8
+    omats = []
+    for _ in range(nobj):
+        kd = rng.uniform(0.2, 0.8, 3)
+        if rng.random() < 0.3:
+            ks = rng.uniform(0.1, 0.4, 3)
+            omats.append([*(kd * 0.6), *ks, float(np.exp(rng.uniform(np.log(10), np.log(1000)))), 1])
+        else:
+            omats.append([*kd, 0, 0, 0, 1, 0])
+    mats = np.concatenate([mats, np.array(omats, np.float32)])
+    room_tris = len(idx)
+    level = int(np.clip(np.floor(np.log(max(target - room_tris, 20) / 20.0) / np.log(4.0)), 3, 8))
+    sp, si = icosphere(level, 0.1, 4, int(rng.integers(1 << 30)))
+    sp = (sp * 0.16 + np.float32([0.5, 0.25, 0.55])).astype(np.float32)
+    sn = vertex_normals(sp, si)
+    parts_p, parts_n, parts_i = [pos, sp], [nrm, sn], [idx, si + len(pos)]
+    parts_m = [tm, np.full(len(si), nbase + int(rng.integers(nobj)), np.int32)]
+    parts_l = [tl, np.full(len(si), -1, np.int32)]
+    nb = max(0, (target - room_tris - len(si)) // 12)
+    if nb:
+        bp, bi = boxes(nb, 0.004, 0.03, 0.3, int(rng.integers(1 << 30)))
+        bp = (bp + np.float32([0.5, 0.45, 0.5])).astype(np.float32)
+        bn = vertex_normals(bp, bi)
+        off = len(pos) + len(sp)
+        parts_p.append(bp)
+        parts_n.append(bn)
+        parts_i.append(bi + off)
+        parts_m.append((nbase + rng.integers(nobj, size=nb).repeat(12)).astype(np.int32))
+        parts_l.append(np.full(len(bi), -1, np.int32))
+    arrays = (np.concatenate(parts_p).astype(np.float32), np.concatenate(parts_n).astype(np.float32),
+              np.concatenate(parts_i).astype(np.int32), np.concatenate(parts_m).astype(np.int32),
+              np.concatenate(parts_l).astype(np.int32), mats, lights)
+    side = int(rng.integers(128, 1025))
+    return arrays, room_camera(), side, target
