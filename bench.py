@@ -175,6 +175,69 @@ def large_scene_leg(ctx, ilm, workloads, args):
     return out
 
 
+def large_render_leg(ctx, ilm, workloads, args):
+    """11,085,808 triangles (two displaced level-9 icospheres, 50,000 boxes and
+    a 48-triangle room with three panel lights; 30% glossy), eight 3840 x 2160
+    cameras, depth 12, rendered as one batch (one scene, eight views): plain
+    frames and neural-cache frames (2-bounce prefix, 10-bounce training
+    suffix), 1 spp per view per frame (a 64-spp frame is 64 of them). The
+    plain frames' per-view means judge the cache frames' (after `skip`
+    training frames). HBM footprint = device memory in use after the frames
+    minus before the scene was created."""
+    free0, total = ctx.memory()
+    t0 = time.perf_counter()
+    pos, nrm, idx, tm, tl, mats, lights = workloads.large_render_scene()
+    gen_s = time.perf_counter() - t0
+    m = ilm.Mesh(ctx, pos, idx)
+    info = m.build(builder=ilm.ILM_BUILDER_LBVH)
+    sc = ilm.Scene(ctx, m, nrm, tm, tl, mats, lights)
+    b = ilm.Batch(ctx, [sc])
+    sc.close()
+    m.close()
+    del pos, nrm, idx, tm, tl
+    W, H, nv, depth, prefix = 3840, 2160, 8, 12, 2
+    views = [(0, c, W, H, 1) for c in workloads.large_cameras(nv, W / H)]
+    npix = nv * W * H
+    acc = ctx.buffer(npix * 24).zero()
+
+    def view_means():
+        img = acc.download(np.float64, npix * 3).reshape(nv, -1)
+        return img.mean(axis=1)
+
+    b.render(views, acc, max_depth=depth, seed=300)   # allocates the wavefront state
+    acc.zero()
+    plain = [b.render(views, acc, max_depth=depth, seed=301 + i) for i in range(args.large_frames)]
+    plain_means = view_means() / args.large_frames
+    net = ilm.Nrc(ctx, *workloads.LARGE_BOUNDS, lr=args.cache_lr, seed=5)
+    acc.zero()
+    skip, cache = args.large_skip, []
+    for i in range(skip + args.large_frames):
+        cache.append(b.render_nrc(net, views, acc if i >= skip else None, seed=400 + i, prefix_depth=prefix,
+                                  suffix_depth=depth - prefix, train_fraction=1 / 16, train_batch=65536,
+                                  train_steps=1))
+    cache_means = view_means() / args.large_frames
+    free1, _ = ctx.memory()
+    net.close()
+    acc.free()
+    b.close()
+
+    def summary(frames):
+        ms = float(np.mean([f["ms"] for f in frames]))
+        rays = frames[0]["camera_rays"] + frames[0]["extension_rays"] + frames[0]["shadow_rays"]
+        return {"frame_ms": round(ms, 1), "rays_per_frame": int(rays), "mrays_s": round(rays / ms / 1e3, 1),
+                "ms_64spp": round(64 * ms, 0)}
+
+    out = {"workload": f"{info['triangles'] if 'triangles' in info else 11085808} tris, {nv} cameras "
+                       f"{W}x{H}, depth {depth}, 1 spp per view per frame, one batch",
+           "generate_s": round(gen_s, 1), "lbvh_build_ms": round(info["build_ms"], 1),
+           "hbm_footprint_gib": round((free0 - free1) / 2**30, 2), "hbm_total_gib": round(total / 2**30, 1),
+           "plain": summary(plain), "neural_cache": summary(cache[skip:]),
+           "cache_training_frames": skip,
+           "cache_over_plain_view_means": [round(float(c / p), 4) for c, p in zip(cache_means, plain_means)]}
+    log(f"large render: {out}")
+    return out
+
+
 def batch_leg(ctx, ilm, workloads, args):
     """64 box rooms, triangle counts log-uniform in [1k, 2M], square cameras of
     128..1024 pixels, depth 8, rendered by one frame's launch sequence (a
@@ -380,6 +443,10 @@ def main():
     ap.add_argument("--no-large", action="store_true", help="skip the 11M-triangle leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batched leg")
     ap.add_argument("--batch-only", action="store_true", help="run only the 64-scene batched leg")
+    ap.add_argument("--no-large-render", action="store_true", help="skip the 8-camera 4K batched render leg")
+    ap.add_argument("--large-render-only", action="store_true", help="run only the 8-camera 4K render leg")
+    ap.add_argument("--large-frames", type=int, default=2, help="timed frames per renderer (large render leg)")
+    ap.add_argument("--large-skip", type=int, default=6, help="cache training frames before timing (large render)")
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--batch-spp", type=int, default=2)
     ap.add_argument("--nodes", choices=["float", "quantized"], default="float",
@@ -416,6 +483,10 @@ def main():
         return
     if args.batch_only:
         print(json.dumps({"batched_scenes": batch_leg(ctx, ilm, workloads, args)}), flush=True)
+        ctx.close()
+        return
+    if args.large_render_only:
+        print(json.dumps({"large_render": large_render_leg(ctx, ilm, workloads, args)}), flush=True)
         ctx.close()
         return
     if args.cache_only:
@@ -480,6 +551,7 @@ def main():
     cache = None if (args.only or args.no_cache) else cache_convergence(ctx, ilm, workloads, args)
     large = None if (args.only or args.no_large) else large_scene_leg(ctx, ilm, workloads, args)
     batch = None if (args.only or args.no_batch) else batch_leg(ctx, ilm, workloads, args)
+    lrender = None if (args.only or args.no_large_render) else large_render_leg(ctx, ilm, workloads, args)
     out = {
         "metric": METRIC,
         "value": round(inc["wall_mrays_s"], 3),

@@ -112,6 +112,7 @@ SIGNATURES = {
     "ilm_last_error": (C.c_char_p, []),
     "ilm_device_count": (C.c_int, [C.POINTER(C.c_int)]),
     "ilm_device_info": (C.c_int, [_vp, C.c_char_p, C.c_int, C.POINTER(C.c_int)]),
+    "ilm_device_memory": (C.c_int, [_vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "ilm_alloc": (C.c_int, [_vp, _sz, C.POINTER(_vp)]),
     "ilm_free": (C.c_int, [_vp, _vp]),
     "ilm_upload": (C.c_int, [_vp, _vp, _vp, _sz]),
@@ -176,6 +177,8 @@ SIGNATURES = {
                                      C.POINTER(BatchTuning)]),
     "ilm_render_nrc": (C.c_int, [_vp, _vp, C.POINTER(RenderParams), C.POINTER(NrcRenderParams), _vp,
                                  C.POINTER(RenderStats)]),
+    "ilm_render_batch_nrc": (C.c_int, [_vp, _vp, C.POINTER(BatchView), C.c_int, C.POINTER(RenderParams),
+                                       C.POINTER(NrcRenderParams), _vp, C.POINTER(RenderStats)]),
 }
 
 
@@ -271,6 +274,12 @@ class Context:
 
     def sync(self):
         check(load().ilm_sync(self.h))
+
+    def memory(self) -> tuple[int, int]:
+        """(free, total) bytes of device memory."""
+        fr, tot = C.c_uint64(), C.c_uint64()
+        check(load().ilm_device_memory(self.h, C.byref(fr), C.byref(tot)))
+        return fr.value, tot.value
 
     def copy(self, dst: DeviceBuffer, src: DeviceBuffer, nbytes: int):
         check(load().ilm_device_copy(self.h, dst.ptr, src.ptr, nbytes, None))
@@ -384,6 +393,20 @@ class Batch:
         p = RenderParams(Camera(), 0, 0, 0, spp_offset, max_depth, rr_depth, eps, seed, batch_paths)
         st = RenderStats()
         check(load().ilm_render_batch(self.h, arr, len(views), C.byref(p), accum.ptr, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in st._fields_}
+
+    def render_nrc(self, nrc: "Nrc", views, accum: DeviceBuffer | None, seed=1234, spp_offset=0, rr_depth=4,
+                   eps=1e-4, prefix_depth=2, suffix_depth=8, train_fraction=1 / 16, train_batch=65536,
+                   train_steps=1, batch_paths=0) -> dict:
+        """The cache renderer over views of the batch (one network for all)."""
+        arr = self._views(views)
+        p = RenderParams(Camera(), 0, 0, 0, spp_offset, prefix_depth + suffix_depth, rr_depth, eps, seed,
+                         batch_paths)
+        q = NrcRenderParams(prefix_depth, suffix_depth, train_fraction, train_batch, train_steps,
+                            1 if accum is not None else 0)
+        st = RenderStats()
+        check(load().ilm_render_batch_nrc(self.h, nrc.h, arr, len(views), C.byref(p), C.byref(q),
+                                          accum.ptr if accum else None, C.byref(st)))
         return {k: getattr(st, k) for k, _ in st._fields_}
 
     def autotune(self, views, max_depth=5, rr_depth=4, eps=1e-4, seed=1234) -> dict:

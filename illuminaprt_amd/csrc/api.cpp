@@ -109,6 +109,16 @@ int ilm_device_info(ilm_ctx* ctx, char* name, int name_len, int* cu_count) {
     return ILM_OK;
 }
 
+int ilm_device_memory(ilm_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes) {
+    if (!ctx) return fail(ILM_ERR_INVALID, "null ctx");
+    ILM_HIP(hipSetDevice(ctx->device));
+    size_t fr = 0, tot = 0;
+    ILM_HIP(hipMemGetInfo(&fr, &tot));
+    if (free_bytes) *free_bytes = fr;
+    if (total_bytes) *total_bytes = tot;
+    return ILM_OK;
+}
+
 int ilm_alloc(ilm_ctx* ctx, size_t bytes, void** dev) {
     if (!ctx || !dev) return fail(ILM_ERR_INVALID, "null argument");
     *dev = nullptr;

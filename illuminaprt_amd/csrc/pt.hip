@@ -1252,6 +1252,18 @@ int ilm_render_batch(ilm_batch* b, const ilm_batch_view* bv, int nviews, const i
     return render_views(b->merged, nullptr, views, p, nullptr, accum, stats);
 }
 
+int ilm_render_batch_nrc(ilm_batch* b, ilm_nrc* nrc, const ilm_batch_view* bv, int nviews,
+                         const ilm_render_params* prm, const ilm_nrc_render_params* np, double* accum,
+                         ilm_render_stats* stats) {
+    if (!b || !nrc || !bv || nviews <= 0 || !prm || !np) return fail(ILM_ERR_INVALID, "bad arguments");
+    std::vector<ilm::View> views;
+    int rc = batch_views(b, bv, nviews, views);
+    if (rc) return rc;
+    ilm_render_params p = *prm;
+    if (p.batch_paths <= 0 && b->batch_paths > 0) p.batch_paths = (int32_t)std::min<int64_t>(b->batch_paths, INT32_MAX);
+    return render_views(b->merged, nrc, views, p, np, accum, stats);
+}
+
 int ilm_batch_autotune(ilm_batch* b, const ilm_batch_view* bv, int nviews, const ilm_render_params* prm,
                        ilm_batch_tuning* out) {
     if (!b || !bv || nviews <= 0 || !prm) return fail(ILM_ERR_INVALID, "bad arguments");

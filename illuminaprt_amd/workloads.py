@@ -296,3 +296,36 @@ def batch_scene(rng):
               np.concatenate(parts_l).astype(np.int32), mats, lights)
     side = int(rng.integers(128, 1025))
     return arrays, room_camera(), side, target
+
+
+# ---------------------------------------------------------------- large rendered scene
+# The 11M-triangle scene made renderable: the two displaced level-9 spheres and
+# 50,000 boxes of large_scene() inside the radiance-cache room (scaled by 1.6)
+# with its three panel lights; 30% of the sphere triangles and of the boxes are
+# glossy. Eight cameras on a circle around the spheres look at the origin.
+LARGE_BOUNDS = ((-4.8, -1.92, -4.8), (4.8, 4.8, 4.8))
+
+
+def large_render_scene(glossy_fraction=0.3, seed=31):
+    from . import vertex_normals
+    rng = np.random.default_rng(seed)
+    pos, idx = large_scene()
+    nt_obj = len(idx)
+    nrm = vertex_normals(pos, idx)
+    tm = np.where(rng.random(nt_obj) < glossy_fraction, 4, 3).astype(np.int32)
+    rp, rn, ri, rm, rl, mats, lights = cache_scene(level=3)
+    nsph = 20 * 4 ** 3
+    rp, rn, ri, rm, rl = rp[642:] * 1.6, rn[642:], ri[nsph:] - 642, rm[nsph:], rl[nsph:]   # the room and panels only
+    off = len(pos)
+    return (np.concatenate([pos, rp]).astype(np.float32), np.concatenate([nrm, rn]).astype(np.float32),
+            np.concatenate([idx, ri + off]).astype(np.int32), np.concatenate([tm, rm]).astype(np.int32),
+            np.concatenate([np.full(nt_obj, -1, np.int32), rl]).astype(np.int32), mats, lights)
+
+
+def large_cameras(n=8, aspect=16 / 9):
+    cams = []
+    for k in range(n):
+        a = 2 * np.pi * (k + 0.5) / n
+        cams.append(Camera.make((4.0 * np.sin(a), 0.8, -4.0 * np.cos(a)), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0),
+                                55.0, aspect))
+    return cams

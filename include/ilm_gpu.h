@@ -76,6 +76,10 @@ int ilm_device_count(int* count);
 /* Device name and compute-unit count. */
 int ilm_device_info(ilm_ctx* ctx, char* name, int name_len, int* cu_count);
 
+/* Free and total device memory of the context's device (hipMemGetInfo);
+ * used to report a workload's HBM footprint. */
+int ilm_device_memory(ilm_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes);
+
 /* Device buffers owned by the context (freed by ilm_ctx_destroy). */
 int ilm_alloc(ilm_ctx* ctx, size_t bytes, void** dev);
 int ilm_free(ilm_ctx* ctx, void* dev);
@@ -507,6 +511,12 @@ typedef struct {
  * Synchronous; accum may be NULL when accumulate is 0. */
 int ilm_render_nrc(ilm_scene* scene, ilm_nrc* nrc, const ilm_render_params* params,
                    const ilm_nrc_render_params* nrc_params, double* accum, ilm_render_stats* stats);
+/* The cache renderer over a batch's views (as ilm_render_nrc). All views
+ * share the one network, so this is meant for several views of one scene
+ * (views of scenes that overlap in space would share cache entries). */
+int ilm_render_batch_nrc(ilm_batch* batch, ilm_nrc* nrc, const ilm_batch_view* views, int nviews,
+                         const ilm_render_params* params, const ilm_nrc_render_params* nrc_params, double* accum,
+                         ilm_render_stats* stats);
 
 #ifdef __cplusplus
 }
