@@ -214,7 +214,7 @@ def large_render_leg(ctx, ilm, workloads, args):
     for i in range(skip + args.large_frames):
         cache.append(b.render_nrc(net, views, acc if i >= skip else None, seed=400 + i, prefix_depth=prefix,
                                   suffix_depth=depth - prefix, train_fraction=1 / 16, train_batch=65536,
-                                  train_steps=1))
+                                  train_steps=args.large_train_steps))
     cache_means = view_means() / args.large_frames
     free1, _ = ctx.memory()
     net.close()
@@ -232,7 +232,7 @@ def large_render_leg(ctx, ilm, workloads, args):
            "generate_s": round(gen_s, 1), "lbvh_build_ms": round(info["build_ms"], 1),
            "hbm_footprint_gib": round((free0 - free1) / 2**30, 2), "hbm_total_gib": round(total / 2**30, 1),
            "plain": summary(plain), "neural_cache": summary(cache[skip:]),
-           "cache_training_frames": skip,
+           "cache_training_frames": skip, "cache_train_steps_per_frame": args.large_train_steps,
            "cache_over_plain_view_means": [round(float(c / p), 4) for c, p in zip(cache_means, plain_means)]}
     log(f"large render: {out}")
     return out
@@ -446,7 +446,8 @@ def main():
     ap.add_argument("--no-large-render", action="store_true", help="skip the 8-camera 4K batched render leg")
     ap.add_argument("--large-render-only", action="store_true", help="run only the 8-camera 4K render leg")
     ap.add_argument("--large-frames", type=int, default=2, help="timed frames per renderer (large render leg)")
-    ap.add_argument("--large-skip", type=int, default=6, help="cache training frames before timing (large render)")
+    ap.add_argument("--large-skip", type=int, default=8, help="cache training frames before timing (large render)")
+    ap.add_argument("--large-train-steps", type=int, default=8, help="Adam steps per cache frame (large render)")
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--batch-spp", type=int, default=2)
     ap.add_argument("--nodes", choices=["float", "quantized"], default="float",
@@ -505,6 +506,10 @@ def main():
     hits = ctx.buffer(n * HIT_BYTES)
     occ = ctx.buffer(n)
     visits = ctx.buffer(n * 8)
+    # idle-lane refill threshold per set: the library default (32) for the
+    # incoherent and shadow sets, 64 (refill when the whole wave is idle) for
+    # coherent camera rays, as the path integrator traces them
+    refill = {"incoherent": 32, "primary": 64, "shadow": 32}
     sets = {
         "incoherent": (lambda: mesh.trace_closest(rs.incoherent, n, hits), rs.incoherent, False),
         "primary": (lambda: mesh.trace_closest(rs.primary, n, hits), rs.primary, False),
@@ -514,6 +519,7 @@ def main():
         sets = {args.only: sets[args.only]}
     res = {}
     for name, (launch, buf, anyhit) in sets.items():
+        ctx.set_trace_refill(refill[name])
         mesh.trace_stats(buf, n, anyhit, visits)
         ctx.sync()
         v = visits.download(np.int32, 2 * n).reshape(n, 2)
@@ -532,8 +538,10 @@ def main():
             "tris_per_ray": nt,
             "bytes_per_ray": RAY_BYTES + (1 if anyhit else HIT_BYTES) + nn * info["node_stride"] + nt * TRI_BYTES,
             "steps": steps,
+            "refill_idle_lanes": refill[name],
         }
         log(f"{name}: {res[name]}")
+    ctx.set_trace_refill(32)
     ctx.sync()
     if "incoherent" not in res:
         print(json.dumps(res), flush=True)

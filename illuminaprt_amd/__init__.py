@@ -113,6 +113,7 @@ SIGNATURES = {
     "ilm_device_count": (C.c_int, [C.POINTER(C.c_int)]),
     "ilm_device_info": (C.c_int, [_vp, C.c_char_p, C.c_int, C.POINTER(C.c_int)]),
     "ilm_device_memory": (C.c_int, [_vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "ilm_ctx_set_trace_refill": (C.c_int, [_vp, C.c_int]),
     "ilm_alloc": (C.c_int, [_vp, _sz, C.POINTER(_vp)]),
     "ilm_free": (C.c_int, [_vp, _vp]),
     "ilm_upload": (C.c_int, [_vp, _vp, _vp, _sz]),
@@ -151,6 +152,7 @@ SIGNATURES = {
     "ilm_obj_load": (C.c_int, [C.c_char_p, C.POINTER(_vp)]),
     "ilm_obj_counts": (C.c_int, [_vp, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "ilm_obj_arrays": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "ilm_obj_material_info": (C.c_int, [_vp, _vp, C.c_char_p, C.c_int]),
     "ilm_obj_destroy": (C.c_int, [_vp]),
     "ilm_image_resolve": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_double, C.c_int, _vp, _vp]),
     "ilm_image_tonemap": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _vp]),
@@ -274,6 +276,9 @@ class Context:
 
     def sync(self):
         check(load().ilm_sync(self.h))
+
+    def set_trace_refill(self, idle_lanes: int):
+        check(load().ilm_ctx_set_trace_refill(self.h, idle_lanes))
 
     def memory(self) -> tuple[int, int]:
         """(free, total) bytes of device memory."""

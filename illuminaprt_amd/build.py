@@ -79,7 +79,8 @@ def build_plugin(verbose: bool = False) -> None:
     cxx = shutil.which("g++") or "g++"
     flags = ["-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include")]
     tmp = PLUGIN + ".tmp"
-    _run([cxx, *flags, "-fPIC", "-shared", os.path.join(PLUGIN_SRC, "ilm_plugin.cpp"), "-o", tmp,
+    _run([cxx, *flags, "-fPIC", "-shared", os.path.join(PLUGIN_SRC, "ilm_plugin.cpp"),
+          os.path.join(PLUGIN_SRC, "ilm_environment.cpp"), "-o", tmp,
           "-L", HERE, "-lilm_gpu", "-Wl,-rpath,$ORIGIN", "-lpthread"], "plug-in build")
     os.replace(tmp, PLUGIN)
     tmp = SANDBOX + ".tmp"

@@ -38,6 +38,7 @@ struct ilm_ctx {
     unsigned long long* counters = nullptr;  // [0] next ray, [1] finished waves, [2] error flags
     uint32_t* spill = nullptr;
     size_t spill_words = 0;
+    int trace_refill = 32;  // idle lanes of a wave that trigger its ray refill
     std::mutex mu;
 };
 
@@ -69,7 +70,7 @@ void camera_frame(const ilm_camera& c, float out[12]);
 // rooted: each ray's tmin field holds (as bits) the BVH node to start from,
 // so one launch can trace rays of several BVHs packed in one node array.
 int trace_queue(ilm_mesh* m, const ilm_ray* rays, const unsigned long long* n_dev, int64_t capacity, ilm_hit* hits,
-                uint8_t* occ, hipStream_t stream, bool rooted = false);
+                uint8_t* occ, hipStream_t stream, bool rooted = false, int refill = 0);
 // Builds m's BVH4 on its device (linear BVH + collapse); fills the device
 // arrays, info and host copies as ilm_bvh_build does.
 int build_lbvh(ilm_mesh* m, int max_leaf, int node_format);

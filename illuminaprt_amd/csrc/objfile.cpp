@@ -32,6 +32,8 @@ struct ilm_obj {
     std::vector<float> pos, nrm;
     std::vector<int32_t> idx, tri_mat, tri_light;
     std::vector<float> mats, lights;
+    std::vector<int32_t> illum;  // per material, 2 when absent
+    std::string mtllib;          // file name of the first material library
 };
 
 namespace {
@@ -40,6 +42,7 @@ struct Mtl {
     std::string name;
     float kd[3] = {0, 0, 0}, ks[3] = {0, 0, 0}, ke[3] = {0, 0, 0};
     float ns = 0;
+    int illum = 2;
 };
 
 std::vector<std::string> split(const std::string& s, const char* seps) {
@@ -92,6 +95,9 @@ bool load_mtl(const std::string& path, std::vector<Mtl>& out) {
             for (int k = 0; k < 3; ++k) to_float(t[1 + k], dst[k]);
         } else if (t[0] == "Ns" && t.size() == 2) {
             to_float(t[1], m.ns);
+        } else if (t[0] == "illum" && t.size() == 2) {
+            long v = 2;
+            if (to_int(t[1], v)) m.illum = (int)v;
         }
     }
     return true;
@@ -171,6 +177,7 @@ int ilm_obj_load(const char* path, ilm_obj** out) {
                 o->tri_mat.push_back(cur_mat);
             }
         } else if (t[0] == "mtllib" && t.size() == 2) {
+            if (o->mtllib.empty()) o->mtllib = t[1];
             size_t before = mtls.size();
             load_mtl(dir_of(path) + t[1], mtls);
             for (size_t k = before; k < mtls.size(); ++k)
@@ -208,6 +215,7 @@ int ilm_obj_load(const char* path, ilm_obj** out) {
         bool spec = m.ks[0] > 0 || m.ks[1] > 0 || m.ks[2] > 0;
         const float rec[8] = {m.kd[0], m.kd[1], m.kd[2], m.ks[0], m.ks[1], m.ks[2], m.ns, spec ? 1.0f : 0.0f};
         o->mats.insert(o->mats.end(), rec, rec + 8);
+        o->illum.push_back(m.illum);
         if (m.ke[0] > 0 || m.ke[1] > 0 || m.ke[2] > 0) {
             light_of[k] = (int32_t)(o->lights.size() / 4);
             const float le[4] = {m.ke[0], m.ke[1], m.ke[2], 0.0f};
@@ -239,6 +247,16 @@ int ilm_obj_arrays(ilm_obj* o, float* pos, float* nrm, int32_t* idx, int32_t* tr
     cp(tri_light, o->tri_light.data(), o->tri_light.size() * sizeof(int32_t));
     cp(mats, o->mats.data(), o->mats.size() * sizeof(float));
     cp(lights, o->lights.data(), o->lights.size() * sizeof(float));
+    return ILM_OK;
+}
+
+int ilm_obj_material_info(ilm_obj* o, int32_t* illum, char* mtllib, int mtllib_len) {
+    if (!o) return fail(ILM_ERR_INVALID, "null object");
+    if (illum && !o->illum.empty()) std::memcpy(illum, o->illum.data(), o->illum.size() * sizeof(int32_t));
+    if (mtllib && mtllib_len > 0) {
+        std::strncpy(mtllib, o->mtllib.c_str(), (size_t)mtllib_len - 1);
+        mtllib[mtllib_len - 1] = 0;
+    }
     return ILM_OK;
 }
 

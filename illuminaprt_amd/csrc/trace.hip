@@ -4,7 +4,9 @@
 // Design: persistent workgroups sized to the chip's occupancy; each wave
 // refills idle lanes with new rays from a global counter (one atomic per
 // wave per refill, prefix via ballot/mbcnt), so incoherent rays of very
-// different traversal length keep SIMD lanes busy. Per lane: a 16-entry
+// different traversal length keep SIMD lanes busy. A wave refills once at
+// least refill_min of its lanes are idle (32 by default; 64, i.e. only when
+// the whole wave is idle, for coherent camera rays). Per lane: a 16-entry
 // traversal stack in LDS (lane-strided, conflict-free) that spills its
 // oldest entries to a global area only when deeper than 16. Nodes are BVH4,
 // 112 bytes, loaded as seven 16-byte vectors; children are sorted by entry
@@ -21,7 +23,6 @@ namespace ilm {
 
 constexpr int kBlock = 256;
 constexpr int kLds = 16;
-constexpr int kRefillMin = 16;
 
 struct TraceArgs {
     const float4* __restrict__ nodes;  // 7 float4 per BVH4 node
@@ -35,6 +36,7 @@ struct TraceArgs {
     uint32_t* __restrict__ spill;
     int32_t spill_depth;
     int32_t rooted;                   // start node of each ray from its tmin bits
+    int32_t refill_min;               // idle lanes that trigger a wave's refill (1..64)
     int64_t grid_threads;
     unsigned long long* counters;
     uint32_t total_waves;
@@ -158,7 +160,7 @@ __global__ void __launch_bounds__(kBlock) k_trace4(TraceArgs a) {
 
     for (;;) {
         unsigned long long idle = __ballot(!active);
-        if (!exhausted && idle != 0ull && (__popcll(idle) >= kRefillMin || idle == ~0ull)) {
+        if (!exhausted && idle != 0ull && (__popcll(idle) >= a.refill_min || idle == ~0ull)) {
             unsigned long long base = 0;
             if (__lane_id() == 0) base = atomicAdd(&a.counters[0], (unsigned long long)__popcll(idle));
             base = __shfl(base, 0);
@@ -235,7 +237,8 @@ __global__ void __launch_bounds__(kBlock) k_trace4(TraceArgs a) {
 
 template <bool ANY, bool STATS>
 static int launch_trace(ilm_mesh* m, const ilm_ray* rays, int64_t n, ilm_hit* hits, uint8_t* occ, int32_t* stats,
-                        void* stream, const unsigned long long* n_dev = nullptr, bool rooted = false) {
+                        void* stream, const unsigned long long* n_dev = nullptr, bool rooted = false,
+                        int refill = 0) {
     if (!m) return fail(ILM_ERR_INVALID, "null mesh");
     if (!m->built) return fail(ILM_ERR_STATE, "BVH not built");
     if (!m->ctx) return fail(ILM_ERR_STATE, "mesh was created without a device context");
@@ -266,6 +269,7 @@ static int launch_trace(ilm_mesh* m, const ilm_ray* rays, int64_t n, ilm_hit* hi
     a.stats = stats;
     a.grid_threads = blocks * kBlock;
     a.rooted = rooted ? 1 : 0;
+    a.refill_min = refill > 0 ? refill : ctx->trace_refill;
     a.spill_depth = std::max(1, m->info.stack_need + 1);
     int rc = ensure_spill(ctx, (size_t)a.spill_depth * (size_t)a.grid_threads);
     if (rc) return rc;
@@ -496,15 +500,22 @@ __global__ void k_copy(const float4* __restrict__ src, float4* __restrict__ dst,
 
 namespace ilm {
 int trace_queue(ilm_mesh* m, const ilm_ray* rays, const unsigned long long* n_dev, int64_t capacity, ilm_hit* hits,
-                uint8_t* occ, hipStream_t stream, bool rooted) {
-    if (occ) return launch_trace<true, false>(m, rays, capacity, nullptr, occ, nullptr, stream, n_dev, rooted);
-    return launch_trace<false, false>(m, rays, capacity, hits, nullptr, nullptr, stream, n_dev, rooted);
+                uint8_t* occ, hipStream_t stream, bool rooted, int refill) {
+    if (occ) return launch_trace<true, false>(m, rays, capacity, nullptr, occ, nullptr, stream, n_dev, rooted, refill);
+    return launch_trace<false, false>(m, rays, capacity, hits, nullptr, nullptr, stream, n_dev, rooted, refill);
 }
 }  // namespace ilm
 
 using namespace ilm;
 
 extern "C" {
+
+int ilm_ctx_set_trace_refill(ilm_ctx* ctx, int idle_lanes) {
+    if (!ctx) return fail(ILM_ERR_INVALID, "null ctx");
+    if (idle_lanes < 1 || idle_lanes > 64) return fail(ILM_ERR_INVALID, "idle_lanes must be in 1..64");
+    ctx->trace_refill = idle_lanes;
+    return ILM_OK;
+}
 
 int ilm_trace_closest(ilm_mesh* mesh, const ilm_ray* rays, int64_t n, ilm_hit* hits, void* stream) {
     return launch_trace<false, false>(mesh, rays, n, hits, nullptr, nullptr, stream);

@@ -59,7 +59,68 @@ bool ArgumentMap::GetArgument(const std::string& name, float& v) const {
 bool ArgumentMap::GetArgument(const std::string& name, std::string& v) const {
     auto it = m_args.find(name);
     if (it == m_args.end()) return false;
-    v = it->second;
+    std::stringstream ss(it->second);
+    std::string x;
+    if (!(ss >> x)) return false;
+    v = x;
+    return true;
+}
+
+bool ArgumentMap::GetArgument(const std::string& name, Vector3& v) const {
+    auto it = m_args.find(name);
+    if (it == m_args.end()) return false;
+    std::stringstream ss(it->second);
+    char sep;
+    float x, y, z;
+    if (!(ss >> sep >> x >> sep >> y >> sep >> z)) return false;
+    v = Vector3(x, y, z);
+    return true;
+}
+
+bool ArgumentMap::GetArgument(const std::string& name, float (&m)[9]) const {
+    auto it = m_args.find(name);
+    if (it == m_args.end()) return false;
+    std::stringstream ss(it->second);
+    char sep;
+    float e[9];
+    if (!(ss >> sep)) return false;
+    for (int k = 0; k < 9; ++k) {
+        if (!(ss >> e[k])) return false;
+        if (!(ss >> sep)) return false;  // ',' between elements, '}' after the last
+    }
+    for (int k = 0; k < 9; ++k) m[k] = e[k];
+    return true;
+}
+
+bool ArgumentMap::GetArgument(const std::string& name, std::vector<std::string>& v) const {
+    auto it = m_args.find(name);
+    if (it == m_args.end()) return true;  // as the reference: an absent list reads as empty
+    std::string cur;
+    for (char c : it->second + "}") {
+        if (c == '{' || c == ',' || c == '}') {
+            std::string t = trim(cur);
+            if (!t.empty()) v.push_back(t);
+            cur.clear();
+        } else {
+            cur += c;
+        }
+    }
+    return true;
+}
+
+bool ArgumentMap::GetArgument(const std::string& name, std::vector<Vector3>& v) const {
+    auto it = m_args.find(name);
+    if (it == m_args.end()) return true;
+    // "{{x, y, z}, ..., {x, y, z}}" (ArgumentMap.h:145-167)
+    std::stringstream ss(it->second);
+    char sep = 0;
+    float x, y, z;
+    ss >> sep;
+    do {
+        if (!(ss >> sep >> x >> sep >> y >> sep >> z >> sep)) return false;
+        v.push_back(Vector3(x, y, z));
+        if (!(ss >> sep)) return false;
+    } while (sep == ',');
     return true;
 }
 

@@ -30,23 +30,34 @@ inline void Check(int rc) {
     if (rc != ILM_OK) throw new Exception(ilm_last_error());
 }
 
-// "Key = Value; Key = Value" pairs (the reference tokenises .ilm blocks into
-// the same name/value form, Core/System/ArgumentMap.h).
-class ArgumentMap {
-public:
-    ArgumentMap() {}
-    explicit ArgumentMap(const std::string& text);
-    bool GetArgument(const std::string& name, int& v) const;
-    bool GetArgument(const std::string& name, float& v) const;
-    bool GetArgument(const std::string& name, std::string& v) const;
-private:
-    std::map<std::string, std::string> m_args;
-};
-
 struct Vector3 {
     float X = 0, Y = 0, Z = 0;
     Vector3() {}
     Vector3(float x, float y, float z) : X(x), Y(y), Z(z) {}
+};
+
+// Name -> value text, read as the reference reads it (Core/System/ArgumentMap.h):
+// numbers and strings through a stream (a string is its first
+// whitespace-delimited word), vectors as "{x, y, z}" (one separator
+// character between numbers), a Matrix3x3 as nine numbers in storage order,
+// and lists as "{a, b, ...}" split at '{', ',' and '}' (list items are
+// trimmed here; the reference keeps their leading blanks).
+class ArgumentMap {
+public:
+    ArgumentMap() {}
+    // "Key = Value; Key = Value" (ArgumentMap::Initialise(string), :53-73)
+    explicit ArgumentMap(const std::string& text);
+    explicit ArgumentMap(const std::map<std::string, std::string>& args) : m_args(args) {}
+    bool ContainsArgument(const std::string& name) const { return m_args.count(name) != 0; }
+    bool GetArgument(const std::string& name, int& v) const;
+    bool GetArgument(const std::string& name, float& v) const;
+    bool GetArgument(const std::string& name, std::string& v) const;
+    bool GetArgument(const std::string& name, Vector3& v) const;
+    bool GetArgument(const std::string& name, float (&m)[9]) const;
+    bool GetArgument(const std::string& name, std::vector<std::string>& v) const;
+    bool GetArgument(const std::string& name, std::vector<Vector3>& v) const;
+private:
+    std::map<std::string, std::string> m_args;
 };
 
 // Geometry/Ray.h:17-54: Min is kept but, as in the reference triangle test,

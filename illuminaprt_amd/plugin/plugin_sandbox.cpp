@@ -26,6 +26,17 @@
 //            int32 idx[3nt]; float nrm[3nv]; int32 tri_mat[nt], tri_light[nt];
 //            float mats[8 nmat], lights[4 nlight]
 //       out: float image[width * height * 3] (mean radiance per pixel)
+//   plugin_sandbox <plugin.so> ilm-load <script.ilm> <out.bin>
+//       loads a scene script on the host (no GPU) and writes the resolved job:
+//       int64 nv, nt, nmat, nlight; int32 width, height, spp, ray depth,
+//       builder; float epsilon; float camera[11]; float pos[3nv]; int32
+//       idx[3nt]; float nrm[3nv]; int32 tri_mat[nt], tri_light[nt]; float
+//       mats[8 nmat], lights[4 nlight]; int64 nprims; int64
+//       first_triangle[nprims + 1]. Prints the loader's warnings.
+//   plugin_sandbox <plugin.so> ilm-render <script.ilm> <seed> <out.bin>
+//       loads the script, renders it on the GPU, writes the image device's
+//       file (PPM, or PFM with Format = PFM) and the mean radiance in the
+//       device's pixel order to out.bin.
 //
 // Exit status 0 on success; a plug-in exception prints its message and exits 3.
 #include <dlfcn.h>
@@ -36,6 +47,7 @@
 #include <string>
 #include <vector>
 
+#include "ilm_environment.hpp"
 #include "ilm_plugin.hpp"
 
 using namespace IlluminaGpu;
@@ -314,6 +326,59 @@ static int run_render(IPlugIn* plugin, const char* in, const char* out, const ch
     return 0;
 }
 
+template <class T>
+static void write_vec(FILE* o, const std::vector<T>& v) {
+    if (!v.empty()) fwrite(v.data(), sizeof(T), v.size(), o);
+}
+
+static int run_ilm_load(const char* script, const char* out) {
+    EnvironmentLoader loader;
+    if (!loader.Load(std::string(script))) throw new Exception(std::string("cannot open ") + script);
+    for (const std::string& w : loader.Warnings()) printf("warning: %s\n", w.c_str());
+    const RenderJob& j = loader.Job();
+    FILE* o = fopen(out, "wb");
+    if (!o) throw new Exception(std::string("cannot open ") + out);
+    const int64_t head[4] = {(int64_t)j.Positions.size() / 3, (int64_t)j.Indices.size() / 3,
+                             (int64_t)j.Materials.size() / 8, (int64_t)j.Lights.size() / 4};
+    const int32_t ints[5] = {j.Width, j.Height, j.Samples, j.RayDepth, j.Builder};
+    const ilm_camera& c = j.Camera;
+    const float cam[11] = {c.pos[0], c.pos[1], c.pos[2], c.target[0], c.target[1], c.target[2],
+                           c.up[0],  c.up[1],  c.up[2],  c.fov_deg,   c.aspect};
+    fwrite(head, sizeof head, 1, o);
+    fwrite(ints, sizeof ints, 1, o);
+    fwrite(&j.Epsilon, sizeof(float), 1, o);
+    fwrite(cam, sizeof cam, 1, o);
+    write_vec(o, j.Positions);
+    write_vec(o, j.Indices);
+    write_vec(o, j.Normals);
+    write_vec(o, j.TriMaterial);
+    write_vec(o, j.TriLight);
+    write_vec(o, j.Materials);
+    write_vec(o, j.Lights);
+    const int64_t np = (int64_t)j.PrimitiveIds.size();
+    fwrite(&np, sizeof np, 1, o);
+    write_vec(o, j.PrimitiveFirstTriangle);
+    fclose(o);
+    printf("ilm-load ok: %lld primitives, %lld triangles, %lld lights\n", (long long)np, (long long)head[1],
+           (long long)head[3]);
+    return 0;
+}
+
+static int run_ilm_render(const char* script, const char* seed, const char* out) {
+    EnvironmentLoader loader;
+    if (!loader.Load(std::string(script))) throw new Exception(std::string("cannot open ") + script);
+    for (const std::string& w : loader.Warnings()) printf("warning: %s\n", w.c_str());
+    std::vector<float> image;
+    loader.Job().Render((uint32_t)strtoul(seed, nullptr, 10), image, true);
+    FILE* o = fopen(out, "wb");
+    if (!o) throw new Exception(std::string("cannot open ") + out);
+    write_vec(o, image);
+    fclose(o);
+    printf("ilm-render ok: %dx%d, %d spp -> %s\n", loader.Job().Width, loader.Job().Height, loader.Job().Samples,
+           loader.Job().Filename.c_str());
+    return 0;
+}
+
 int main(int argc, char** argv) {
     if (argc < 3) {
         fprintf(stderr, "usage: %s <plugin.so> host | trace|render <in> <out> <args>\n", argv[0]);
@@ -342,6 +407,10 @@ int main(int argc, char** argv) {
             rc = run_space(plugin, argv[3], argv[4]);
         else if (mode == "render" && argc == 6)
             rc = run_render(plugin, argv[3], argv[4], argv[5]);
+        else if (mode == "ilm-load" && argc == 5)
+            rc = run_ilm_load(argv[3], argv[4]);
+        else if (mode == "ilm-render" && argc == 6)
+            rc = run_ilm_render(argv[3], argv[4], argv[5]);
         else
             fprintf(stderr, "bad mode or arguments\n");
     } catch (Exception* e) {

@@ -172,6 +172,12 @@ int ilm_bvh_export(ilm_mesh* mesh, void* nodes, int32_t* tri_order);
 
 /* ---------------------------------------------------------------- tracing */
 
+/* Sets how many idle lanes (1..64, default 32) a traversal wave waits for
+ * before it fetches new rays for them: fewer means more refills (one atomic
+ * and a divergent ray load each), more means idle lanes for longer. Applies
+ * to later traversal launches of the context. No reference counterpart. */
+int ilm_ctx_set_trace_refill(ilm_ctx* ctx, int idle_lanes);
+
 /* Closest hit for n rays. Replaces IShape::Intersects(const Ray&,
  * DifferentialSurface&) of BVHMesh (Shape/BVHMesh.cpp:114-119,169-198) and
  * ISpace::Intersects(const Ray&, Intersection&) (Space/BasicSpace.cpp:58-73),
@@ -375,6 +381,12 @@ int ilm_obj_counts(ilm_obj* obj, int64_t* nv, int64_t* nt, int32_t* nmat, int32_
  * mats (nmat*8), lights (nlight*4). Any pointer may be NULL. */
 int ilm_obj_arrays(ilm_obj* obj, float* pos, float* nrm, int32_t* idx, int32_t* tri_mat, int32_t* tri_light,
                    float* mats, float* lights);
+/* Per material (nmat entries): the MTL illumination model ("illum", 2 when
+ * absent; the reference reads 4 as glass and 5 as mirror,
+ * Scene/WavefrontSceneLoader.cpp:214-228), and the file name of the first
+ * "mtllib" (empty when none; the reference names the material group after
+ * it, :306-307). Either pointer may be NULL. */
+int ilm_obj_material_info(ilm_obj* obj, int32_t* illum, char* mtllib, int mtllib_len);
 int ilm_obj_destroy(ilm_obj* obj);
 
 /* ---------------------------------------------------------------- frames and images */

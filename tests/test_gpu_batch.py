@@ -42,7 +42,9 @@ def test_batch_views_match_single_scenes(gpu_ctx, ilm):
     scenes = [sc for _, sc in objs]
     cam_room = w.room_camera()
     cam_box = ilm.Camera.make((0.5, 0.5, 0.1), (0.5, 0.5, 1.0), (0, 1, 0), 60.0, 1.0)
-    views = [(0, cam_room, 64, 64, 8), (1, cam_room, 48, 40, 8), (2, cam_box, 32, 32, 16), (0, cam_room, 40, 64, 8)]
+    # the glossy view's mean is dominated by its highlights' variance: 32 spp
+    # keeps the mean comparison below well clear of the noise
+    views = [(0, cam_room, 64, 64, 8), (1, cam_room, 48, 40, 32), (2, cam_box, 32, 32, 128), (0, cam_room, 40, 64, 8)]
     b = ilm.Batch(gpu_ctx, scenes)
     npix = sum(v[2] * v[3] for v in views)
     acc = gpu_ctx.buffer(npix * 24).zero()
@@ -56,10 +58,12 @@ def test_batch_views_match_single_scenes(gpu_ctx, ilm):
     # first view: the same streams as the scene alone
     alone = _single(ilm, gpu_ctx, scenes[0], cam_room, 64, 64, 8, seed=77)
     assert np.array_equal(imgs[0], alone)
-    # furnace view: its closed form (only its own six lights are sampled)
+    # furnace view: its closed form (only its own six lights are sampled).
+    # The image mean's standard deviation at 32 x 32 x 128 paths is about
+    # 0.3% (0.8% at 16 spp, measured on the oracle over 12 seeds): 1.2% is 4 sigma.
     want = le * sum(rho ** k for k in range(6))
     print(f"[batch furnace] {imgs[2].mean():.5f} vs {want:.5f}", flush=True)
-    assert abs(imgs[2].mean() - want) < 0.01 * want
+    assert abs(imgs[2].mean() - want) < 0.012 * want
     # the other room views: the noise level of equal-spp single renders
     from illuminaprt_amd import workloads as ww
     for k, (s, cam, W, H, spp) in enumerate(views):

@@ -124,13 +124,13 @@ def test_training_fits_a_smooth_field(gpu_ctx, ilm):
 
 def test_trained_cache_beats_plain_pt_at_equal_time(gpu_ctx, ilm):
     """What the learned part gains: on the radiance-cache workload at
-    960 x 540, cache rendering (trained online from a cold start) reaches a
+    1920 x 1080, cache rendering (trained online from a cold start) reaches a
     lower relRMSE than plain path tracing given the same GPU time, and the
     same renderer with training switched off is far worse (the untrained
     network's bias). Errors are against a 256-spp plain render whose own
     noise, estimated from two independent halves, is removed."""
     from illuminaprt_amd import workloads as w
-    W, H, depth = 960, 540, 10
+    W, H, depth = 1920, 1080, 10
     pos, nrm, idx, tm, tl, mats, lights = w.cache_scene()
     m = ilm.Mesh(gpu_ctx, pos, idx)
     m.build()
@@ -160,7 +160,7 @@ def test_trained_cache_beats_plain_pt_at_equal_time(gpu_ctx, ilm):
         ms = 0.0
         for f in range(frames):
             st = sc.render_nrc(net, cam, W, H, 1, buf if f >= skip else None, seed=7000 + f, prefix_depth=2,
-                               suffix_depth=depth - 2, train_fraction=1 / 16, train_batch=32768,
+                               suffix_depth=depth - 2, train_fraction=1 / 16, train_batch=65536,
                                train_steps=train_steps)
             ms += st["ms"]
         net.close()
