@@ -601,6 +601,7 @@ def main():
         "bvh_build": build,
         "large_scene": large,
         "batched_scenes": batch,
+        "large_render": lrender,
     }
     print(json.dumps(out), flush=True)
     for b in (hits, occ, visits):

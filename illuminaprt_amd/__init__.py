@@ -154,6 +154,11 @@ SIGNATURES = {
     "ilm_obj_arrays": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "ilm_obj_material_info": (C.c_int, [_vp, _vp, C.c_char_p, C.c_int]),
     "ilm_obj_destroy": (C.c_int, [_vp]),
+    "ilm_persistent_load": (C.c_int, [C.c_char_p, C.POINTER(_vp)]),
+    "ilm_persistent_counts": (C.c_int, [_vp, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64)]),
+    "ilm_persistent_arrays": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "ilm_persistent_build": (C.c_int, [_vp, _vp, C.c_int]),
+    "ilm_persistent_destroy": (C.c_int, [_vp]),
     "ilm_image_resolve": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_double, C.c_int, _vp, _vp]),
     "ilm_image_tonemap": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, _vp, _vp]),
     "ilm_image_write": (C.c_int, [C.c_char_p, _vp, C.c_int, C.c_int, C.c_int]),
@@ -430,6 +435,37 @@ class Batch:
     def close(self):
         if self.h:
             load().ilm_batch_destroy(self.h)
+            self.h = None
+
+
+class Persistent:
+    """PersistentMesh files (<trunk>.ver / .tri / .tre): a mesh and its prebuilt kd-tree."""
+
+    def __init__(self, trunk: str):
+        h = C.c_void_p()
+        check(load().ilm_persistent_load(trunk.encode(), C.byref(h)))
+        self.h = h.value
+        nv, nt, nn, nr = _i64(), _i64(), _i64(), _i64()
+        check(load().ilm_persistent_counts(self.h, C.byref(nv), C.byref(nt), C.byref(nn), C.byref(nr)))
+        self.nv, self.nt, self.tree_nodes, self.tree_refs = nv.value, nt.value, nn.value, nr.value
+
+    def arrays(self):
+        """pos, nrm (nv x 3 float32), idx (nt x 3 int32), group (nt int32)."""
+        pos = np.zeros((self.nv, 3), np.float32)
+        nrm = np.zeros((self.nv, 3), np.float32)
+        idx = np.zeros((self.nt, 3), np.int32)
+        group = np.zeros(self.nt, np.int32)
+        check(load().ilm_persistent_arrays(self.h, _ptr(pos), _ptr(nrm), _ptr(idx), _ptr(group)))
+        return pos, nrm, idx, group
+
+    def build(self, mesh: "Mesh", node_format=ILM_NODES_FLOAT) -> dict:
+        """Installs the file's kd-tree as mesh's BVH (mesh made from arrays())."""
+        check(load().ilm_persistent_build(self.h, mesh.h, node_format))
+        return mesh.info()
+
+    def close(self):
+        if self.h:
+            load().ilm_persistent_destroy(self.h)
             self.h = None
 
 

@@ -231,6 +231,63 @@ int ilm_mesh_destroy(ilm_mesh* m) {
     return ILM_OK;
 }
 
+}  // extern "C"
+
+namespace ilm {
+
+int install_bvh(ilm_mesh* m, Bvh2& b, int node_format, std::chrono::steady_clock::time_point t0) {
+    WideBvh w;
+    try {
+        w = collapse(b, 4);
+    } catch (const std::exception& e) {
+        return fail(ILM_ERR_INVALID, std::string("BVH build failed: ") + e.what());
+    }
+    std::vector<float> tris;
+    pack_triangles(m->pos.data(), m->idx.data(), w.order, tris);
+    if (node_format == ILM_NODES_QUANTIZED) {
+        std::vector<uint8_t> q((size_t)w.nnodes * sizeof(QNode4));
+        const auto* src = reinterpret_cast<const WideNode<4>*>(w.nodes.data());
+        auto* dst = reinterpret_cast<QNode4*>(q.data());
+        for (int64_t i = 0; i < w.nnodes; ++i) quantize_node(src[i], dst[i]);
+        w.nodes.swap(q);
+    }
+    m->node_format = node_format;
+    m->info.node_format = node_format;
+    m->info.node_stride = node_format == ILM_NODES_QUANTIZED ? (int32_t)sizeof(QNode4) : (int32_t)sizeof(WideNode<4>);
+    auto t1 = std::chrono::steady_clock::now();
+    if (m->ctx) {  // a host-only mesh keeps the flattened tree for export only
+        if (m->d_nodes) { ilm_free(m->ctx, m->d_nodes); m->d_nodes = nullptr; }
+        if (m->d_tris) { ilm_free(m->ctx, m->d_tris); m->d_tris = nullptr; }
+        if (m->d_slot_of) { ilm_free(m->ctx, m->d_slot_of); m->d_slot_of = nullptr; }
+        int rc = ilm_alloc(m->ctx, w.nodes.size(), &m->d_nodes);
+        if (rc) return rc;
+        rc = ilm_alloc(m->ctx, tris.size() * sizeof(float), (void**)&m->d_tris);
+        if (rc) return rc;
+        rc = ilm_upload(m->ctx, m->d_nodes, w.nodes.data(), w.nodes.size());
+        if (rc) return rc;
+        rc = ilm_upload(m->ctx, m->d_tris, tris.data(), tris.size() * sizeof(float));
+        if (rc) return rc;
+    }
+    m->width = w.width;
+    m->info.nodes = w.nnodes;
+    m->info.node_bytes = (int64_t)w.nodes.size();
+    m->info.tri_bytes = (int64_t)(tris.size() * sizeof(float));
+    m->info.width = w.width;
+    m->info.depth = w.depth;
+    m->info.max_leaf = w.max_leaf;
+    m->info.stack_need = w.stack_need;
+    m->info.build_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    m->info.sah_cost = w.sah_cost;
+    m->h_nodes.swap(w.nodes);
+    m->h_order.swap(w.order);
+    m->built = true;
+    return ILM_OK;
+}
+
+}  // namespace ilm
+
+extern "C" {
+
 int ilm_bvh_build(ilm_mesh* m, const ilm_bvh_params* params) {
     if (!m) return fail(ILM_ERR_INVALID, "null mesh");
     ilm_bvh_params p{};
@@ -244,9 +301,8 @@ int ilm_bvh_build(ilm_mesh* m, const ilm_bvh_params* params) {
         return ilm::build_lbvh(m, p.max_leaf > 0 ? p.max_leaf : 4, p.node_format);
     }
     auto t0 = std::chrono::steady_clock::now();
-    ilm::WideBvh w;
+    ilm::Bvh2 b;
     try {
-        ilm::Bvh2 b;
         if (p.builder == ILM_BUILDER_REFERENCE) {
             b = ilm::build_reference(m->pos.data(), m->idx.data(), m->nt, p.max_leaf > 0 ? p.max_leaf : 10,
                                      p.max_depth > 0 ? p.max_depth : 35);
@@ -263,65 +319,10 @@ int ilm_bvh_build(ilm_mesh* m, const ilm_bvh_params* params) {
         } else {
             return fail(ILM_ERR_INVALID, "unknown builder");
         }
-        w = ilm::collapse(b, p.width);
     } catch (const std::exception& e) {
         return fail(ILM_ERR_INVALID, std::string("BVH build failed: ") + e.what());
     }
-    std::vector<float> tris;
-    ilm::pack_triangles(m->pos.data(), m->idx.data(), w.order, tris);
-    if (p.node_format == ILM_NODES_QUANTIZED) {
-        std::vector<uint8_t> q((size_t)w.nnodes * sizeof(ilm::QNode4));
-        const auto* src = reinterpret_cast<const ilm::WideNode<4>*>(w.nodes.data());
-        auto* dst = reinterpret_cast<ilm::QNode4*>(q.data());
-        for (int64_t i = 0; i < w.nnodes; ++i) ilm::quantize_node(src[i], dst[i]);
-        w.nodes.swap(q);
-    }
-    m->node_format = p.node_format;
-    m->info.node_format = p.node_format;
-    m->info.node_stride = p.node_format == ILM_NODES_QUANTIZED ? (int32_t)sizeof(ilm::QNode4)
-                                                               : (int32_t)sizeof(ilm::WideNode<4>);
-    auto t1 = std::chrono::steady_clock::now();
-    if (!m->ctx) {  // host-only mesh: keep the flattened tree for export
-        m->width = w.width;
-        m->info.nodes = w.nnodes;
-        m->info.node_bytes = (int64_t)w.nodes.size();
-        m->info.tri_bytes = (int64_t)(tris.size() * sizeof(float));
-        m->info.width = w.width;
-        m->info.depth = w.depth;
-        m->info.max_leaf = w.max_leaf;
-        m->info.stack_need = w.stack_need;
-        m->info.build_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
-        m->info.sah_cost = w.sah_cost;
-        m->h_nodes.swap(w.nodes);
-        m->h_order.swap(w.order);
-        m->built = true;
-        return ILM_OK;
-    }
-    if (m->d_nodes) { ilm_free(m->ctx, m->d_nodes); m->d_nodes = nullptr; }
-    if (m->d_tris) { ilm_free(m->ctx, m->d_tris); m->d_tris = nullptr; }
-    if (m->d_slot_of) { ilm_free(m->ctx, m->d_slot_of); m->d_slot_of = nullptr; }
-    int rc = ilm_alloc(m->ctx, w.nodes.size(), &m->d_nodes);
-    if (rc) return rc;
-    rc = ilm_alloc(m->ctx, tris.size() * sizeof(float), (void**)&m->d_tris);
-    if (rc) return rc;
-    rc = ilm_upload(m->ctx, m->d_nodes, w.nodes.data(), w.nodes.size());
-    if (rc) return rc;
-    rc = ilm_upload(m->ctx, m->d_tris, tris.data(), tris.size() * sizeof(float));
-    if (rc) return rc;
-    m->width = w.width;
-    m->info.nodes = w.nnodes;
-    m->info.node_bytes = (int64_t)w.nodes.size();
-    m->info.tri_bytes = (int64_t)(tris.size() * sizeof(float));
-    m->info.width = w.width;
-    m->info.depth = w.depth;
-    m->info.max_leaf = w.max_leaf;
-    m->info.stack_need = w.stack_need;
-    m->info.build_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
-    m->info.sah_cost = w.sah_cost;
-    m->h_nodes.swap(w.nodes);
-    m->h_order.swap(w.order);
-    m->built = true;
-    return ILM_OK;
+    return ilm::install_bvh(m, b, p.node_format, t0);
 }
 
 int ilm_bvh_get_info(ilm_mesh* m, ilm_bvh_info* info) {

@@ -334,6 +334,77 @@ Bvh2 build_kdtree(const float* pos, const int32_t* idx, int64_t nt, int max_leaf
     return out;
 }
 
+Bvh2 import_kdtree(const uint8_t* tre, size_t bytes, const float* pos, int64_t nv, int64_t nt) {
+    if (nv <= 0 || nt <= 0) throw std::invalid_argument("empty mesh");
+    if (nt > (1 << 26) - 1) throw std::invalid_argument("more than 2^26 - 1 triangles");
+    RefBox root;
+    for (int k = 0; k < 3; ++k) { root.mn[k] = FLT_MAX; root.mx[k] = -FLT_MAX; }
+    for (int64_t v = 0; v < nv; ++v)
+        for (int k = 0; k < 3; ++k) {
+            root.mn[k] = std::min(root.mn[k], pos[3 * v + k]);
+            root.mx[k] = std::max(root.mx[k], pos[3 * v + k]);
+        }
+    for (int k = 0; k < 3; ++k) { root.mn[k] -= 1e-4f; root.mx[k] += 1e-4f; }
+    auto word = [&](size_t off) {
+        uint32_t w;
+        std::memcpy(&w, tre + off, 4);
+        return w;
+    };
+    Bvh2 out;
+    std::vector<uint8_t> seen((bytes + 3) / 4, 0);
+    struct Item { size_t off; int32_t node; RefBox cell; int depth; };
+    std::vector<Item> stack;
+    out.nodes.emplace_back();
+    stack.push_back({0, 0, root, 0});
+    while (!stack.empty()) {
+        Item it = stack.back();
+        stack.pop_back();
+        if (it.off % 4 || it.off + 8 > bytes) throw std::invalid_argument("tree node outside the file");
+        if (seen[it.off / 4]) throw std::invalid_argument("tree node reached twice");
+        seen[it.off / 4] = 1;
+        float part;
+        std::memcpy(&part, tre + it.off, 4);
+        const uint32_t bits = word(it.off + 4);
+        const int axis = (int)(bits & 3u);
+        const uint32_t count = bits >> 2;
+        Bvh2Node& n = out.nodes[it.node];
+        for (int k = 0; k < 3; ++k) { n.lo[k] = it.cell.mn[k]; n.hi[k] = it.cell.mx[k]; }
+        if (axis == 3) {
+            if (it.off + 8 + 4 * (size_t)count > bytes) throw std::invalid_argument("leaf runs past the file");
+            n.left = n.right = -1;
+            n.first = (int32_t)out.order.size();
+            n.count = (int32_t)count;
+            if ((int64_t)out.order.size() + count > (int64_t)(1 << 26) - 1)
+                throw std::invalid_argument("kd-tree references exceed 2^26 - 1 triangle slots");
+            for (uint32_t i = 0; i < count; ++i) {
+                const int32_t t = (int32_t)word(it.off + 8 + 4 * (size_t)i);
+                if (t < 0 || t >= nt) throw std::invalid_argument("leaf triangle index out of range");
+                out.order.push_back(t);
+            }
+            continue;
+        }
+        if (it.off + 16 > bytes) throw std::invalid_argument("interior node runs past the file");
+        const int32_t c0 = (int32_t)word(it.off + 8), c1 = (int32_t)word(it.off + 12);
+        if (c0 < 0 || c1 < 0) throw std::invalid_argument("negative child offset");
+        if (!std::isfinite(part)) throw std::invalid_argument("partition is not finite");
+        const float cut = std::min(std::max(part, it.cell.mn[axis]), it.cell.mx[axis]);
+        RefBox below = it.cell, above = it.cell;
+        below.mx[axis] = cut;
+        above.mn[axis] = cut;
+        const int32_t l = (int32_t)out.nodes.size();
+        out.nodes.emplace_back();
+        out.nodes.emplace_back();
+        out.nodes[it.node].left = l;
+        out.nodes[it.node].right = l + 1;
+        out.nodes[it.node].count = 0;
+        // the reference traverses with a 64-entry stack (PersistentMesh.cpp:189-190)
+        if (it.depth + 1 > 64) throw std::invalid_argument("tree deeper than 64 levels");
+        stack.push_back({(size_t)c1, l + 1, above, it.depth + 1});
+        stack.push_back({(size_t)c0, l, below, it.depth + 1});
+    }
+    return out;
+}
+
 Bvh2 build_sah(const float* pos, const int32_t* idx, int64_t nt, int max_leaf, int threads) {
     if (nt <= 0) throw std::invalid_argument("empty mesh");
     if (nt > (1 << 26) - 1) throw std::invalid_argument("more than 2^26 - 1 triangles");

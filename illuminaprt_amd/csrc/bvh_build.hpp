@@ -35,6 +35,19 @@ Bvh2 build_reference(const float* pos, const int32_t* idx, int64_t nt, int max_l
 // Shape/KDTreeMeshEx.cpp:59-146) when it is true.
 Bvh2 build_kdtree(const float* pos, const int32_t* idx, int64_t nt, int max_leaf, int max_depth, bool median);
 
+// A kd-tree stored in PersistentMesh's tree file (Shape/PersistentMesh.cpp:
+// 171-245, written by KDTreeMesh.cpp:167-257): nodes of an 8-byte header
+// (float partition; uint32 axis in bits 0-1, item count in bits 2-31), an
+// interior node (axis 0-2) followed by the byte offsets of its below / above
+// children, a leaf (axis 3) by its triangle indices. Returns the tree of
+// cells: the root cell is the vertices' bounds widened by 1e-4 (the
+// KDTreeMesh build's root cell, KDTreeMesh.cpp:951), a child's cell is the
+// parent's cut at the partition (clamped to the cell). Throws
+// std::invalid_argument for a malformed file (offsets outside the file or
+// misaligned, a node reached twice, triangle indices out of range, more
+// than 64 levels).
+Bvh2 import_kdtree(const uint8_t* tre, size_t bytes, const float* pos, int64_t nv, int64_t nt);
+
 struct WideBvh {
     int width = 4;
     std::vector<uint8_t> nodes;    // WideNode<width> array, root at 0

@@ -4,12 +4,14 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <chrono>
 #include <mutex>
 #include <set>
 #include <string>
 #include <vector>
 
 #include "../../include/ilm_gpu.h"
+#include "bvh_build.hpp"
 
 namespace ilm {
 
@@ -69,6 +71,10 @@ void camera_frame(const ilm_camera& c, float out[12]);
 // device memory (*n_dev, at most `capacity`), without a host round trip.
 // rooted: each ray's tmin field holds (as bits) the BVH node to start from,
 // so one launch can trace rays of several BVHs packed in one node array.
+// Collapses b to BVH4, packs the triangles in leaf order, quantises the
+// nodes when asked, uploads (device meshes) and fills m's info and host
+// copies, as every host builder ends. t0 = when the build started.
+int install_bvh(ilm_mesh* m, Bvh2& b, int node_format, std::chrono::steady_clock::time_point t0);
 int trace_queue(ilm_mesh* m, const ilm_ray* rays, const unsigned long long* n_dev, int64_t capacity, ilm_hit* hits,
                 uint8_t* occ, hipStream_t stream, bool rooted = false, int refill = 0);
 // Builds m's BVH4 on its device (linear BVH + collapse); fills the device

@@ -308,7 +308,24 @@ EnvironmentLoader::Shape EnvironmentLoader::MakeShape(const std::string& type, c
         return s;
     }
     if (type == "PersistentMesh") {
-        s.unsupported = "PersistentMesh files are not read";
+        // ShapeFactories.h:337-370: TrunkName names <trunk>.ver / .tri / .tre.
+        // The geometry and group ids are read; the script's space is
+        // flattened and gets one tree, built as a KDTreeMesh.
+        std::string trunk;
+        if (!args.GetArgument("TrunkName", trunk)) throw new Exception("PersistentMesh needs TrunkName");
+        trunk = Resolve(trim_ws(trunk) + ".ver");
+        trunk = trunk.substr(0, trunk.size() - 4);
+        ilm_persistent* p = nullptr;
+        Check(ilm_persistent_load(trunk.c_str(), &p));
+        std::unique_ptr<ilm_persistent, int (*)(ilm_persistent*)> guard(p, ilm_persistent_destroy);
+        int64_t nv = 0, nt = 0;
+        Check(ilm_persistent_counts(p, &nv, &nt, nullptr, nullptr));
+        s.pos.resize(3 * nv);
+        s.nrm.resize(3 * nv);
+        s.idx.resize(3 * nt);
+        s.group.resize(nt);
+        Check(ilm_persistent_arrays(p, s.pos.data(), s.nrm.data(), s.idx.data(), s.group.data()));
+        s.builder = ILM_BUILDER_KDTREE;
         return s;
     }
     throw new Exception("No factory registered under specified name!");

@@ -7,7 +7,9 @@
 //
 // Resolution is host-only (no device is touched), so a script can be loaded
 // and inspected without a GPU; RenderJob::Render draws it with the GPU path.
-// What the GPU path cannot draw (spheres, mirror and glass materials, point,
+// Shapes: Quad, Triangle, WavefrontModel and PersistentMesh (geometry and
+// group ids; the flattened space gets one tree). What the GPU path cannot
+// draw (spheres, mirror and glass materials, point,
 // spot, directional and environment lights, the thin-lens camera, integrators
 // other than PathTracing) is recorded when its section is parsed and becomes
 // an error only if the environment uses it, as the reference reports a failed

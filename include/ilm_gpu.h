@@ -314,6 +314,32 @@ int ilm_scenegen_boxes(int n, float emin, float emax, float radius, uint32_t see
 int ilm_scenegen_vertex_normals(const float* pos, int64_t nv, const int32_t* idx, int64_t nt,
                                 float* nrm);
 
+/* ---------------------------------------------------------------- PersistentMesh files */
+
+/* The reference's on-disk mesh and kd-tree (Shape/PersistentMesh.h:22-36,
+ * PersistentMesh.cpp:77-89, 171-245; written by KDTreeMesh's export,
+ * KDTreeMesh.cpp:143-257): <trunk>.ver (48-byte Vertex records: position,
+ * normal, uv), <trunk>.tri (48-byte records: three vertex ids, group id, two
+ * edges), <trunk>.tre (kd-tree nodes: float partition, uint32 axis | count
+ * << 2, then the children's byte offsets or the leaf's triangle indices).
+ * Load validates all three files on the host (offsets inside the file and
+ * 4-byte aligned, every node reached once, indices in range, at most 64
+ * levels). */
+typedef struct ilm_persistent ilm_persistent;
+int ilm_persistent_load(const char* trunk, ilm_persistent** out);
+/* Vertex and triangle counts, the tree's node count (as a binary tree of
+ * cells) and its triangle references. Any pointer may be NULL. */
+int ilm_persistent_counts(ilm_persistent* p, int64_t* nv, int64_t* nt, int64_t* tree_nodes, int64_t* tree_refs);
+/* Host copies: positions and normals (nv*3), triangle vertex ids (nt*3) and
+ * group ids (nt, the material group of each triangle). Any pointer may be NULL. */
+int ilm_persistent_arrays(ilm_persistent* p, float* pos, float* nrm, int32_t* idx, int32_t* group);
+/* Gives mesh (created from this file's positions and indices, with or
+ * without a context) the file's kd-tree as its BVH: the cells collapsed to
+ * BVH4 nodes of the given ILM_NODES_* format, as ILM_BUILDER_KDTREE trees
+ * are, instead of building one. */
+int ilm_persistent_build(ilm_persistent* p, ilm_mesh* mesh, int node_format);
+int ilm_persistent_destroy(ilm_persistent* p);
+
 /* ---------------------------------------------------------------- path integrator */
 
 /* Material: 8 floats — kd rgb, ks rgb, phong exponent, type (0 = Lambertian
