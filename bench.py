@@ -117,6 +117,43 @@ def path_tracing_room(ctx, ilm, workloads, args):
     return out
 
 
+def frames_leg(ctx, ilm, workloads, args):
+    """A few plain and neural-cache frames of the radiance-cache workload
+    (1920 x 1080, depth 10, 1 spp each): a short run for kernel-level
+    profiles of the wavefront integrator."""
+    W, H, depth = args.cache_width, args.cache_height, 10
+    pos, nrm, idx, tm, tl, mats, lights = workloads.cache_scene()
+    mesh = ilm.Mesh(ctx, pos, idx)
+    mesh.build()
+    sc = ilm.Scene(ctx, mesh, nrm, tm, tl, mats, lights)
+    cam = workloads.cache_camera(W / H)
+    acc = ctx.buffer(W * H * 24).zero()
+    net = ilm.Nrc(ctx, *workloads.CACHE_BOUNDS, lr=args.cache_lr, seed=5)
+    plain, cache = [], []
+    for i in range(args.warmup + args.steps):
+        st = sc.render(cam, W, H, 1, acc, max_depth=depth, seed=i)
+        if i >= args.warmup:
+            plain.append(st)
+    for i in range(args.warmup + args.steps):
+        st = sc.render_nrc(net, cam, W, H, 1, acc, seed=500 + i, prefix_depth=2, suffix_depth=depth - 2)
+        if i >= args.warmup:
+            cache.append(st)
+    net.close()
+    acc.free()
+    sc.close()
+    mesh.close()
+
+    def summary(fr):
+        ms = float(np.mean([f["ms"] for f in fr]))
+        rays = fr[0]["camera_rays"] + fr[0]["extension_rays"] + fr[0]["shadow_rays"]
+        return {"frame_ms": round(ms, 3), "rays_per_frame": int(rays), "mrays_s": round(rays / ms / 1e3, 1)}
+
+    out = {"workload": f"radiance-cache scene {len(idx)} tris, {W}x{H}, 1 spp, depth {depth}",
+           "plain": summary(plain), "neural_cache": summary(cache)}
+    log(f"frames: {out}")
+    return out
+
+
 def _trace_rate(ctx, mesh, buf, n, steps, warmup, hits):
     for _ in range(warmup):
         mesh.trace_closest(buf, n, hits)
@@ -453,6 +490,8 @@ def main():
     ap.add_argument("--nodes", choices=["float", "quantized"], default="float",
                     help="BVH4 node format of the traced trees")
     ap.add_argument("--cache-only", action="store_true", help="run only the time-to-relRMSE leg")
+    ap.add_argument("--frames-only", action="store_true",
+                    help="run only a few plain and cache frames of the cache workload (profiling)")
     ap.add_argument("--cache-width", type=int, default=1920)
     ap.add_argument("--cache-height", type=int, default=1080)
     ap.add_argument("--cache-ref-spp", type=int, default=256, help="spp of each of the two reference renders")
@@ -484,6 +523,10 @@ def main():
         return
     if args.batch_only:
         print(json.dumps({"batched_scenes": batch_leg(ctx, ilm, workloads, args)}), flush=True)
+        ctx.close()
+        return
+    if args.frames_only:
+        print(json.dumps({"frames": frames_leg(ctx, ilm, workloads, args)}), flush=True)
         ctx.close()
         return
     if args.large_render_only:
