@@ -99,6 +99,18 @@ class NrcRenderParams(C.Structure):
                 ("train_batch", C.c_int32), ("train_steps", C.c_int32), ("accumulate", C.c_int32)]
 
 
+class IcParams(C.Structure):
+    _fields_ = [("cache_depth", C.c_int32), ("error_threshold", C.c_float), ("ambient_resolution", C.c_float),
+                ("ambient_multiplier", C.c_float), ("azimuth_strata", C.c_int32), ("altitude_strata", C.c_int32),
+                ("ray_depth", C.c_int32), ("eps", C.c_float), ("indirect_factor", C.c_float),
+                ("miss_radiance", C.c_float * 3), ("placement_stride", C.c_int32)]
+
+
+class IcStats(C.Structure):
+    _fields_ = [("records_created", C.c_int64), ("records_total", C.c_int64), ("gather_rays", C.c_int64),
+                ("lookups", C.c_int64), ("ms", C.c_double)]
+
+
 class RenderStats(C.Structure):
     _fields_ = [("camera_rays", C.c_int64), ("extension_rays", C.c_int64), ("shadow_rays", C.c_int64),
                 ("ms", C.c_double)]
@@ -154,6 +166,16 @@ SIGNATURES = {
     "ilm_obj_arrays": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "ilm_obj_material_info": (C.c_int, [_vp, _vp, C.c_char_p, C.c_int]),
     "ilm_obj_destroy": (C.c_int, [_vp]),
+    "ilm_ic_create": (C.c_int, [_vp, C.POINTER(IcParams), C.POINTER(_vp)]),
+    "ilm_ic_destroy": (C.c_int, [_vp]),
+    "ilm_ic_reset": (C.c_int, [_vp]),
+    "ilm_ic_render": (C.c_int, [_vp, C.POINTER(RenderParams), _vp, C.POINTER(IcStats)]),
+    "ilm_ic_insert": (C.c_int, [_vp, _vp, _i64]),
+    "ilm_ic_info": (C.c_int, [_vp, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64), C.POINTER(C.c_float),
+                              C.POINTER(C.c_float), _vp]),
+    "ilm_ic_records": (C.c_int, [_vp, _vp]),
+    "ilm_ic_node_records": (C.c_int, [_vp, _i64, C.POINTER(C.c_int32), C.POINTER(C.c_int32), _vp, C.c_int32]),
+    "ilm_ic_lookup": (C.c_int, [_vp, _vp, _i64, _vp, _vp]),
     "ilm_persistent_load": (C.c_int, [C.c_char_p, C.POINTER(_vp)]),
     "ilm_persistent_counts": (C.c_int, [_vp, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64)]),
     "ilm_persistent_arrays": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
@@ -435,6 +457,77 @@ class Batch:
     def close(self):
         if self.h:
             load().ilm_batch_destroy(self.h)
+            self.h = None
+
+
+class IrradianceCache:
+    """ICIntegrator's irradiance cache on the GPU (ilm_ic_*), over a Scene."""
+
+    def __init__(self, scene: "Scene", cache_depth=0, error_threshold=0.0, ambient_resolution=0.0,
+                 ambient_multiplier=0.0, azimuth_strata=0, altitude_strata=0, ray_depth=0, eps=0.0,
+                 indirect_factor=0.0, miss_radiance=(20.0, 50.0, 100.0), placement_stride=0):
+        p = IcParams(cache_depth, error_threshold, ambient_resolution, ambient_multiplier, azimuth_strata,
+                     altitude_strata, ray_depth, eps, indirect_factor, (C.c_float * 3)(*miss_radiance),
+                     placement_stride)
+        h = C.c_void_p()
+        check(load().ilm_ic_create(scene.h, C.byref(p), C.byref(h)))
+        self.h = h.value
+        self.ctx = scene.ctx
+
+    def render(self, cam: Camera, width, height, spp, accum: DeviceBuffer, seed=1234, spp_offset=0,
+               batch_paths=0) -> dict:
+        p = RenderParams(cam, width, height, spp, spp_offset, 1, 4, 1e-4, seed, batch_paths)
+        st = IcStats()
+        check(load().ilm_ic_render(self.h, C.byref(p), accum.ptr, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in st._fields_}
+
+    def insert(self, records):
+        """records: n x 10 float32 (P, N, E, Ri)."""
+        r = np.ascontiguousarray(records, np.float32).reshape(-1, 10)
+        check(load().ilm_ic_insert(self.h, _ptr(r), len(r)))
+
+    def info(self) -> dict:
+        nr, nn, nf = _i64(), _i64(), _i64()
+        rmin, rmax = C.c_float(), C.c_float()
+        root = np.zeros(6, np.float32)
+        check(load().ilm_ic_info(self.h, C.byref(nr), C.byref(nn), C.byref(nf), C.byref(rmin), C.byref(rmax),
+                                 _ptr(root)))
+        return {"records": nr.value, "nodes": nn.value, "refs": nf.value, "rmin": rmin.value, "rmax": rmax.value,
+                "root": root}
+
+    def records(self):
+        n = self.info()["records"]
+        out = np.zeros((n, 11), np.float32)
+        if n:
+            check(load().ilm_ic_records(self.h, _ptr(out)))
+        return out
+
+    def node(self, i):
+        cnt, child = C.c_int32(), C.c_int32()
+        check(load().ilm_ic_node_records(self.h, i, C.byref(cnt), C.byref(child), None, 0))
+        recs = np.zeros(max(1, cnt.value), np.int32)
+        check(load().ilm_ic_node_records(self.h, i, C.byref(cnt), C.byref(child), _ptr(recs), cnt.value))
+        return child.value, recs[:cnt.value]
+
+    def lookup(self, points):
+        """points: n x 6 (P, N) -> n x 4 (weighted mean irradiance, weight sum)."""
+        pts = np.ascontiguousarray(points, np.float32).reshape(-1, 6)
+        n = len(pts)
+        pd = self.ctx.to_device(pts)
+        od = self.ctx.buffer(max(16, n * 16))
+        check(load().ilm_ic_lookup(self.h, pd.ptr, n, od.ptr, None))
+        self.ctx.sync()
+        out = od.download(np.float32, 4 * n).reshape(n, 4)
+        pd.free()
+        od.free()
+        return out
+
+    def reset(self):
+        check(load().ilm_ic_reset(self.h))
+
+    def close(self):
+        if self.h:
+            load().ilm_ic_destroy(self.h)
             self.h = None
 
 

@@ -75,6 +75,25 @@ void camera_frame(const ilm_camera& c, float out[12]);
 // nodes when asked, uploads (device meshes) and fills m's info and host
 // copies, as every host builder ends. t0 = when the build started.
 int install_bvh(ilm_mesh* m, Bvh2& b, int node_format, std::chrono::steady_clock::time_point t0);
+// Variants of a frame of the wavefront path integrator used by the
+// irradiance cache (one view of width x height x spp paths, paths numbered
+// pixel-major): gather paths start from given rays; first_t and gbuf record
+// per path the first hit's distance and its shading point (3 float4: P and
+// kind 0 hit without material / 1 reflector / 2 emitter / 3 miss, Ns, albedo).
+struct PathMode {
+    const ilm_ray* gather = nullptr;
+    float* first_t = nullptr;
+    float* gbuf = nullptr;
+    int rr_rule = 0;    // 0 PathIntegrator's roulette, 1 ICIntegrator::PathLi's
+    bool emit0 = true;  // count emission at depth 0
+};
+ilm_ctx* scene_ctx(ilm_scene* s);
+ilm_mesh* scene_mesh(ilm_scene* s);
+// One frame of p (camera, size, spp, depth, roulette depth, eps, seed,
+// batch size) with the given mode; adds the paths' radiance sums per pixel
+// to accum (width*height*3 doubles, may be null only if nothing is accumulated).
+int render_paths(ilm_scene* s, const ilm_render_params& p, const PathMode& mode, double* accum,
+                 ilm_render_stats* stats);
 int trace_queue(ilm_mesh* m, const ilm_ray* rays, const unsigned long long* n_dev, int64_t capacity, ilm_hit* hits,
                 uint8_t* occ, hipStream_t stream, bool rooted = false, int refill = 0);
 // Builds m's BVH4 on its device (linear BVH + collapse); fills the device

@@ -165,6 +165,12 @@ struct PtArgs {
     float4* qout;            // cache predictions per query
     float* tq;               // training queries (train_cap x 16)
     float4* tt;              // training targets (train_cap x (rgb, weight))
+    // irradiance-cache passes
+    const ilm_ray* gather;   // non-null: path g starts from gather[g] instead of a camera ray
+    float* first_t;          // non-null: first-hit distance of path g (1 when its first ray finds no material)
+    float4* gbuf;            // non-null: first-hit record of path g (3 float4: P + kind, Ns, albedo)
+    int rr_rule;             // roulette: 0 q = min(0.5, luminance), 1 q = min(0.5, 0.33 (r + g + b))
+    int emit0;               // count emission at depth 0
 };
 
 // last view whose first path (or pixel) is <= g
@@ -202,6 +208,18 @@ __global__ void k_generate(PtArgs a) {
     path_pixel(a, p, vi, pixel, sample);
     const View& v = a.views[vi];
     Block8 b = sample_block((uint32_t)a.frame[0], (uint64_t)pixel, sample, 0, 0);
+    if (a.gather) {
+        // ICIntegrator::ComputeRecord's gather rays (ICIntegrator.cpp:512-519)
+        ilm_ray r = a.gather[a.bpath0 + p];
+        r.tmin = __uint_as_float(v.root);
+        a.rays[0][p] = r;
+        a.qpath[0][p] = (int32_t)p;
+        a.beta[p] = make_float4(1.f, 1.f, 1.f, 0.f);
+        a.L[3 * p] = 0.f;
+        a.L[3 * p + 1] = 0.f;
+        a.L[3 * p + 2] = 0.f;
+        return;
+    }
     const int64_t lp = pixel - v.pix0;
     int x = (int)(lp % v.W), y = (int)(lp / v.W);
     // PerspectiveCamera::GetRay (Camera/PerspectiveCamera.cpp:53-60) as fed by
@@ -277,6 +295,23 @@ __global__ void __launch_bounds__(kPtBlock) k_shade(PtArgs a, int bounce, int cu
         }
         Material mat;
         if (live) mat = load_material(a.mats, mat_id);
+        if (bounce == 0 && k < count) {
+            const int64_t g = a.bpath0 + p;
+            // ICIntegrator::PathLi keeps 1 as the first segment's length unless
+            // its first hit has a material (ICIntegrator.cpp:566-573)
+            if (a.first_t) a.first_t[g] = live ? h.t : 1.0f;
+            if (a.gbuf) {
+                // first-hit record for ICIntegrator::Radiance (ICIntegrator.cpp:380-407):
+                // kind 0 a hit without material, 1 reflector, 2 emitter, 3 miss;
+                // albedo = Rho (kd; kd + ks for the Phong lobe, which has no
+                // reference counterpart)
+                const float kind = h.prim < 0 ? 3.f : !live ? 0.f : (light_id >= 0 ? 2.f : 1.f);
+                f3 rho = live ? (mat.type == 1 ? mat.kd + mat.ks : mat.kd) : mk(0, 0, 0);
+                a.gbuf[3 * g] = make_float4(P.x, P.y, P.z, kind);
+                a.gbuf[3 * g + 1] = make_float4(Ns.x, Ns.y, Ns.z, 0.f);
+                a.gbuf[3 * g + 2] = make_float4(rho.x, rho.y, rho.z, 0.f);
+            }
+        }
         // radiance cache: at the query vertex a non-training path hands its
         // remaining radiance to the cache; a training path records the query and
         // keeps going, measuring that radiance from here on with tbeta
@@ -324,7 +359,7 @@ __global__ void __launch_bounds__(kPtBlock) k_shade(PtArgs a, int bounce, int cu
         const bool measure = train && bounce >= a.nrc_prefix && a.nrc_prefix >= 0;
         const bool last = bounce + 1 >= (train ? a.max_depth_train : a.max_depth);
         // emission at depth 0 (no specular BxDFs), DiffuseAreaLight::Radiance (DiffuseAreaLight.cpp:47-50)
-        if (live && bounce == 0 && light_id >= 0 && dot(Ns, wOut) > 0.f) {
+        if (live && a.emit0 && bounce == 0 && light_id >= 0 && dot(Ns, wOut) > 0.f) {
             float4 le = a.lights[light_id];
             atomicAdd(&a.L[3 * p], beta.x * le.x);
             atomicAdd(&a.L[3 * p + 1], beta.y * le.y);
@@ -403,7 +438,9 @@ __global__ void __launch_bounds__(kPtBlock) k_shade(PtArgs a, int bounce, int cu
                 if (measure) tbeta = mul(tbeta, wgt);
                 cont = true;
                 if (bounce >= a.rr_depth) {
-                    float q = fminf(0.5f, luminance(beta));
+                    // PathIntegrator: luminance; ICIntegrator::PathLi: 0.33 (r + g + b)
+                    // (PathIntegrator.cpp:164-171, ICIntegrator.cpp:626-632)
+                    float q = fminf(0.5f, a.rr_rule ? 0.33f * (beta.x + beta.y + beta.z) : luminance(beta));
                     if (b.v[5] > q) cont = false;
                     else {
                         beta = beta / q;
@@ -745,7 +782,8 @@ namespace {
 // views reference). Paths are numbered view by view, pixel-major, spp per
 // pixel; batches of at most batch_paths paths end at pixel boundaries.
 int render_views(ilm_scene* s, ilm_nrc* nrc, const std::vector<ilm::View>& views_in, const ilm_render_params& p,
-                 const ilm_nrc_render_params* np, double* accum, ilm_render_stats* stats) {
+                 const ilm_nrc_render_params* np, double* accum, ilm_render_stats* stats,
+                 const ilm::PathMode* mode = nullptr) {
     if (!accum && !(np && !np->accumulate)) return fail(ILM_ERR_INVALID, "null accumulator");
     if (views_in.empty() || p.max_depth <= 0) return fail(ILM_ERR_INVALID, "no views or max_depth not positive");
     const bool cache = nrc != nullptr;
@@ -833,6 +871,11 @@ int render_views(ilm_scene* s, ilm_nrc* nrc, const std::vector<ilm::View>& views
     a.eps = p.eps;
     a.accum = accum;
     a.frame = s->d_frame;
+    a.gather = mode ? mode->gather : nullptr;
+    a.first_t = mode ? mode->first_t : nullptr;
+    a.gbuf = mode ? reinterpret_cast<float4*>(mode->gbuf) : nullptr;
+    a.rr_rule = mode ? mode->rr_rule : 0;
+    a.emit0 = mode ? (mode->emit0 ? 1 : 0) : 1;
     hipStream_t st = ctx->stream;
     int ev0 = 0, ev1 = 0;
     rc = ilm_timer_mark(ctx, &ev0);
@@ -886,7 +929,7 @@ int render_views(ilm_scene* s, ilm_nrc* nrc, const std::vector<ilm::View>& views
         }
         return ILM_OK;
     };
-    if (!s->use_graph) {
+    if (!s->use_graph || mode) {
         rc = launch_frame();
         if (rc) return rc;
     } else {
@@ -973,6 +1016,20 @@ int render_impl(ilm_scene* s, ilm_nrc* nrc, const ilm_render_params* prm, const 
 }
 
 }  // namespace
+
+namespace ilm {
+
+ilm_ctx* scene_ctx(ilm_scene* s) { return s->ctx; }
+ilm_mesh* scene_mesh(ilm_scene* s) { return s->mesh; }
+
+int render_paths(ilm_scene* s, const ilm_render_params& p, const PathMode& mode, double* accum,
+                 ilm_render_stats* stats) {
+    if (!s) return fail(ILM_ERR_INVALID, "null scene");
+    std::vector<View> views(1, view_of(p, s->nlights));
+    return render_views(s, nullptr, views, p, nullptr, accum, stats, &mode);
+}
+
+}  // namespace ilm
 
 extern "C" {
 

@@ -485,6 +485,68 @@ typedef struct {
 int ilm_batch_autotune(ilm_batch* batch, const ilm_batch_view* views, int nviews, const ilm_render_params* params,
                        ilm_batch_tuning* out);
 
+/* ---------------------------------------------------------------- irradiance cache */
+
+/* The irradiance cache of ICIntegrator (Integrator/ICIntegrator.cpp:29-668)
+ * on the GPU, the comparator of the neural radiance cache: an octree of
+ * irradiance records (position, shading normal, irradiance, harmonic mean
+ * distance), Ward's weights for lookups, records gathered with stratified
+ * cosine-distributed PathLi paths. Zero fields take the ICIntegrator
+ * factory's defaults (Integrator/IntegratorFactories.h:159-183). */
+typedef struct {
+    int32_t cache_depth;       /* CacheDepth (16) */
+    float error_threshold;     /* ErrorThreshold (1) */
+    float ambient_resolution;  /* AmbientResolution (1e-3): RMin = longest scene edge * this */
+    float ambient_multiplier;  /* AmbientMultiplier (1e3): RMax = longest scene edge * this */
+    int32_t azimuth_strata;    /* AzimuthStrata (12) */
+    int32_t altitude_strata;   /* AltitudeStrata (4) */
+    int32_t ray_depth;         /* RayDepth of the gather paths (6) */
+    float eps;                 /* Epsilon (1e-4) */
+    float indirect_factor;     /* indirect = irradiance * albedo * this; 0 = 1/pi as the reference
+                                  (ICIntegrator.cpp:394); 1 makes it the radiance a Lambertian
+                                  surface reflects, consistent with the path integrator */
+    float miss_radiance[3];    /* camera rays that hit nothing (the reference returns (20, 50, 100),
+                                  ICIntegrator.cpp:405) */
+    int32_t placement_stride;  /* coarsest pixel stride of the record placement passes (16) */
+} ilm_ic_params;
+
+typedef struct {
+    int64_t records_created, records_total;
+    int64_t gather_rays;   /* PathLi paths traced for new records */
+    int64_t lookups;       /* samples shaded from the cache (not from their own record) */
+    double ms;             /* wall time of the call */
+} ilm_ic_stats;
+
+typedef struct ilm_ic ilm_ic;
+/* ICIntegrator::Initialise: the octree's root cell from the scene's bounds. */
+int ilm_ic_create(ilm_scene* scene, const ilm_ic_params* params, ilm_ic** out);
+int ilm_ic_destroy(ilm_ic* ic);
+/* Drops every record. */
+int ilm_ic_reset(ilm_ic* ic);
+/* One frame of ICIntegrator::Radiance for params (camera, size, spp,
+ * spp_offset, seed; max_depth is ignored): direct light at the first hit,
+ * new records where the cache does not cover a sample (placement passes at
+ * pixel strides placement_stride, ..., 2, 1, then every sample), indirect
+ * light from the cache. ADDS the samples' radiance sums per pixel to accum.
+ * The records stay for later frames. Synchronous. */
+int ilm_ic_render(ilm_ic* ic, const ilm_render_params* params, double* accum, ilm_ic_stats* stats);
+/* Inserts records given on the host: n x 10 floats (P, N, E, Ri); RiClamp
+ * is Ri clamped to [RMin, RMax] (IrradianceCache::Insert). */
+int ilm_ic_insert(ilm_ic* ic, const float* records, int64_t n);
+/* Record, octree node and node-reference counts, RMin, RMax and the root
+ * cell (6 floats: lo, hi). Any pointer may be NULL. */
+int ilm_ic_info(ilm_ic* ic, int64_t* records, int64_t* nodes, int64_t* refs, float* rmin, float* rmax,
+                float* root_lo_hi);
+/* Host copy of the records: n x 11 floats (P, N, E, Ri, RiClamp). */
+int ilm_ic_records(ilm_ic* ic, float* out);
+/* One octree node: its record count, first child (-1: none; children are 8
+ * consecutive nodes in SetBounds order) and up to cap record indices. */
+int ilm_ic_node_records(ilm_ic* ic, int64_t node, int32_t* count, int32_t* child, int32_t* records, int32_t cap);
+/* Cache lookups at n points (dev, 6 floats each: P, N): out (dev, 4 floats
+ * each) = Ward-weighted mean irradiance rgb and the weight sum (0: no
+ * record covers the point) (IrradianceCache::FindRecords, ICIntegrator.cpp:142-200). */
+int ilm_ic_lookup(ilm_ic* ic, const float* points, int64_t n, float* out, void* stream);
+
 /* ---------------------------------------------------------------- neural radiance cache */
 
 /* A small MLP (96 -> 5 x 64 ReLU -> 16, bf16 weights and activations, FP32

@@ -1,0 +1,91 @@
+"""The irradiance-cache restatement (oracle/icache.py) against what its
+definitions fix: a lookup finds exactly the records a brute-force scan over
+all records accepts (an accepted record lies within RiClamp of the point, so
+its sphere reaches the leaf holding the point, and Insert put it on that
+leaf's root path); every record sits only in cells its sphere overlaps and
+covers the part of its sphere inside the root; the strata of the cosine
+sampler tile the unit square and average to the cosine moments."""
+import numpy as np
+
+from oracle import icache as ic
+
+
+def _records(n, seed):
+    rng = np.random.default_rng(seed)
+    P = rng.random((n, 3)).astype(np.float32) * 2 - 1
+    N = rng.normal(size=(n, 3))
+    N = (N / np.linalg.norm(N, axis=1, keepdims=True)).astype(np.float32)
+    E = rng.random((n, 3)).astype(np.float32)
+    rc = (0.02 + 0.3 * rng.random(n) ** 2).astype(np.float32)
+    return P, N, E, rc
+
+
+def _tree(P, N, E, rc, depth=8):
+    lo, hi, _ = ic.root_cell(np.array([[-1, -1, -1], [1, 1, 1]], np.float32))
+    t = ic.Octree(lo, hi, depth)
+    for i in range(len(P)):
+        t.insert(P[i], N[i], E[i], rc[i])
+    return t
+
+
+def test_root_cell():
+    lo, hi, L = ic.root_cell(np.array([[0, 0, 0], [2, 1, 0.5]], np.float32))
+    assert L == 2
+    np.testing.assert_allclose(lo, [-1, -1.5, -1.75], rtol=1e-6)
+    np.testing.assert_allclose(hi, [3, 2.5, 2.25], rtol=1e-6)
+
+
+def test_lookup_equals_brute_force():
+    P, N, E, rc = _records(400, 1)
+    t = _tree(P, N, E, rc)
+    rng = np.random.default_rng(2)
+    for threshold in (1.0, 0.5, 0.2):
+        inv = np.float32(1) / np.float32(threshold)
+        for _ in range(300):
+            p = (rng.random(3) * 2 - 1).astype(np.float32)
+            k = int(rng.integers(len(P)))
+            n = N[k] if rng.random() < 0.5 else (rng.normal(size=3)).astype(np.float32)
+            n = (n / np.linalg.norm(n)).astype(np.float32)
+            _, den, ids = t.lookup(p, n, threshold)
+            brute = [r for r in range(len(P)) if t.weight(p, n, r, inv) > 0]
+            assert sorted(ids) == brute
+
+
+def test_records_cover_their_spheres():
+    P, N, E, rc = _records(120, 3)
+    t = _tree(P, N, E, rc, depth=6)
+    holders = {r: [] for r in range(len(P))}
+    for node, recs in enumerate(t.recs):
+        for r in recs:
+            holders[r].append(node)
+            assert ic.sphere_box(t.lo[node], t.hi[node], P[r], rc[r])
+    rng = np.random.default_rng(4)
+    for r in range(len(P)):
+        d = rng.normal(size=(50, 3))
+        pts = P[r] + (d / np.linalg.norm(d, axis=1, keepdims=True)) * rng.random((50, 1)) ** (1 / 3) * rc[r] * 0.999
+        for q in pts.astype(np.float32):
+            if not t._contains(t.lo[0], t.hi[0], q):
+                continue
+            assert any(t._contains(t.lo[n], t.hi[n], q) for n in holders[r])
+
+
+def test_stratified_cosine_directions():
+    alt_n, az_n = 4, 12
+    rng = np.random.default_rng(5)
+    u, v = rng.random(200000), rng.random(200000)
+    alt, az = rng.integers(alt_n, size=u.shape), rng.integers(az_n, size=u.shape)
+    d = ic.cosine_stratified(u, v, alt, az, alt_n, az_n)
+    np.testing.assert_allclose(np.linalg.norm(d, axis=1), 1, atol=1e-12)
+    # cosine-weighted hemisphere: E[cos] = 2/3, E[cos^2] = 1/2, azimuth uniform
+    assert abs(d[:, 2].mean() - 2 / 3) < 0.003 and abs((d[:, 2] ** 2).mean() - 0.5) < 0.003
+    assert abs(d[:, 0].mean()) < 0.005 and abs(d[:, 1].mean()) < 0.005
+    # each (altitude, azimuth) stratum maps to its own cell of the disk
+    a = 1 - d[:, 2] ** 2
+    assert np.array_equal(np.floor(a * alt_n + 1e-12).astype(int), alt)
+
+
+def test_record_formula():
+    E, ri, rc = ic.record([[1, 2, 3], [3, 2, 1]], [1.0, 3.0], 0.1, 1.0)
+    assert np.allclose(E, [2, 2, 2]) and abs(ri - 1.5) < 1e-12 and rc == 1.0
+    assert ic.record([[0, 0, 0]], [100.0], 0.1, 1000.0)[2] == 100.0
+    assert ic.record([[0, 0, 0]], [0.001], 0.1, 1000.0)[2] == 0.1
