@@ -246,6 +246,9 @@ __global__ void k_generate(PtArgs a) {
 }
 
 // One bounce of PathIntegrator::Radiance for every path in queue `cur`.
+// CACHE: the frame renders with the radiance cache (query vertices,
+// training paths); plain frames use the variant without that state.
+template <bool CACHE>
 __global__ void __launch_bounds__(kPtBlock) k_shade(PtArgs a, int bounce, int cur) {
     const int nxt = cur ^ 1;
     const int64_t count = (int64_t)a.counts[cur];
@@ -285,7 +288,7 @@ __global__ void __launch_bounds__(kPtBlock) k_shade(PtArgs a, int bounce, int cu
             wOut = -normalize(rd);
             float4 b4 = a.beta[p];
             beta = mk(b4.x, b4.y, b4.z);
-            train = b4.w != 0.f;
+            train = CACHE && b4.w != 0.f;
             if (train && bounce > a.nrc_prefix) {
                 float4 t4 = a.tbeta[p];
                 tbeta = mk(t4.x, t4.y, t4.z);
@@ -316,13 +319,13 @@ __global__ void __launch_bounds__(kPtBlock) k_shade(PtArgs a, int bounce, int cu
         // remaining radiance to the cache; a training path records the query and
         // keeps going, measuring that radiance from here on with tbeta
         bool to_cache = false, new_train = false;
-        if (live && a.nrc_prefix >= 0 && bounce == a.nrc_prefix) {
+        if (CACHE && live && a.nrc_prefix >= 0 && bounce == a.nrc_prefix) {
             to_cache = !train;
             new_train = train;
             if (train) tbeta = mk(1.f, 1.f, 1.f);
         }
         float qrec[16];
-        if (to_cache || new_train) {
+        if (CACHE && (to_cache || new_train)) {
             float ct = fminf(fmaxf(wOut.z, -1.f), 1.f), cn = fminf(fmaxf(Ns.z, -1.f), 1.f);
             qrec[0] = P.x; qrec[1] = P.y; qrec[2] = P.z;
             qrec[3] = acosf(ct) * kInvPi;
@@ -336,7 +339,7 @@ __global__ void __launch_bounds__(kPtBlock) k_shade(PtArgs a, int bounce, int cu
             qrec[13] = mat.type == 1 ? mat.ks.z : 0.f;
             qrec[14] = 0.f; qrec[15] = 0.f;
         }
-        {
+        if (CACHE) {
             int64_t qs = wave_append(&a.counts[5], to_cache);
             if (to_cache) {
                 float4* d = reinterpret_cast<float4*>(a.queries + 16 * qs);
@@ -356,7 +359,7 @@ __global__ void __launch_bounds__(kPtBlock) k_shade(PtArgs a, int bounce, int cu
                 a.T[3 * p + 2] = 0.f;
             }
         }
-        const bool measure = train && bounce >= a.nrc_prefix && a.nrc_prefix >= 0;
+        const bool measure = CACHE && train && bounce >= a.nrc_prefix && a.nrc_prefix >= 0;
         const bool last = bounce + 1 >= (train ? a.max_depth_train : a.max_depth);
         // emission at depth 0 (no specular BxDFs), DiffuseAreaLight::Radiance (DiffuseAreaLight.cpp:47-50)
         if (live && a.emit0 && bounce == 0 && light_id >= 0 && dot(Ns, wOut) > 0.f) {
@@ -419,7 +422,7 @@ __global__ void __launch_bounds__(kPtBlock) k_shade(PtArgs a, int bounce, int cu
             if (want) {
                 a.srays[slot] = sray;
                 a.scontrib[slot] = make_float4(contrib.x, contrib.y, contrib.z, __int_as_float(p));
-                if (a.nrc_prefix >= 0) a.scontrib2[slot] = make_float4(contrib2.x, contrib2.y, contrib2.z, 0.f);
+                if (CACHE && a.nrc_prefix >= 0) a.scontrib2[slot] = make_float4(contrib2.x, contrib2.y, contrib2.z, 0.f);
             }
         }
         // BSDF sampling in the shading frame (PathIntegrator.cpp:138-177)
@@ -906,7 +909,10 @@ int render_views(ilm_scene* s, ilm_nrc* nrc, const std::vector<ilm::View>& views
                 int r2 = ilm::trace_queue(s->mesh, a.rays[cur], a.counts + cur, cap, a.hits, nullptr, st, true,
                                           b == 0 ? 64 : 0);
                 if (r2) return r2;
-                hipLaunchKernelGGL(ilm::k_shade, dim3(grid), dim3(ilm::kPtBlock), 0, st, a, b, cur);
+                if (cache)
+                    hipLaunchKernelGGL(ilm::k_shade<true>, dim3(grid), dim3(ilm::kPtBlock), 0, st, a, b, cur);
+                else
+                    hipLaunchKernelGGL(ilm::k_shade<false>, dim3(grid), dim3(ilm::kPtBlock), 0, st, a, b, cur);
                 if (s->nlights > 0) {
                     r2 = ilm::trace_queue(s->mesh, a.srays, a.counts + 2, cap * max_vl, nullptr, a.socc, st, true);
                     if (r2) return r2;

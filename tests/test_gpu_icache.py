@@ -112,12 +112,14 @@ def test_records_and_image_in_furnace(gpu_ctx, ilm, factor):
     m = ilm.Mesh(gpu_ctx, arr[0], arr[2])
     m.build()
     sc = ilm.Scene(gpu_ctx, m, arr[1], *arr[3:])
+    # every pixel of the first pass makes a record (stride 1): a few hundred
+    # records of 512 gather paths each pin the record mean to ~0.2%
     ic = ilm.IrradianceCache(sc, ray_depth=depth, azimuth_strata=32, altitude_strata=16, error_threshold=0.3,
-                             indirect_factor=factor, placement_stride=4)
+                             indirect_factor=factor, placement_stride=1)
     cam = ilm.Camera.make((0.5, 0.5, 0.1), (0.5, 0.5, 1.0), (0, 1, 0), 8.0, 1.0)  # the quad fills the middle
     W = H = 32
     acc = gpu_ctx.buffer(W * H * 24).zero()
-    spp = 16
+    spp = 64
     st = ic.render(cam, W, H, spp, acc, seed=3)
     recs = ic.records()
     print(f"[ic furnace] {st} records {len(recs)}", flush=True)
@@ -125,7 +127,7 @@ def test_records_and_image_in_furnace(gpu_ctx, ilm, factor):
     want_e = le * sum(rho ** k for k in range(1, depth + 1))
     e = recs[:, 6:9].mean()
     print(f"[ic furnace] record E {e:.5f} vs {want_e:.5f}; per-record spread {recs[:, 6:9].std():.4f}", flush=True)
-    assert abs(e - want_e) < 0.02 * want_e
+    assert len(recs) > 200 and abs(e - want_e) < 0.01 * want_e
     # Ri: harmonic mean of the hit distances of cosine-distributed rays from
     # the quad's centre toward -z, by ray casting against the unit cube
     rng = np.random.default_rng(1)
@@ -143,11 +145,11 @@ def test_records_and_image_in_furnace(gpu_ctx, ilm, factor):
     f = factor if factor > 0 else 1 / np.pi
     want = rho_q * le + f * rho_q * want_e
     print(f"[ic furnace] quad pixels {centre:.5f} vs {want:.5f}", flush=True)
-    assert abs(centre - want) < 0.02 * want
+    assert abs(centre - want) < 0.015 * want
     # a second frame reuses the cache
     acc.zero()
     st2 = ic.render(cam, W, H, 1, acc, seed=4, spp_offset=1)
-    assert st2["records_created"] <= st["records_created"] // 4
+    assert st2["records_created"] <= st["records_created"] // 10
     ic.close()
     sc.close()
     m.close()
@@ -172,7 +174,7 @@ def test_room_image_converges_near_path_tracing(gpu_ctx, ilm):
     frames = 8
     for f in range(frames):
         ic.render(cam, W, H, 4, acc, seed=10 + f, spp_offset=4 * f)
-    img = acc.download(np.float64, W * H * 3).reshape(H, W, 3) / spp / (4 * frames)
+    img = acc.download(np.float64, W * H * 3).reshape(H, W, 3) / (4 * frames)
     e = w.rel_rmse(img, ref)
     mean_ratio = img.mean() / ref.mean()
     print(f"[ic room] relRMSE vs PT {e:.4f}, mean ratio {mean_ratio:.4f}, records {ic.info()['records']}", flush=True)
