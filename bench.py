@@ -413,7 +413,7 @@ def cache_convergence(ctx, ilm, workloads, args):
     while frames < args.cache_max_frames:
         st = sc.render_nrc(net, cam, W, H, 1, buf if frames >= skip else None, seed=20001 + frames,
                            prefix_depth=prefix, suffix_depth=depth - prefix, train_fraction=1 / 16,
-                           train_batch=65536, train_steps=1)
+                           train_batch=65536, train_steps=args.cache_train_steps)
         nrc_ms += st["ms"]
         frames += 1
         if frames > skip:
@@ -426,8 +426,38 @@ def cache_convergence(ctx, ilm, workloads, args):
                     break
     nrc_out = {"ms_per_frame": round(nrc_ms / frames, 3), "frames_to_target": nrc_hit[0] if nrc_hit else None,
                "ms_to_target": round(nrc_hit[1], 1) if nrc_hit else None, "skip_frames": skip,
-               "trace": ntrace}
+               "train_steps_per_frame": args.cache_train_steps, "trace": ntrace}
     log(f"cache nrc: {nrc_out}")
+
+    # Irradiance cache (the reference's comparator): 1-spp frames, the cache
+    # kept across frames; indirect factor 1 so that it converges to what the
+    # path integrator does. Stops at the target or at the plain renderer's
+    # time to it.
+    ic_out = None
+    if not args.no_ic:
+        ic = ilm.IrradianceCache(sc, ray_depth=depth - 1, error_threshold=args.ic_threshold, indirect_factor=1.0,
+                                 miss_radiance=(0.0, 0.0, 0.0))
+        buf.zero()
+        ic_ms, f, itrace, ic_hit = 0.0, 0, [], None
+        budget = plain_out["ms_to_target"] or plain_out["ms_fit"]
+        check = 1
+        while ic_ms < budget and f < args.cache_max_frames:
+            st = ic.render(cam, W, H, 1, buf, seed=30001 + f, spp_offset=f)
+            ic_ms += st["ms"]
+            f += 1
+            if f == check:
+                e, _ = _corrected(buf.download(np.float64, n3).reshape(H, W, 3) / f, ref, noise2)
+                itrace.append((f, round(ic_ms, 2), round(e, 5), st["records_total"]))
+                if e <= target:
+                    ic_hit = (f, ic_ms)
+                    break
+                check = f * 2 if f < 8 else f + max(1, f // 4)
+        ic_out = {"error_threshold": args.ic_threshold, "ms_per_frame": round(ic_ms / f, 3),
+                  "frames_to_target": ic_hit[0] if ic_hit else None,
+                  "ms_to_target": round(ic_hit[1], 1) if ic_hit else None,
+                  "best_rel_rmse": min(t[2] for t in itrace), "records": ic.info()["records"], "trace": itrace}
+        ic.close()
+        log(f"cache ic: {ic_out}")
 
     # MLP inference throughput on one frame's worth of queries.
     nq = W * H
@@ -462,7 +492,8 @@ def cache_convergence(ctx, ilm, workloads, args):
                         f"(cache after {prefix} bounces)",
             "target_rel_rmse": target, "reference_spp": 2 * args.cache_ref_spp,
             "reference_noise_rel_rmse": round(float(np.sqrt(noise2)), 5),
-            "plain_pt": plain_out, "neural_cache": nrc_out, "speedup_time_to_target": speedup, "mlp": mlp}
+            "plain_pt": plain_out, "neural_cache": nrc_out, "irradiance_cache": ic_out,
+            "speedup_time_to_target": speedup, "mlp": mlp}
 
 
 def main():
@@ -501,6 +532,9 @@ def main():
     ap.add_argument("--cache-max-frames", type=int, default=400)
     ap.add_argument("--cache-skip", type=int, default=16, help="neural-cache frames rendered before accumulating")
     ap.add_argument("--cache-lr", type=float, default=1e-2)
+    ap.add_argument("--cache-train-steps", type=int, default=1, help="Adam steps per neural-cache frame")
+    ap.add_argument("--no-ic", action="store_true", help="skip the irradiance-cache comparator")
+    ap.add_argument("--ic-threshold", type=float, default=0.3, help="irradiance-cache error threshold")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
