@@ -23,6 +23,8 @@ ILM_BUILDER_KDTREE = 2
 ILM_BUILDER_KDTREE_EX = 3
 ILM_BUILDER_LBVH = 4
 ILM_NODES_FLOAT, ILM_NODES_QUANTIZED = 0, 1
+ILM_IC_WARD = 0
+ILM_IC_TABELION = 1
 ILM_TONE_GLOBAL, ILM_TONE_DRAGO = 0, 1
 ILM_IMAGE_PFM, ILM_IMAGE_PPM = 0, 1
 ILM_SPAWN_SHADOW = 0
@@ -103,7 +105,7 @@ class IcParams(C.Structure):
     _fields_ = [("cache_depth", C.c_int32), ("error_threshold", C.c_float), ("ambient_resolution", C.c_float),
                 ("ambient_multiplier", C.c_float), ("azimuth_strata", C.c_int32), ("altitude_strata", C.c_int32),
                 ("ray_depth", C.c_int32), ("eps", C.c_float), ("indirect_factor", C.c_float),
-                ("miss_radiance", C.c_float * 3), ("placement_stride", C.c_int32)]
+                ("miss_radiance", C.c_float * 3), ("placement_stride", C.c_int32), ("variant", C.c_int32)]
 
 
 class IcStats(C.Structure):
@@ -465,10 +467,11 @@ class IrradianceCache:
 
     def __init__(self, scene: "Scene", cache_depth=0, error_threshold=0.0, ambient_resolution=0.0,
                  ambient_multiplier=0.0, azimuth_strata=0, altitude_strata=0, ray_depth=0, eps=0.0,
-                 indirect_factor=0.0, miss_radiance=(20.0, 50.0, 100.0), placement_stride=0):
+                 indirect_factor=0.0, miss_radiance=(20.0, 50.0, 100.0), placement_stride=0, variant=0):
+        """variant: ILM_IC_WARD (ICIntegrator) or ILM_IC_TABELION (MLICIntegrator)."""
         p = IcParams(cache_depth, error_threshold, ambient_resolution, ambient_multiplier, azimuth_strata,
                      altitude_strata, ray_depth, eps, indirect_factor, (C.c_float * 3)(*miss_radiance),
-                     placement_stride)
+                     placement_stride, variant)
         h = C.c_void_p()
         check(load().ilm_ic_create(scene.h, C.byref(p), C.byref(h)))
         self.h = h.value

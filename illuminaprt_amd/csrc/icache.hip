@@ -22,6 +22,10 @@
 //   - shading: direct lighting (next-event estimation) plus
 //     irradiance * albedo * 1/pi at the first hit, Le at emitters, a
 //     constant (20, 50, 100) for camera rays that hit nothing (Radiance, :348-441).
+// The MLICIntegrator variant as the reference compiles it (Tabelion weights,
+// Integrator/MLICIntegrator.cpp:28-34) differs in the weight
+// (1 - max(d / RiClamp, 9.0124 (1 - cos)) / threshold), Ri (the nearest gather
+// hit), the insertion test (half-size along x) and the gather origin (P).
 // Different on purpose: the reference fills the cache lazily, one pixel after
 // another, so a record serves the pixels traced after it. Here records are
 // placed in passes over the image (every 16th pixel, then every 8th, ... then
@@ -53,6 +57,7 @@ struct IcArgs {
     const int32_t* refs;
     const float4* recs;
     float inv_threshold;
+    int variant;  // ILM_IC_WARD or ILM_IC_TABELION
 };
 
 __device__ __forceinline__ bool contains(float4 lo, float4 hi, f3 p) {
@@ -76,8 +81,15 @@ __device__ float ic_lookup(const IcArgs& c, f3 p, f3 n, f3& e) {
             const float dx = p.x - pr.x, dy = p.y - pr.y, dz = p.z - pr.z;
             const float dist = sqrtf(dx * dx + dy * dy + dz * dz) / pr.w;
             const float cosv = n.x * nr.x + n.y * nr.y + n.z * nr.z;
-            if (cosv <= 0.f) continue;
-            const float w = 1.f / (dist + 1.f - cosv) - c.inv_threshold;
+            float w;
+            if (c.variant == ILM_IC_TABELION) {
+                // MLIrradianceCache::W_Tabelion (MLICIntegrator.cpp:285-318)
+                w = 1.f - c.inv_threshold * fmaxf(dist, (1.f - cosv) * 9.0124f);
+            } else {
+                // IrradianceCache::W_Ward (ICIntegrator.cpp:185-196)
+                if (cosv <= 0.f) continue;
+                w = 1.f / (dist + 1.f - cosv) - c.inv_threshold;
+            }
             if (w > 0.f) {
                 e = e + w * mk(er.x, er.y, er.z);
                 den += w;
@@ -119,6 +131,8 @@ __global__ void k_ic_probe(IcArgs c, const float4* gbuf, const int32_t* list, in
 // j = altitude * azimuth_strata + azimuth, in ComputeRecord's loop order.
 __global__ void k_ic_gather_rays(const float4* gbuf, const int32_t* list, int64_t nrec, int az_n, int alt_n,
                                  float eps, uint32_t seed, uint32_t pass, ilm_ray* out) {
+    // ICIntegrator starts gather rays at P + w eps; MLICIntegrator at P with
+    // tmin 2 eps (MLICIntegrator.cpp:697) -- pass eps = 0 for that
     const int mn = az_n * alt_n;
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nrec * mn) return;
@@ -141,19 +155,21 @@ __global__ void k_ic_gather_rays(const float4* gbuf, const int32_t* list, int64_
 
 // New records from their gather paths' radiance sums and first-hit distances.
 __global__ void k_ic_reduce(const float4* gbuf, const int32_t* list, int64_t nrec, int mn, const double* L,
-                            const float* first_t, float rmin, float rmax, float4* recs_out) {
+                            const float* first_t, float rmin, float rmax, int variant, float4* recs_out) {
     int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nrec) return;
     double e0 = 0, e1 = 0, e2 = 0;
-    float inv = 0.f;
+    float inv = 0.f, tmin = FLT_MAX;
     for (int j = 0; j < mn; ++j) {
         const int64_t k = r * mn + j;
         e0 += L[3 * k];
         e1 += L[3 * k + 1];
         e2 += L[3 * k + 2];
         inv += 1.f / first_t[k];
+        tmin = fminf(tmin, first_t[k]);
     }
-    const float ri = (float)mn / inv;
+    // Ward: harmonic mean distance; Tabelion: the nearest hit (MLICIntegrator.cpp:717-725)
+    const float ri = variant == ILM_IC_TABELION ? tmin : (float)mn / inv;
     const float rc = fmaxf(rmin, fminf(rmax, ri));
     const int64_t g = list[r];
     const float4 a = gbuf[3 * g], b = gbuf[3 * g + 1];
@@ -283,10 +299,13 @@ bool sphere_box(const float lo[3], const float hi[3], const float c[3], float r)
     return d <= r * r;
 }
 
-// IrradianceCache::Insert (ICIntegrator.cpp:107-140)
+// IrradianceCache::Insert (ICIntegrator.cpp:107-140); MLIrradianceCache::Insert
+// compares RiClamp with the half-size along x only (MLICIntegrator.cpp:172-176)
 void insert(ilm_ic* ic, int32_t node, int32_t rec, int depth) {
     const ilm_ic::Rec& r = ic->recs[rec];
-    if (depth <= 0 || r.rc > half_max(ic->nodes[node])) {
+    const ilm_ic::Node& nd = ic->nodes[node];
+    const float ext = ic->prm.variant == ILM_IC_TABELION ? (nd.hi[0] - nd.lo[0]) * 0.5f : half_max(nd);
+    if (depth <= 0 || r.rc > ext) {
         ic->nodes[node].recs.push_back(rec);
         return;
     }
@@ -353,6 +372,7 @@ ilm::IcArgs args_of(const ilm_ic* ic) {
     a.refs = ic->d_refs;
     a.recs = ic->d_recs;
     a.inv_threshold = 1.0f / ic->prm.error_threshold;
+    a.variant = ic->prm.variant;
     return a;
 }
 
@@ -384,6 +404,7 @@ int ilm_ic_create(ilm_scene* scene, const ilm_ic_params* params, ilm_ic** out) {
     if (!(p.eps > 0)) p.eps = 1e-4f;
     if (!(p.indirect_factor > 0)) p.indirect_factor = 1.0f / ilm::kPiF;
     if (p.placement_stride <= 0) p.placement_stride = 16;
+    if (p.variant != ILM_IC_WARD && p.variant != ILM_IC_TABELION) return fail(ILM_ERR_INVALID, "unknown cache variant");
     if (p.cache_depth > 40) return fail(ILM_ERR_INVALID, "cache depth above 40");
     ilm_mesh* m = ilm::scene_mesh(scene);
     if (!m || m->pos.empty()) return fail(ILM_ERR_STATE, "scene has no geometry");
@@ -590,8 +611,9 @@ int ilm_ic_render(ilm_ic* ic, const ilm_render_params* prm, double* accum, ilm_i
         if (!rc) {
             hipLaunchKernelGGL(ilm::k_ic_gather_rays, dim3(ilm::grid_of(nray)), dim3(256), 0, st,
                                reinterpret_cast<const float4*>(gbuf), (const int32_t*)slist, nrec,
-                               ic->prm.azimuth_strata, ic->prm.altitude_strata, ic->prm.eps, prm->seed,
-                               ic->passes, (ilm_ray*)rays);
+                               ic->prm.azimuth_strata, ic->prm.altitude_strata,
+                               ic->prm.variant == ILM_IC_TABELION ? 0.f : ic->prm.eps, prm->seed, ic->passes,
+                               (ilm_ray*)rays);
             // PathLi for every gather ray: one row of nray paths
             ilm_render_params pg = *prm;
             pg.width = (int32_t)nray;
@@ -614,7 +636,8 @@ int ilm_ic_render(ilm_ic* ic, const ilm_render_params* prm, double* accum, ilm_i
         if (!rc) {
             hipLaunchKernelGGL(ilm::k_ic_reduce, dim3(ilm::grid_of(nrec)), dim3(256), 0, st,
                                reinterpret_cast<const float4*>(gbuf), (const int32_t*)slist, nrec, mn,
-                               (const double*)L, (const float*)ft, ic->rmin, ic->rmax, (float4*)recs);
+                               (const double*)L, (const float*)ft, ic->rmin, ic->rmax, ic->prm.variant,
+                               (float4*)recs);
             // the sample that made a record takes that record's irradiance
             // (GetIrradiance returns it, ICIntegrator.cpp:487-493)
             hipLaunchKernelGGL(ilm::k_ic_own, dim3(ilm::grid_of(nrec)), dim3(256), 0, st, (const int32_t*)slist, nrec,

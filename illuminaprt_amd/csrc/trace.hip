@@ -103,9 +103,21 @@ __device__ __forceinline__ void node_hits(const float4* __restrict__ nodes, uint
     }
 }
 
+#ifndef ILM_STACK_DIST
+#define ILM_STACK_DIST 0
+#endif
+
 template <bool ANY, bool STATS, bool QUANT>
 __global__ void __launch_bounds__(kBlock) k_trace4(TraceArgs a) {
+#if ILM_STACK_DIST
+    // stack entries carry the child's entry distance, so that nodes beyond
+    // the closest hit found so far are dropped when popped
+    constexpr int kLdsE = kLds / 2;
+    __shared__ uint2 lds[kLdsE * kBlock];
+    float pd = 0.f;
+#else
     __shared__ uint32_t lds[kLds * kBlock];
+#endif
     const int tid = threadIdx.x;
     const int64_t gtid = (int64_t)blockIdx.x * kBlock + tid;
     const int64_t n = a.n_dev ? (int64_t)min((unsigned long long)a.n, *a.n_dev) : a.n;
@@ -120,7 +132,47 @@ __global__ void __launch_bounds__(kBlock) k_trace4(TraceArgs a) {
     int32_t best_prim = -1;
     int32_t n_nodes = 0, n_tris = 0;
 
-    auto push = [&](uint32_t x) {
+#if ILM_STACK_DIST
+    auto push = [&](uint32_t x, float d) {
+        if (lcnt == kLdsE) {
+            int pos = sp - kLdsE;
+            if (pos < a.spill_depth) {
+                uint2 e = lds[(sp % kLdsE) * kBlock + tid];
+                a.spill[(int64_t)(2 * pos) * a.grid_threads + gtid] = e.x;
+                a.spill[(int64_t)(2 * pos + 1) * a.grid_threads + gtid] = e.y;
+            } else {
+                atomicOr(&a.counters[2], 1ull);  // stack bound exceeded (never expected)
+                return;
+            }
+        } else {
+            ++lcnt;
+        }
+        lds[(sp % kLdsE) * kBlock + tid] = make_uint2(x, __float_as_uint(d));
+        ++sp;
+    };
+    auto pop_one = [&]() -> uint32_t {
+        if (sp == 0) return kNoNode;
+        --sp;
+        uint2 e;
+        if (lcnt > 0) {
+            --lcnt;
+            e = lds[(sp % kLdsE) * kBlock + tid];
+        } else {
+            e.x = a.spill[(int64_t)(2 * sp) * a.grid_threads + gtid];
+            e.y = a.spill[(int64_t)(2 * sp + 1) * a.grid_threads + gtid];
+        }
+        pd = __uint_as_float(e.y);
+        return e.x;
+    };
+    // closest hit: skip entries that start beyond the current hit window
+    auto pop = [&]() -> uint32_t {
+        uint32_t x = pop_one();
+        if (!ANY)
+            while (x != kNoNode && pd > r.thi * 1.00000048f) x = pop_one();
+        return x;
+    };
+#else
+    auto push = [&](uint32_t x, float) {
         if (lcnt == kLds) {
             int pos = sp - kLds;
             if (pos < a.spill_depth) {
@@ -144,6 +196,7 @@ __global__ void __launch_bounds__(kBlock) k_trace4(TraceArgs a) {
         }
         return a.spill[(int64_t)sp * a.grid_threads + gtid];
     };
+#endif
     auto finish = [&]() {
         if (ANY) {
             a.occ[ray_id] = best_prim >= 0 ? 1 : 0;
@@ -191,9 +244,9 @@ __global__ void __launch_bounds__(kBlock) k_trace4(TraceArgs a) {
             uint32_t c0, c1, c2, c3;
             node_hits<QUANT>(a.nodes, cur, r, d0, d1, d2, d3, c0, c1, c2, c3);
             ILM_CSWAP(0, 1) ILM_CSWAP(2, 3) ILM_CSWAP(0, 2) ILM_CSWAP(1, 3) ILM_CSWAP(1, 2)
-            if (d3 < INFINITY) push(c3);
-            if (d2 < INFINITY) push(c2);
-            if (d1 < INFINITY) push(c1);
+            if (d3 < INFINITY) push(c3, d3);
+            if (d2 < INFINITY) push(c2, d2);
+            if (d1 < INFINITY) push(c1, d1);
             if (d0 < INFINITY) {
                 cur = c0;
             } else {
@@ -271,7 +324,8 @@ static int launch_trace(ilm_mesh* m, const ilm_ray* rays, int64_t n, ilm_hit* hi
     a.rooted = rooted ? 1 : 0;
     a.refill_min = refill > 0 ? refill : ctx->trace_refill;
     a.spill_depth = std::max(1, m->info.stack_need + 1);
-    int rc = ensure_spill(ctx, (size_t)a.spill_depth * (size_t)a.grid_threads);
+    // words per spilled entry: node, and its entry distance when the stack keeps one
+    int rc = ensure_spill(ctx, (size_t)(ILM_STACK_DIST ? 2 : 1) * (size_t)a.spill_depth * (size_t)a.grid_threads);
     if (rc) return rc;
     a.spill = ctx->spill;
     a.counters = ctx->counters;

@@ -508,7 +508,13 @@ typedef struct {
     float miss_radiance[3];    /* camera rays that hit nothing (the reference returns (20, 50, 100),
                                   ICIntegrator.cpp:405) */
     int32_t placement_stride;  /* coarsest pixel stride of the record placement passes (16) */
+    int32_t variant;           /* ILM_IC_WARD: ICIntegrator; ILM_IC_TABELION: MLICIntegrator as the
+                                  reference compiles it (MLICIntegrator.cpp:28-34): Tabelion weights,
+                                  Ri = nearest gather hit, insertion by the half-size along x,
+                                  gather rays from P; its ComputeRecord/PathLi are otherwise ICIntegrator's */
 } ilm_ic_params;
+#define ILM_IC_WARD 0
+#define ILM_IC_TABELION 1
 
 typedef struct {
     int64_t records_created, records_total;

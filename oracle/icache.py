@@ -12,6 +12,10 @@ Follows Integrator/ICIntegrator.cpp:
   - FindRecords (:142-183) with W_Ward (:185-196): records with w > 0 on the
     root-to-leaf path of the point, averaged by weight (GetIrradiance,
     :478-486);
+  - MLICIntegrator as the reference compiles it (Tabelion weights,
+    MLICIntegrator.cpp:28-34): w = 1 - max(d / RiClamp, 9.0124 (1 - cos)) /
+    threshold (W_Tabelion, :285-318), insertion by the half-size along x
+    (Insert, :172-176), Ri = the nearest gather hit (ComputeRecord, :717-725);
   - ComputeRecord's formulas (:498-543): stratified cosine directions
     (Montecarlo::CosineSampleHemisphere with strata, Maths/Montecarlo.h:20-29),
     E = mean, Ri = mn / sum(1 / distance), RiClamp = clamp(Ri, RMin, RMax).
@@ -57,12 +61,15 @@ def sphere_box(lo, hi, c, r):
 
 
 class Octree:
-    def __init__(self, lo, hi, depth):
+    def __init__(self, lo, hi, depth, variant="ward"):
+        self.variant = variant
         self.lo, self.hi, self.child, self.recs = [np.asarray(lo, np.float32)], [np.asarray(hi, np.float32)], [-1], [[]]
         self.depth = depth
         self.P, self.N, self.E, self.rc = [], [], [], []
 
     def half(self, n):
+        if self.variant == "tabelion":
+            return F((self.hi[n][0] - self.lo[n][0]) * F(0.5))
         return F(np.max(np.abs((self.hi[n] - self.lo[n]) * F(0.5))))
 
     def insert(self, P, N, E, rc):
@@ -99,6 +106,8 @@ class Octree:
         d = self.P[r] - p
         dist = F(np.sqrt(F(F(d[0] * d[0]) + F(d[1] * d[1]) + F(d[2] * d[2])))) / self.rc[r]
         cos = F(F(n[0] * self.N[r][0]) + F(n[1] * self.N[r][1]) + F(n[2] * self.N[r][2]))
+        if self.variant == "tabelion":
+            return F(1) - F(inv_threshold) * max(dist, F(F(F(1) - cos) * F(9.0124)))
         if cos <= 0:
             return F(-1)
         return F(F(1) / F(F(dist + F(1)) - cos)) - F(inv_threshold)
@@ -137,9 +146,11 @@ def cosine_stratified(u, v, alt, az, alt_n, az_n):
     return np.stack([d * np.cos(c), d * np.sin(c), np.sqrt(np.maximum(0.0, 1 - a))], axis=-1)
 
 
-def record(radiance, first_t, rmin, rmax):
-    """ComputeRecord: E = mean radiance, Ri = mn / sum(1 / t), clamped radius."""
+def record(radiance, first_t, rmin, rmax, variant="ward"):
+    """ComputeRecord: E = mean radiance, Ri = mn / sum(1 / t) (Ward) or the
+    nearest hit (Tabelion), clamped radius."""
     radiance = np.asarray(radiance, np.float64).reshape(-1, 3)
     mn = len(radiance)
-    ri = mn / np.sum(1.0 / np.asarray(first_t, np.float64))
+    t = np.asarray(first_t, np.float64)
+    ri = float(t.min()) if variant == "tabelion" else mn / np.sum(1.0 / t)
     return radiance.mean(axis=0), ri, min(max(ri, rmin), rmax)

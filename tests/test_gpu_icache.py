@@ -32,9 +32,11 @@ def _room(ilm, ctx):
     return m, ilm.Scene(ctx, m, nrm, tm, tl, mats, lights), pos
 
 
-def test_octree_and_lookups_match_restatement(gpu_ctx, ilm):
+@pytest.mark.parametrize("variant", [0, 1])
+def test_octree_and_lookups_match_restatement(gpu_ctx, ilm, variant):
     m, sc, pos = _room(ilm, gpu_ctx)
-    ic = ilm.IrradianceCache(sc, cache_depth=8, error_threshold=0.5, ambient_resolution=0.01, ambient_multiplier=0.3)
+    ic = ilm.IrradianceCache(sc, cache_depth=8, error_threshold=0.5, ambient_resolution=0.01, ambient_multiplier=0.3,
+                             variant=variant)
     info = ic.info()
     lo, hi, L = oic.root_cell(pos)
     assert np.array_equal(info["root"], np.concatenate([lo, hi]))
@@ -47,7 +49,7 @@ def test_octree_and_lookups_match_restatement(gpu_ctx, ilm):
     E = rng.random((n, 3)).astype(np.float32)
     Ri = (0.005 + 0.4 * rng.random(n) ** 3).astype(np.float32)
     ic.insert(np.concatenate([P, N, E, Ri[:, None]], axis=1))
-    t = oic.Octree(lo, hi, 8)
+    t = oic.Octree(lo, hi, 8, ["ward", "tabelion"][variant])
     rc = np.clip(Ri, info["rmin"], info["rmax"]).astype(np.float32)
     for i in range(n):
         t.insert(P[i], N[i], E[i], rc[i])
@@ -156,7 +158,8 @@ def test_records_and_image_in_furnace(gpu_ctx, ilm, factor):
     acc.free()
 
 
-def test_room_image_converges_near_path_tracing(gpu_ctx, ilm):
+@pytest.mark.parametrize("variant", [0, 1])
+def test_room_image_converges_near_path_tracing(gpu_ctx, ilm, variant):
     """With the consistent indirect factor and a tight error threshold, the
     cache's image of the box room (closed scene front: camera rays that miss
     are excluded) stays within a modest relRMSE of a converged path-traced
@@ -169,7 +172,7 @@ def test_room_image_converges_near_path_tracing(gpu_ctx, ilm):
     sc.render(cam, W, H, 1024, acc, seed=9, max_depth=6)
     ref = acc.download(np.float64, W * H * 3).reshape(H, W, 3) / 1024
     ic = ilm.IrradianceCache(sc, ray_depth=5, error_threshold=0.25, indirect_factor=1.0,
-                             miss_radiance=(0.0, 0.0, 0.0))
+                             miss_radiance=(0.0, 0.0, 0.0), variant=variant)
     acc.zero()
     frames = 8
     for f in range(frames):
@@ -177,7 +180,8 @@ def test_room_image_converges_near_path_tracing(gpu_ctx, ilm):
     img = acc.download(np.float64, W * H * 3).reshape(H, W, 3) / (4 * frames)
     e = w.rel_rmse(img, ref)
     mean_ratio = img.mean() / ref.mean()
-    print(f"[ic room] relRMSE vs PT {e:.4f}, mean ratio {mean_ratio:.4f}, records {ic.info()['records']}", flush=True)
+    print(f"[ic room v{variant}] relRMSE vs PT {e:.4f}, mean ratio {mean_ratio:.4f}, records {ic.info()['records']}",
+          flush=True)
     assert e < 0.2 and abs(mean_ratio - 1) < 0.05
     ic.close()
     sc.close()

@@ -6,6 +6,7 @@ leaf's root path); every record sits only in cells its sphere overlaps and
 covers the part of its sphere inside the root; the strata of the cosine
 sampler tile the unit square and average to the cosine moments."""
 import numpy as np
+import pytest
 
 from oracle import icache as ic
 
@@ -20,9 +21,9 @@ def _records(n, seed):
     return P, N, E, rc
 
 
-def _tree(P, N, E, rc, depth=8):
+def _tree(P, N, E, rc, depth=8, variant="ward"):
     lo, hi, _ = ic.root_cell(np.array([[-1, -1, -1], [1, 1, 1]], np.float32))
-    t = ic.Octree(lo, hi, depth)
+    t = ic.Octree(lo, hi, depth, variant)
     for i in range(len(P)):
         t.insert(P[i], N[i], E[i], rc[i])
     return t
@@ -35,9 +36,10 @@ def test_root_cell():
     np.testing.assert_allclose(hi, [3, 2.5, 2.25], rtol=1e-6)
 
 
-def test_lookup_equals_brute_force():
+@pytest.mark.parametrize("variant", ["ward", "tabelion"])
+def test_lookup_equals_brute_force(variant):
     P, N, E, rc = _records(400, 1)
-    t = _tree(P, N, E, rc)
+    t = _tree(P, N, E, rc, variant=variant)
     rng = np.random.default_rng(2)
     for threshold in (1.0, 0.5, 0.2):
         inv = np.float32(1) / np.float32(threshold)
@@ -51,9 +53,10 @@ def test_lookup_equals_brute_force():
             assert sorted(ids) == brute
 
 
-def test_records_cover_their_spheres():
+@pytest.mark.parametrize("variant", ["ward", "tabelion"])
+def test_records_cover_their_spheres(variant):
     P, N, E, rc = _records(120, 3)
-    t = _tree(P, N, E, rc, depth=6)
+    t = _tree(P, N, E, rc, depth=6, variant=variant)
     holders = {r: [] for r in range(len(P))}
     for node, recs in enumerate(t.recs):
         for r in recs:
@@ -84,8 +87,24 @@ def test_stratified_cosine_directions():
     assert np.array_equal(np.floor(a * alt_n + 1e-12).astype(int), alt)
 
 
+def test_tabelion_weight_closed_forms():
+    lo, hi, _ = ic.root_cell(np.array([[-1, -1, -1], [1, 1, 1]], np.float32))
+    t = ic.Octree(lo, hi, 4, "tabelion")
+    t.insert([0, 0, 0], [0, 0, 1], [1, 1, 1], 0.5)
+    # (the last argument is 1 / threshold) at the record: w = 1; at half the
+    # radius along the plane: 1 - 0.5 / threshold; tilted normals: the angle
+    # term 9.0124 (1 - cos)
+    assert t.weight(np.float32([0, 0, 0]), np.float32([0, 0, 1]), 0, 1.0) == 1.0
+    assert abs(t.weight(np.float32([0.25, 0, 0]), np.float32([0, 0, 1]), 0, 1.0) - 0.5) < 1e-6
+    c = np.cos(0.2)
+    n = np.float32([np.sin(0.2), 0, c])
+    assert abs(t.weight(np.float32([0, 0, 0]), n, 0, 0.5) - (1 - 0.5 * 9.0124 * (1 - c))) < 1e-5
+    assert t.weight(np.float32([0, 0, 0]), np.float32([0, 0, -1]), 0, 1.0) < 0
+
+
 def test_record_formula():
     E, ri, rc = ic.record([[1, 2, 3], [3, 2, 1]], [1.0, 3.0], 0.1, 1.0)
     assert np.allclose(E, [2, 2, 2]) and abs(ri - 1.5) < 1e-12 and rc == 1.0
     assert ic.record([[0, 0, 0]], [100.0], 0.1, 1000.0)[2] == 100.0
     assert ic.record([[0, 0, 0]], [0.001], 0.1, 1000.0)[2] == 0.1
+    assert ic.record([[0, 0, 0], [1, 1, 1]], [2.0, 0.5], 0.1, 10.0, "tabelion")[1] == 0.5
