@@ -314,8 +314,8 @@ def batch_leg(ctx, ilm, workloads, args):
     out = {"workload": f"{len(views)} box rooms, {tris} tris (1k-2M log-uniform per room), {pix} pixels, "
                        f"{args.batch_spp} spp per frame, depth 8",
            "triangles": tris, "pixels": pix, "device_build_ms": round(build_ms, 1),
-           "autotune": {"node_format": tune["node_format"], "batch_paths": tune["batch_paths"],
-                        "table_ms": [[f, bp, round(ms, 2)] for f, bp, ms in tune["table"]]}}
+           "autotune": {"node_format": tune["node_format"], "width": tune["width"], "batch_paths": tune["batch_paths"],
+                        "table_ms": [[f, wd, bp, round(ms, 2)] for f, wd, bp, ms in tune["table"]]}}
     for graph in (False, True):
         fr = [f for g, f in frames if g == graph]
         ms = float(np.mean([f["ms"] for f in fr]))
@@ -520,6 +520,7 @@ def main():
     ap.add_argument("--batch-spp", type=int, default=2)
     ap.add_argument("--nodes", choices=["float", "quantized"], default="float",
                     help="BVH4 node format of the traced trees")
+    ap.add_argument("--width", type=int, choices=[2, 4, 8], default=4, help="BVH width (float nodes)")
     ap.add_argument("--cache-only", action="store_true", help="run only the time-to-relRMSE leg")
     ap.add_argument("--frames-only", action="store_true",
                     help="run only a few plain and cache frames of the cache workload (profiling)")
@@ -530,9 +531,9 @@ def main():
     ap.add_argument("--cache-target", type=float, default=0.05)
     ap.add_argument("--cache-max-spp", type=int, default=1024)
     ap.add_argument("--cache-max-frames", type=int, default=400)
-    ap.add_argument("--cache-skip", type=int, default=16, help="neural-cache frames rendered before accumulating")
+    ap.add_argument("--cache-skip", type=int, default=4, help="neural-cache frames rendered before accumulating")
     ap.add_argument("--cache-lr", type=float, default=1e-2)
-    ap.add_argument("--cache-train-steps", type=int, default=1, help="Adam steps per neural-cache frame")
+    ap.add_argument("--cache-train-steps", type=int, default=4, help="Adam steps per neural-cache frame")
     ap.add_argument("--no-ic", action="store_true", help="skip the irradiance-cache comparator")
     ap.add_argument("--ic-threshold", type=float, default=0.3, help="irradiance-cache error threshold")
     args = ap.parse_args()
@@ -575,7 +576,7 @@ def main():
     nodes_fmt = ilm.ILM_NODES_QUANTIZED if args.nodes == "quantized" else ilm.ILM_NODES_FLOAT
     pos, idx = workloads.intersection_scene()
     mesh = ilm.Mesh(ctx, pos, idx)
-    info = mesh.build(node_format=nodes_fmt)
+    info = mesh.build(node_format=nodes_fmt, width=args.width)
     rs = workloads.intersection_rays(ctx, mesh)
     n = rs.n
     log(f"setup {time.perf_counter() - t0:.1f} s: {len(idx)} tris, bvh {info}")

@@ -81,14 +81,16 @@ class BatchView(C.Structure):
 
 
 class BatchTuning(C.Structure):
-    _fields_ = [("node_format", C.c_int32), ("batch_paths", C.c_int64), ("best_ms", C.c_double),
-                ("candidates", C.c_int32), ("candidate_format", C.c_int32 * 8), ("candidate_batch", C.c_int64 * 8),
-                ("candidate_ms", C.c_double * 8)]
+    _fields_ = [("node_format", C.c_int32), ("width", C.c_int32), ("batch_paths", C.c_int64), ("best_ms", C.c_double),
+                ("candidates", C.c_int32), ("candidate_format", C.c_int32 * 16), ("candidate_width", C.c_int32 * 16),
+                ("candidate_batch", C.c_int64 * 16), ("candidate_ms", C.c_double * 16)]
 
     def as_dict(self):
         n = self.candidates
-        return {"node_format": self.node_format, "batch_paths": self.batch_paths, "best_ms": self.best_ms,
-                "table": [(self.candidate_format[i], self.candidate_batch[i], self.candidate_ms[i]) for i in range(n)]}
+        return {"node_format": self.node_format, "width": self.width, "batch_paths": self.batch_paths,
+                "best_ms": self.best_ms,
+                "table": [(self.candidate_format[i], self.candidate_width[i], self.candidate_batch[i],
+                           self.candidate_ms[i]) for i in range(n)]}
 
 
 class NrcParams(C.Structure):
@@ -203,7 +205,7 @@ SIGNATURES = {
     "ilm_render_batch": (C.c_int, [_vp, C.POINTER(BatchView), C.c_int, C.POINTER(RenderParams), _vp,
                                    C.POINTER(RenderStats)]),
     "ilm_batch_set_graph": (C.c_int, [_vp, C.c_int]),
-    "ilm_batch_configure": (C.c_int, [_vp, C.c_int, _i64]),
+    "ilm_batch_configure": (C.c_int, [_vp, C.c_int, C.c_int, _i64]),
     "ilm_batch_autotune": (C.c_int, [_vp, C.POINTER(BatchView), C.c_int, C.POINTER(RenderParams),
                                      C.POINTER(BatchTuning)]),
     "ilm_render_nrc": (C.c_int, [_vp, _vp, C.POINTER(RenderParams), C.POINTER(NrcRenderParams), _vp,
@@ -450,8 +452,10 @@ class Batch:
         check(load().ilm_batch_autotune(self.h, arr, len(views), C.byref(p), C.byref(t)))
         return t.as_dict()
 
-    def configure(self, node_format=ILM_NODES_FLOAT, batch_paths=0):
-        check(load().ilm_batch_configure(self.h, node_format, batch_paths))
+    def configure(self, node_format=ILM_NODES_FLOAT, batch_paths=0, width=4):
+        """Node format and BVH width (2, 4, 8; 2 and 8 with float nodes) of
+        later frames, and paths per wavefront batch (0 = default)."""
+        check(load().ilm_batch_configure(self.h, node_format, width, batch_paths))
 
     def set_graph(self, enable: bool):
         check(load().ilm_batch_set_graph(self.h, int(enable)))
@@ -795,8 +799,15 @@ def decode_bvh4(nodes_bytes: np.ndarray, node_format: int = 0):
         lo = p[:, None, :] + np.stack([q[:, 0], q[:, 2], q[:, 4]], axis=-1) * step[:, None, :]
         hi = p[:, None, :] + np.stack([q[:, 1], q[:, 3], q[:, 5]], axis=-1) * step[:, None, :]
         return np.concatenate([lo, hi], axis=-1), child
-    raw = nodes_bytes.view(np.uint8).reshape(-1, 112)
-    f = raw[:, :96].copy().view(np.float32).reshape(-1, 6, 4)   # lox,hix,loy,hiy,loz,hiz
-    child = raw[:, 96:].copy().view(np.uint32).reshape(-1, 4)
-    boxes = np.stack([f[:, 0], f[:, 2], f[:, 4], f[:, 1], f[:, 3], f[:, 5]], axis=-1)  # (n,4,6) lo xyz, hi xyz
+    return decode_bvh(nodes_bytes, 4)
+
+
+def decode_bvh(nodes_bytes: np.ndarray, width: int):
+    """Float nodes of width 2, 4 or 8 (W lox, W hix, W loy, W hiy, W loz, W hiz,
+    W child words; BVH2 nodes padded to 64 bytes) -> boxes (n,W,6), child words (n,W)."""
+    stride = {2: 64, 4: 112, 8: 224}[width]
+    raw = nodes_bytes.view(np.uint8).reshape(-1, stride)
+    f = raw[:, :24 * width].copy().view(np.float32).reshape(-1, 6, width)
+    child = raw[:, 24 * width:28 * width].copy().view(np.uint32).reshape(-1, width)
+    boxes = np.stack([f[:, 0], f[:, 2], f[:, 4], f[:, 1], f[:, 3], f[:, 5]], axis=-1)  # (n,W,6) lo xyz, hi xyz
     return boxes, child

@@ -235,10 +235,11 @@ int ilm_mesh_destroy(ilm_mesh* m) {
 
 namespace ilm {
 
-int install_bvh(ilm_mesh* m, Bvh2& b, int node_format, std::chrono::steady_clock::time_point t0) {
+int install_bvh(ilm_mesh* m, Bvh2& b, int node_format, std::chrono::steady_clock::time_point t0, int width) {
     WideBvh w;
+    if (node_format == ILM_NODES_QUANTIZED && width != 4) return fail(ILM_ERR_INVALID, "quantised nodes are BVH4");
     try {
-        w = collapse(b, 4);
+        w = collapse(b, width);
     } catch (const std::exception& e) {
         return fail(ILM_ERR_INVALID, std::string("BVH build failed: ") + e.what());
     }
@@ -253,7 +254,10 @@ int install_bvh(ilm_mesh* m, Bvh2& b, int node_format, std::chrono::steady_clock
     }
     m->node_format = node_format;
     m->info.node_format = node_format;
-    m->info.node_stride = node_format == ILM_NODES_QUANTIZED ? (int32_t)sizeof(QNode4) : (int32_t)sizeof(WideNode<4>);
+    m->info.node_stride = node_format == ILM_NODES_QUANTIZED ? (int32_t)sizeof(QNode4)
+                          : width == 2                      ? (int32_t)sizeof(WideNode<2>)
+                          : width == 8                      ? (int32_t)sizeof(WideNode<8>)
+                                                            : (int32_t)sizeof(WideNode<4>);
     auto t1 = std::chrono::steady_clock::now();
     if (m->ctx) {  // a host-only mesh keeps the flattened tree for export only
         if (m->d_nodes) { ilm_free(m->ctx, m->d_nodes); m->d_nodes = nullptr; }
@@ -293,10 +297,13 @@ int ilm_bvh_build(ilm_mesh* m, const ilm_bvh_params* params) {
     ilm_bvh_params p{};
     if (params) p = *params;
     if (p.width == 0) p.width = 4;
-    if (p.width != 4) return fail(ILM_ERR_INVALID, "this build traverses BVH4 only");
+    if (p.width != 2 && p.width != 4 && p.width != 8) return fail(ILM_ERR_INVALID, "BVH width must be 2, 4 or 8");
     if (p.node_format != ILM_NODES_FLOAT && p.node_format != ILM_NODES_QUANTIZED)
         return fail(ILM_ERR_INVALID, "unknown node format");
+    if (p.node_format == ILM_NODES_QUANTIZED && p.width != 4)
+        return fail(ILM_ERR_INVALID, "quantised nodes are BVH4");
     if (p.builder == ILM_BUILDER_LBVH) {
+        if (p.width != 4) return fail(ILM_ERR_INVALID, "the device builder makes BVH4 trees");
         if (!m->ctx) return fail(ILM_ERR_STATE, "the GPU builder needs a device context");
         return ilm::build_lbvh(m, p.max_leaf > 0 ? p.max_leaf : 4, p.node_format);
     }
@@ -322,7 +329,7 @@ int ilm_bvh_build(ilm_mesh* m, const ilm_bvh_params* params) {
     } catch (const std::exception& e) {
         return fail(ILM_ERR_INVALID, std::string("BVH build failed: ") + e.what());
     }
-    return ilm::install_bvh(m, b, p.node_format, t0);
+    return ilm::install_bvh(m, b, p.node_format, t0, p.width);
 }
 
 int ilm_bvh_get_info(ilm_mesh* m, ilm_bvh_info* info) {

@@ -558,6 +558,93 @@ struct Collapser {
 
 }  // namespace
 
+WideBvh collapse_from(const Bvh2& b, int32_t root, int width) {
+    if (width != 2 && width != 4 && width != 8) throw std::invalid_argument("width must be 2, 4 or 8");
+    WideBvh w;
+    w.width = width;
+    Collapser c(b, width, w.nodes);
+    const Bvh2Node& r = b.nodes[root];
+    c.root_area = std::max(Collapser::area(r.lo, r.hi), 1e-30f);
+    int need = 0;
+    if (r.leaf()) {
+        int64_t ni = c.alloc();
+        int cn = 0;
+        uint32_t slot = c.leaf_slot(r.first, r.count, r.lo, r.hi, 1, cn);
+        c.set_slot(ni, 0, r.lo, r.hi, slot);
+        for (int k = 1; k < width; ++k) c.set_empty(ni, k);
+        need = cn;
+    } else {
+        c.emit(root, 0, need);
+    }
+    w.nnodes = c.count;
+    w.depth = c.max_depth + 1;
+    w.max_leaf = c.max_leaf;
+    w.stack_need = need + 1;
+    w.sah_cost = c.cost_sum;
+    return w;
+}
+
+namespace {
+
+struct Item { float lo[3], hi[3]; uint32_t word; };
+
+struct Unwider {
+    const WideNode<4>* n;
+    Bvh2& b;
+    int32_t node_of(uint32_t ni, int depth) {
+        if (depth > 128) throw std::invalid_argument("BVH4 deeper than 128 levels");
+        Item it[4];
+        int k = 0;
+        for (int s = 0; s < 4; ++s) {
+            const uint32_t w = n[ni].child[s];
+            if (w == kEmptySlot) continue;
+            it[k].lo[0] = n[ni].lox[s]; it[k].hi[0] = n[ni].hix[s];
+            it[k].lo[1] = n[ni].loy[s]; it[k].hi[1] = n[ni].hiy[s];
+            it[k].lo[2] = n[ni].loz[s]; it[k].hi[2] = n[ni].hiz[s];
+            it[k].word = w;
+            ++k;
+        }
+        if (k == 0) throw std::invalid_argument("BVH4 node without children");
+        return items(it, k, depth);
+    }
+    int32_t items(const Item* it, int k, int depth) {
+        if (k == 1) {
+            if (is_leaf(it[0].word)) {
+                int32_t i = (int32_t)b.nodes.size();
+                b.nodes.emplace_back();
+                Bvh2Node& nd = b.nodes[i];
+                for (int a = 0; a < 3; ++a) { nd.lo[a] = it[0].lo[a]; nd.hi[a] = it[0].hi[a]; }
+                nd.first = (int32_t)leaf_first(it[0].word);
+                nd.count = (int32_t)leaf_count(it[0].word);
+                return i;
+            }
+            return node_of(it[0].word, depth + 1);
+        }
+        const int h = k / 2;
+        int32_t l = items(it, h, depth + 1), r = items(it + h, k - h, depth + 1);
+        int32_t i = (int32_t)b.nodes.size();
+        b.nodes.emplace_back();
+        Bvh2Node& nd = b.nodes[i];
+        for (int a = 0; a < 3; ++a) {
+            nd.lo[a] = std::min(b.nodes[l].lo[a], b.nodes[r].lo[a]);
+            nd.hi[a] = std::max(b.nodes[l].hi[a], b.nodes[r].hi[a]);
+        }
+        nd.left = l;
+        nd.right = r;
+        return i;
+    }
+};
+
+}  // namespace
+
+Bvh2 binary_from_bvh4(const uint8_t* nodes, const std::vector<uint32_t>& roots, std::vector<int32_t>& bin_roots) {
+    Bvh2 b;
+    Unwider u{reinterpret_cast<const WideNode<4>*>(nodes), b};
+    bin_roots.clear();
+    for (uint32_t r : roots) bin_roots.push_back(u.node_of(r, 0));
+    return b;
+}
+
 WideBvh collapse(const Bvh2& b, int width) {
     if (width != 2 && width != 4 && width != 8) throw std::invalid_argument("width must be 2, 4 or 8");
     if (b.order.size() > (size_t)(1 << 26) - 1) throw std::invalid_argument("more than 2^26 - 1 triangle slots");

@@ -61,6 +61,16 @@ struct WideBvh {
 // its largest-area internal child until it holds `width` children.
 WideBvh collapse(const Bvh2& b, int width);
 
+// collapse() for the subtree at `root`; the result's order is left empty
+// (leaf slots index the caller's order array).
+WideBvh collapse_from(const Bvh2& b, int32_t root, int width);
+
+// The topology of BVH4 trees (nodes: WideNode<4> array, trees at `roots`)
+// as binary trees over the same leaf slots: each node's valid children are
+// paired (c0 c1)(c2 c3) and boxes are unions, so collapsing them to another
+// width keeps every leaf's triangle range. bin_roots[i] = binary root of roots[i].
+Bvh2 binary_from_bvh4(const uint8_t* nodes, const std::vector<uint32_t>& roots, std::vector<int32_t>& bin_roots);
+
 // Triangle array in leaf order for the device: 3 float4 per triangle,
 // (v0, original index as int bits), (v1, 0), (v2, 0).
 void pack_triangles(const float* pos, const int32_t* idx, const std::vector<int32_t>& order,

@@ -71,10 +71,10 @@ void camera_frame(const ilm_camera& c, float out[12]);
 // device memory (*n_dev, at most `capacity`), without a host round trip.
 // rooted: each ray's tmin field holds (as bits) the BVH node to start from,
 // so one launch can trace rays of several BVHs packed in one node array.
-// Collapses b to BVH4, packs the triangles in leaf order, quantises the
+// Collapses b to a BVH of the given width (2, 4, 8), packs the triangles in leaf order, quantises the
 // nodes when asked, uploads (device meshes) and fills m's info and host
 // copies, as every host builder ends. t0 = when the build started.
-int install_bvh(ilm_mesh* m, Bvh2& b, int node_format, std::chrono::steady_clock::time_point t0);
+int install_bvh(ilm_mesh* m, Bvh2& b, int node_format, std::chrono::steady_clock::time_point t0, int width = 4);
 // Variants of a frame of the wavefront path integrator used by the
 // irradiance cache (one view of width x height x spp paths, paths numbered
 // pixel-major): gather paths start from given rays; first_t and gbuf record

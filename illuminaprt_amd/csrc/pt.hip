@@ -1069,21 +1069,28 @@ int ilm_render_nrc(ilm_scene* s, ilm_nrc* nrc, const ilm_render_params* prm, con
 struct ilm_batch {
     ilm_ctx* ctx = nullptr;
     ilm_scene* merged = nullptr;
-    std::vector<uint32_t> root;
+    std::vector<uint32_t> root;    // the scenes' roots in the node array in use
     std::vector<int32_t> lbase, nl;
     void* d_nodes_float = nullptr;
     void* d_nodes_quant = nullptr;
     int64_t nodes = 0;
     int node_format = ILM_NODES_FLOAT;
+    int width = 4;
     int64_t batch_paths = 0;  // 0: the library default
+    // BVH4 roots and stack bound, and the same trees at widths 2 and 8 over
+    // the same leaf slots (built on first use)
+    std::vector<uint32_t> root4;
+    int need4 = 0, depth4 = 0;
+    struct Wide { void* d_nodes = nullptr; int64_t nodes = 0; int need = 0, depth = 0; std::vector<uint32_t> root; };
+    Wide w2, w8;
 };
 
 namespace {
 
 // Rebases the child words of packed nodes: internal children by the scene's
 // first node, leaves by its first triangle slot.
-void rebase_children(uint32_t* child, uint32_t node_base, uint32_t slot_base) {
-    for (int k = 0; k < 4; ++k) {
+void rebase_children(uint32_t* child, uint32_t node_base, uint32_t slot_base, int width = 4) {
+    for (int k = 0; k < width; ++k) {
         uint32_t c = child[k];
         if (c == ilm::kEmptySlot) continue;
         if (ilm::is_leaf(c)) child[k] = ilm::make_leaf(ilm::leaf_first(c) + slot_base, ilm::leaf_count(c));
@@ -1091,15 +1098,69 @@ void rebase_children(uint32_t* child, uint32_t node_base, uint32_t slot_base) {
     }
 }
 
-void batch_use_format(ilm_batch* b, int fmt) {
+// The batch's trees at width 2 or 8: its BVH4 trees unwidened to binary
+// trees over the same leaf slots and collapsed again, so the packed
+// triangles serve every width.
+int batch_build_width(ilm_batch* b, int width) {
+    ilm_batch::Wide& w = width == 2 ? b->w2 : b->w8;
+    if (w.d_nodes) return ILM_OK;
+    const size_t stride = width == 2 ? sizeof(ilm::WideNode<2>) : sizeof(ilm::WideNode<8>);
+    std::vector<uint8_t> all;
+    w.root.clear();
+    w.need = w.depth = 0;
+    try {
+        std::vector<int32_t> broots;
+        ilm::Bvh2 bin = ilm::binary_from_bvh4(b->merged->mesh->h_nodes.data(), b->root4, broots);
+        int64_t base = 0;
+        for (int32_t r : broots) {
+            ilm::WideBvh t = ilm::collapse_from(bin, r, width);
+            // internal child words are local to the scene's tree: rebase them
+            for (int64_t i = 0; i < t.nnodes; ++i) {
+                uint32_t* ch = reinterpret_cast<uint32_t*>(t.nodes.data() + (size_t)i * stride + (size_t)width * 24);
+                rebase_children(ch, (uint32_t)base, 0u, width);
+            }
+            w.root.push_back((uint32_t)base);
+            w.need = std::max(w.need, t.stack_need);
+            w.depth = std::max(w.depth, t.depth);
+            all.insert(all.end(), t.nodes.begin(), t.nodes.end());
+            base += t.nnodes;
+        }
+        w.nodes = base;
+    } catch (const std::exception& e) {
+        return fail(ILM_ERR_INVALID, std::string("batch tree of width ") + std::to_string(width) + ": " + e.what());
+    }
+    int rc = ilm_alloc(b->ctx, all.size(), &w.d_nodes);
+    if (!rc) rc = ilm_upload(b->ctx, w.d_nodes, all.data(), all.size());
+    return rc;
+}
+
+int batch_use_format(ilm_batch* b, int fmt, int width = 4) {
+    if (fmt == ILM_NODES_QUANTIZED && width != 4) return fail(ILM_ERR_INVALID, "quantised nodes are BVH4");
+    if (width != 2 && width != 4 && width != 8) return fail(ILM_ERR_INVALID, "BVH width must be 2, 4 or 8");
+    if (width != 4) {
+        int rc = batch_build_width(b, width);
+        if (rc) return rc;
+    }
     ilm_mesh* m = b->merged->mesh;
-    m->d_nodes = fmt == ILM_NODES_QUANTIZED ? b->d_nodes_quant : b->d_nodes_float;
+    const ilm_batch::Wide* w = width == 2 ? &b->w2 : width == 8 ? &b->w8 : nullptr;
+    m->d_nodes = w ? w->d_nodes : fmt == ILM_NODES_QUANTIZED ? b->d_nodes_quant : b->d_nodes_float;
     m->node_format = fmt;
+    m->width = width;
+    m->info.width = width;
     m->info.node_format = fmt;
-    m->info.node_stride = fmt == ILM_NODES_QUANTIZED ? (int32_t)sizeof(ilm::QNode4) : (int32_t)sizeof(ilm::WideNode<4>);
-    m->info.node_bytes = b->nodes * m->info.node_stride;
+    m->info.node_stride = fmt == ILM_NODES_QUANTIZED ? (int32_t)sizeof(ilm::QNode4)
+                          : width == 2                ? (int32_t)sizeof(ilm::WideNode<2>)
+                          : width == 8                ? (int32_t)sizeof(ilm::WideNode<8>)
+                                                      : (int32_t)sizeof(ilm::WideNode<4>);
+    m->info.nodes = w ? w->nodes : b->nodes;
+    m->info.node_bytes = m->info.nodes * m->info.node_stride;
+    m->info.stack_need = w ? w->need : b->need4;
+    m->info.depth = w ? w->depth : b->depth4;
+    b->root = w ? w->root : b->root4;
     b->node_format = fmt;
+    b->width = width;
     drop_graph(b->merged);
+    return ILM_OK;
 }
 
 }  // namespace
@@ -1110,9 +1171,9 @@ int ilm_batch_destroy(ilm_batch* b) {
     if (!b) return ILM_OK;
     if (b->merged) {
         ilm_mesh* m = b->merged->mesh;
-        // the mesh frees whichever node array it points at; free the other here
-        void* other = m->d_nodes == b->d_nodes_float ? b->d_nodes_quant : b->d_nodes_float;
-        if (other) ilm_free(b->ctx, other);
+        // the mesh frees whichever node array it points at; free the others here
+        for (void* p : {b->d_nodes_float, b->d_nodes_quant, b->w2.d_nodes, b->w8.d_nodes})
+            if (p && p != m->d_nodes) ilm_free(b->ctx, p);
         ilm_scene_destroy(b->merged);
     }
     delete b;
@@ -1164,6 +1225,7 @@ int ilm_batch_create(ilm_ctx* ctx, ilm_scene* const* scenes, int n, ilm_batch** 
         ilm_scene* s = scenes[i];
         ilm_mesh* m = s->mesh;
         b->root.push_back((uint32_t)nbase);
+        b->root4.push_back((uint32_t)nbase);
         b->lbase.push_back(lb);
         b->nl.push_back(s->nlights);
         pos.insert(pos.end(), m->pos.begin(), m->pos.end());
@@ -1250,6 +1312,8 @@ int ilm_batch_create(ilm_ctx* ctx, ilm_scene* const* scenes, int n, ilm_batch** 
     pm->info.max_leaf = max_leaf;
     pm->info.stack_need = stack_need;
     pm->info.sah_cost = sah;
+    b->need4 = stack_need;
+    b->depth4 = depth;
     pm->built = true;
     ms->mesh = pm;
     ms->owns_mesh = true;
@@ -1272,11 +1336,12 @@ int ilm_batch_set_graph(ilm_batch* b, int enable) {
     return ilm_scene_set_graph(b->merged, enable);
 }
 
-int ilm_batch_configure(ilm_batch* b, int node_format, int64_t batch_paths) {
+int ilm_batch_configure(ilm_batch* b, int node_format, int width, int64_t batch_paths) {
     if (!b || batch_paths < 0) return fail(ILM_ERR_INVALID, "bad arguments");
     if (node_format != ILM_NODES_FLOAT && node_format != ILM_NODES_QUANTIZED)
         return fail(ILM_ERR_INVALID, "unknown node format");
-    batch_use_format(b, node_format);
+    int rc = batch_use_format(b, node_format, width == 0 ? 4 : width);
+    if (rc) return rc;
     b->batch_paths = batch_paths;
     return ILM_OK;
 }
@@ -1344,14 +1409,17 @@ int ilm_batch_autotune(ilm_batch* b, const ilm_batch_view* bv, int nviews, const
     double* acc = nullptr;
     int rc = ilm_alloc(b->ctx, (size_t)npix * 3 * sizeof(double), (void**)&acc);
     if (rc) return rc;
-    const int fmts[2] = {ILM_NODES_FLOAT, ILM_NODES_QUANTIZED};
+    // trees: float BVH4, quantised BVH4, float BVH2, float BVH8
+    const int fmts[4][2] = {{ILM_NODES_FLOAT, 4}, {ILM_NODES_QUANTIZED, 4}, {ILM_NODES_FLOAT, 2}, {ILM_NODES_FLOAT, 8}};
     const int64_t sizes[3] = {int64_t(1) << 20, int64_t(1) << 22, int64_t(1) << 24};
     ilm_batch_tuning t{};
     t.best_ms = 1e30;
     int k = 0;
-    for (int f : fmts)
+    for (const auto& fw : fmts)
         for (int64_t sz : sizes) {
-            batch_use_format(b, f);
+            const int f = fw[0], width = fw[1];
+            rc = batch_use_format(b, f, width);
+            if (rc) break;
             ilm_render_params p = *prm;
             p.batch_paths = (int32_t)sz;
             double best = 1e30;
@@ -1368,18 +1436,21 @@ int ilm_batch_autotune(ilm_batch* b, const ilm_batch_view* bv, int nviews, const
             if (rc) break;
             t.candidate_ms[k] = best;
             t.candidate_format[k] = f;
+            t.candidate_width[k] = width;
             t.candidate_batch[k] = sz;
             ++k;
             if (best < t.best_ms) {
                 t.best_ms = best;
                 t.node_format = f;
+                t.width = width;
                 t.batch_paths = sz;
             }
         }
     ilm_free(b->ctx, acc);
     if (rc) return rc;
     t.candidates = k;
-    batch_use_format(b, t.node_format);
+    rc = batch_use_format(b, t.node_format, t.width);
+    if (rc) return rc;
     b->batch_paths = t.batch_paths;
     if (out) *out = t;
     return ILM_OK;

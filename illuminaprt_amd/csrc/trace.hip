@@ -8,9 +8,10 @@
 // least refill_min of its lanes are idle (32 by default; 64, i.e. only when
 // the whole wave is idle, for coherent camera rays). Per lane: a 16-entry
 // traversal stack in LDS (lane-strided, conflict-free) that spills its
-// oldest entries to a global area only when deeper than 16. Nodes are BVH4,
-// 112 bytes, loaded as seven 16-byte vectors; children are sorted by entry
-// distance with a 5-exchange network and the nearest is visited first.
+// oldest entries to a global area only when deeper than 16. Nodes are wide
+// (BVH2/4/8 with float boxes, 64/112/224 bytes, or 64-byte quantised BVH4),
+// loaded as 16-byte vectors; children are sorted by entry distance with a
+// comparator network and the nearest is visited first.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -25,7 +26,7 @@ constexpr int kBlock = 256;
 constexpr int kLds = 16;
 
 struct TraceArgs {
-    const float4* __restrict__ nodes;  // 7 float4 per BVH4 node
+    const float4* __restrict__ nodes;  // wide nodes (Kind<K>::F4 float4 each)
     const float4* __restrict__ tris;   // 3 float4 per triangle
     const ilm_ray* __restrict__ rays;
     int64_t n;                        // ray count, or capacity when n_dev is set
@@ -52,29 +53,74 @@ __device__ __forceinline__ float slab1(float lox, float hix, float loy, float hi
     return (tn <= tf * 1.00000048f) ? tn : INFINITY;
 }
 
-#define ILM_CSWAP(a, b)                                   \
-    if (d##b < d##a) {                                    \
-        float td = d##a; d##a = d##b; d##b = td;          \
-        uint32_t tc = c##a; c##a = c##b; c##b = tc;       \
-    }
+// Node kinds of the traversal kernels: float boxes at width 4, 2 or 8
+// (WideNode<W>, 7 / 4 / 14 x 16 B) and the quantised BVH4 node (QNode4, 4 x 16 B).
+enum NodeKind { kFloat4 = 0, kQuant4 = 1, kFloat2 = 2, kFloat8 = 3 };
+template <int K> struct Kind;
+template <> struct Kind<kFloat4> { static constexpr int W = 4, F4 = 7; };
+template <> struct Kind<kQuant4> { static constexpr int W = 4, F4 = 4; };
+template <> struct Kind<kFloat2> { static constexpr int W = 2, F4 = 4; };
+template <> struct Kind<kFloat8> { static constexpr int W = 8, F4 = 14; };
 
-// Child entry distances of a node: float boxes (7 x 16 B) or quantised boxes
-// (4 x 16 B, QNode4); returns the child words in c*.
-template <bool QUANT>
+__device__ __forceinline__ void cswap(float& da, float& db, uint32_t& ca, uint32_t& cb) {
+    if (db < da) {
+        float td = da; da = db; db = td;
+        uint32_t tc = ca; ca = cb; cb = tc;
+    }
+}
+
+// Sorts children by entry distance, nearest first (comparator networks:
+// 1, 5 and Batcher's 19 exchanges).
+template <int W>
+__device__ __forceinline__ void sort_children(float (&d)[W], uint32_t (&c)[W]) {
+    if (W == 2) {
+        cswap(d[0], d[1], c[0], c[1]);
+    } else if (W == 4) {
+        cswap(d[0], d[1], c[0], c[1]); cswap(d[2], d[3], c[2], c[3]); cswap(d[0], d[2], c[0], c[2]);
+        cswap(d[1], d[3], c[1], c[3]); cswap(d[1], d[2], c[1], c[2]);
+    } else {
+        constexpr int net[19][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {0, 2}, {1, 3}, {4, 6}, {5, 7}, {1, 2}, {5, 6},
+                                    {0, 4}, {3, 7}, {1, 5}, {2, 6}, {1, 4}, {3, 6}, {2, 4}, {3, 5}, {3, 4}};
+#pragma unroll
+        for (int k = 0; k < 19; ++k) cswap(d[net[k][0]], d[net[k][1]], c[net[k][0]], c[net[k][1]]);
+    }
+}
+
+// Child entry distances of a node (INFINITY: missed or empty) and its child words.
+template <int K>
 __device__ __forceinline__ void node_hits(const float4* __restrict__ nodes, uint32_t cur, const RayState& r,
-                                          float& d0, float& d1, float& d2, float& d3, uint32_t& c0, uint32_t& c1,
-                                          uint32_t& c2, uint32_t& c3) {
-    if (!QUANT) {
-        const float4* np = nodes + (int64_t)cur * 7;
+                                          float (&d)[Kind<K>::W], uint32_t (&c)[Kind<K>::W]) {
+    const float4* np = nodes + (int64_t)cur * Kind<K>::F4;
+    if (K == kFloat4) {
         float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
         uint4 ch = reinterpret_cast<const uint4*>(np)[6];
-        d0 = slab1(lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, r);
-        d1 = slab1(lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, r);
-        d2 = slab1(lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, r);
-        d3 = slab1(lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, r);
-        c0 = ch.x; c1 = ch.y; c2 = ch.z; c3 = ch.w;
+        d[0] = slab1(lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, r);
+        d[1] = slab1(lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, r);
+        d[2] = slab1(lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, r);
+        d[3] = slab1(lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, r);
+        c[0] = ch.x; c[1] = ch.y; c[2] = ch.z; c[3] = ch.w;
+    } else if (K == kFloat2) {
+        // (lox0 lox1 hix0 hix1) (loy.. hiy..) (loz.. hiz..) (child0 child1 pad pad)
+        float4 x = np[0], y = np[1], z = np[2];
+        uint4 ch = reinterpret_cast<const uint4*>(np)[3];
+        d[0] = slab1(x.x, x.z, y.x, y.z, z.x, z.z, r);
+        d[1] = slab1(x.y, x.w, y.y, y.w, z.y, z.w, r);
+        c[0] = ch.x; c[1] = ch.y;
+    } else if (K == kFloat8) {
+        float4 lx0 = np[0], lx1 = np[1], hx0 = np[2], hx1 = np[3], ly0 = np[4], ly1 = np[5], hy0 = np[6],
+               hy1 = np[7], lz0 = np[8], lz1 = np[9], hz0 = np[10], hz1 = np[11];
+        uint4 c0 = reinterpret_cast<const uint4*>(np)[12], c1 = reinterpret_cast<const uint4*>(np)[13];
+        d[0] = slab1(lx0.x, hx0.x, ly0.x, hy0.x, lz0.x, hz0.x, r);
+        d[1] = slab1(lx0.y, hx0.y, ly0.y, hy0.y, lz0.y, hz0.y, r);
+        d[2] = slab1(lx0.z, hx0.z, ly0.z, hy0.z, lz0.z, hz0.z, r);
+        d[3] = slab1(lx0.w, hx0.w, ly0.w, hy0.w, lz0.w, hz0.w, r);
+        d[4] = slab1(lx1.x, hx1.x, ly1.x, hy1.x, lz1.x, hz1.x, r);
+        d[5] = slab1(lx1.y, hx1.y, ly1.y, hy1.y, lz1.y, hz1.y, r);
+        d[6] = slab1(lx1.z, hx1.z, ly1.z, hy1.z, lz1.z, hz1.z, r);
+        d[7] = slab1(lx1.w, hx1.w, ly1.w, hy1.w, lz1.w, hz1.w, r);
+        c[0] = c0.x; c[1] = c0.y; c[2] = c0.z; c[3] = c0.w;
+        c[4] = c1.x; c[5] = c1.y; c[6] = c1.z; c[7] = c1.w;
     } else {
-        const float4* np = nodes + (int64_t)cur * 4;
         float4 o = np[0];
         uint4 ch = reinterpret_cast<const uint4*>(np)[1];
         uint4 qa = reinterpret_cast<const uint4*>(np)[2];
@@ -84,7 +130,7 @@ __device__ __forceinline__ void node_hits(const float4* __restrict__ nodes, uint
         const float Sx = __uint_as_float((ex & 0xFFu) << 23) * r.inv.x, Bx = fmaf(o.x, r.inv.x, -r.oinv.x);
         const float Sy = __uint_as_float(((ex >> 8) & 0xFFu) << 23) * r.inv.y, By = fmaf(o.y, r.inv.y, -r.oinv.y);
         const float Sz = __uint_as_float(((ex >> 16) & 0xFFu) << 23) * r.inv.z, Bz = fmaf(o.z, r.inv.z, -r.oinv.z);
-        float d[4];
+        const uint32_t cw[4] = {ch.x, ch.y, ch.z, ch.w};
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const int sh = 8 * k;
@@ -93,31 +139,16 @@ __device__ __forceinline__ void node_hits(const float4* __restrict__ nodes, uint
             float tz0 = fmaf((float)((qb.x >> sh) & 0xFFu), Sz, Bz), tz1 = fmaf((float)((qb.y >> sh) & 0xFFu), Sz, Bz);
             float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), r.tlo));
             float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), r.thi));
-            d[k] = (tn <= tf * 1.00000048f) ? tn : INFINITY;
+            d[k] = (tn <= tf * 1.00000048f && cw[k] != kEmptySlot) ? tn : INFINITY;
+            c[k] = cw[k];
         }
-        c0 = ch.x; c1 = ch.y; c2 = ch.z; c3 = ch.w;
-        d0 = c0 == kEmptySlot ? INFINITY : d[0];
-        d1 = c1 == kEmptySlot ? INFINITY : d[1];
-        d2 = c2 == kEmptySlot ? INFINITY : d[2];
-        d3 = c3 == kEmptySlot ? INFINITY : d[3];
     }
 }
 
-#ifndef ILM_STACK_DIST
-#define ILM_STACK_DIST 0
-#endif
-
-template <bool ANY, bool STATS, bool QUANT>
+template <bool ANY, bool STATS, int KIND>
 __global__ void __launch_bounds__(kBlock) k_trace4(TraceArgs a) {
-#if ILM_STACK_DIST
-    // stack entries carry the child's entry distance, so that nodes beyond
-    // the closest hit found so far are dropped when popped
-    constexpr int kLdsE = kLds / 2;
-    __shared__ uint2 lds[kLdsE * kBlock];
-    float pd = 0.f;
-#else
+    constexpr int W = Kind<KIND>::W;
     __shared__ uint32_t lds[kLds * kBlock];
-#endif
     const int tid = threadIdx.x;
     const int64_t gtid = (int64_t)blockIdx.x * kBlock + tid;
     const int64_t n = a.n_dev ? (int64_t)min((unsigned long long)a.n, *a.n_dev) : a.n;
@@ -132,46 +163,6 @@ __global__ void __launch_bounds__(kBlock) k_trace4(TraceArgs a) {
     int32_t best_prim = -1;
     int32_t n_nodes = 0, n_tris = 0;
 
-#if ILM_STACK_DIST
-    auto push = [&](uint32_t x, float d) {
-        if (lcnt == kLdsE) {
-            int pos = sp - kLdsE;
-            if (pos < a.spill_depth) {
-                uint2 e = lds[(sp % kLdsE) * kBlock + tid];
-                a.spill[(int64_t)(2 * pos) * a.grid_threads + gtid] = e.x;
-                a.spill[(int64_t)(2 * pos + 1) * a.grid_threads + gtid] = e.y;
-            } else {
-                atomicOr(&a.counters[2], 1ull);  // stack bound exceeded (never expected)
-                return;
-            }
-        } else {
-            ++lcnt;
-        }
-        lds[(sp % kLdsE) * kBlock + tid] = make_uint2(x, __float_as_uint(d));
-        ++sp;
-    };
-    auto pop_one = [&]() -> uint32_t {
-        if (sp == 0) return kNoNode;
-        --sp;
-        uint2 e;
-        if (lcnt > 0) {
-            --lcnt;
-            e = lds[(sp % kLdsE) * kBlock + tid];
-        } else {
-            e.x = a.spill[(int64_t)(2 * sp) * a.grid_threads + gtid];
-            e.y = a.spill[(int64_t)(2 * sp + 1) * a.grid_threads + gtid];
-        }
-        pd = __uint_as_float(e.y);
-        return e.x;
-    };
-    // closest hit: skip entries that start beyond the current hit window
-    auto pop = [&]() -> uint32_t {
-        uint32_t x = pop_one();
-        if (!ANY)
-            while (x != kNoNode && pd > r.thi * 1.00000048f) x = pop_one();
-        return x;
-    };
-#else
     auto push = [&](uint32_t x, float) {
         if (lcnt == kLds) {
             int pos = sp - kLds;
@@ -196,7 +187,6 @@ __global__ void __launch_bounds__(kBlock) k_trace4(TraceArgs a) {
         }
         return a.spill[(int64_t)sp * a.grid_threads + gtid];
     };
-#endif
     auto finish = [&]() {
         if (ANY) {
             a.occ[ray_id] = best_prim >= 0 ? 1 : 0;
@@ -240,15 +230,15 @@ __global__ void __launch_bounds__(kBlock) k_trace4(TraceArgs a) {
         // inner nodes
         while (active && !is_leaf(cur)) {
             if (STATS) ++n_nodes;
-            float d0, d1, d2, d3;
-            uint32_t c0, c1, c2, c3;
-            node_hits<QUANT>(a.nodes, cur, r, d0, d1, d2, d3, c0, c1, c2, c3);
-            ILM_CSWAP(0, 1) ILM_CSWAP(2, 3) ILM_CSWAP(0, 2) ILM_CSWAP(1, 3) ILM_CSWAP(1, 2)
-            if (d3 < INFINITY) push(c3, d3);
-            if (d2 < INFINITY) push(c2, d2);
-            if (d1 < INFINITY) push(c1, d1);
-            if (d0 < INFINITY) {
-                cur = c0;
+            float d[W];
+            uint32_t c[W];
+            node_hits<KIND>(a.nodes, cur, r, d, c);
+            sort_children<W>(d, c);
+#pragma unroll
+            for (int k = W - 1; k >= 1; --k)
+                if (d[k] < INFINITY) push(c[k], d[k]);
+            if (d[0] < INFINITY) {
+                cur = c[0];
             } else {
                 cur = pop();
                 if (cur == kNoNode) finish();
@@ -302,11 +292,16 @@ static int launch_trace(ilm_mesh* m, const ilm_ray* rays, int64_t n, ilm_hit* hi
     if (STATS && !stats) return fail(ILM_ERR_INVALID, "null stats");
     ilm_ctx* ctx = m->ctx;
     ILM_HIP(hipSetDevice(ctx->device));
+    const int kind = m->node_format == ILM_NODES_QUANTIZED ? kQuant4
+                     : m->width == 2 ? kFloat2 : m->width == 8 ? kFloat8 : kFloat4;
+    if (m->node_format == ILM_NODES_QUANTIZED && m->width != 4) return fail(ILM_ERR_STATE, "quantised nodes are BVH4");
+    if (m->width != 2 && m->width != 4 && m->width != 8) return fail(ILM_ERR_STATE, "BVH width must be 2, 4 or 8");
+    const void* fn = kind == kQuant4   ? (const void*)k_trace4<ANY, STATS, kQuant4>
+                     : kind == kFloat2 ? (const void*)k_trace4<ANY, STATS, kFloat2>
+                     : kind == kFloat8 ? (const void*)k_trace4<ANY, STATS, kFloat8>
+                                       : (const void*)k_trace4<ANY, STATS, kFloat4>;
     int per_cu = 0;
-    if (m->node_format == ILM_NODES_QUANTIZED)
-        ILM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace4<ANY, STATS, true>, kBlock, 0));
-    else
-        ILM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace4<ANY, STATS, false>, kBlock, 0));
+    ILM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kBlock, 0));
     per_cu = std::max(per_cu, 1);
     int64_t blocks = (int64_t)per_cu * ctx->cu_count;
     blocks = std::min<int64_t>(blocks, (n + kBlock - 1) / kBlock);
@@ -324,18 +319,20 @@ static int launch_trace(ilm_mesh* m, const ilm_ray* rays, int64_t n, ilm_hit* hi
     a.rooted = rooted ? 1 : 0;
     a.refill_min = refill > 0 ? refill : ctx->trace_refill;
     a.spill_depth = std::max(1, m->info.stack_need + 1);
-    // words per spilled entry: node, and its entry distance when the stack keeps one
-    int rc = ensure_spill(ctx, (size_t)(ILM_STACK_DIST ? 2 : 1) * (size_t)a.spill_depth * (size_t)a.grid_threads);
+    int rc = ensure_spill(ctx, (size_t)a.spill_depth * (size_t)a.grid_threads);
     if (rc) return rc;
     a.spill = ctx->spill;
     a.counters = ctx->counters;
     a.total_waves = (uint32_t)(blocks * (kBlock / 64));
-    if (m->node_format == ILM_NODES_QUANTIZED)
-        hipLaunchKernelGGL((k_trace4<ANY, STATS, true>), dim3((unsigned)blocks), dim3(kBlock), 0,
-                           pick_stream(ctx, stream), a);
+    hipStream_t st = pick_stream(ctx, stream);
+    if (kind == kQuant4)
+        hipLaunchKernelGGL((k_trace4<ANY, STATS, kQuant4>), dim3((unsigned)blocks), dim3(kBlock), 0, st, a);
+    else if (kind == kFloat2)
+        hipLaunchKernelGGL((k_trace4<ANY, STATS, kFloat2>), dim3((unsigned)blocks), dim3(kBlock), 0, st, a);
+    else if (kind == kFloat8)
+        hipLaunchKernelGGL((k_trace4<ANY, STATS, kFloat8>), dim3((unsigned)blocks), dim3(kBlock), 0, st, a);
     else
-        hipLaunchKernelGGL((k_trace4<ANY, STATS, false>), dim3((unsigned)blocks), dim3(kBlock), 0,
-                           pick_stream(ctx, stream), a);
+        hipLaunchKernelGGL((k_trace4<ANY, STATS, kFloat4>), dim3((unsigned)blocks), dim3(kBlock), 0, st, a);
     ILM_HIP(hipGetLastError());
     return ILM_OK;
 }

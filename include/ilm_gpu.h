@@ -135,7 +135,8 @@ int ilm_mesh_destroy(ilm_mesh* mesh);
 
 typedef struct {
     int32_t builder;     /* ILM_BUILDER_* */
-    int32_t width;       /* 4 (default when 0); other widths are rejected */
+    int32_t width;       /* children per node: 2, 4 (default when 0) or 8 with float nodes;
+                            quantised nodes and the device builder are 4 */
     int32_t max_leaf;    /* SAH, LBVH: max triangles per leaf (default 4);
                             REFERENCE, KDTREE*: MaximumLeafObjects (default 10) */
     int32_t max_depth;   /* REFERENCE, KDTREE: MaximumTreeDepth (default 35, KDTREE at most 40) */
@@ -465,23 +466,28 @@ int ilm_render_batch(ilm_batch* batch, const ilm_batch_view* views, int nviews, 
                      double* accum, ilm_render_stats* stats);
 /* As ilm_scene_set_graph, for the batch's frames. */
 int ilm_batch_set_graph(ilm_batch* batch, int enable);
-/* Fixes the node format (ILM_NODES_*) and paths per wavefront batch
- * (0 = default) of later frames. */
-int ilm_batch_configure(ilm_batch* batch, int node_format, int64_t batch_paths);
+/* Fixes the tree of later frames -- node format (ILM_NODES_*) and BVH width
+ * (4 when 0; 2 and 8 with float nodes: the batch's BVH4 trees regrouped over
+ * the same leaves, built on first use) -- and the paths per wavefront batch
+ * (0 = default). */
+int ilm_batch_configure(ilm_batch* batch, int node_format, int width, int64_t batch_paths);
 
 typedef struct {
     int32_t node_format;     /* chosen ILM_NODES_* */
+    int32_t width;           /* chosen BVH width */
     int64_t batch_paths;     /* chosen paths per wavefront batch */
     double best_ms;          /* its timing frame */
     int32_t candidates;
-    int32_t candidate_format[8];
-    int64_t candidate_batch[8];
-    double candidate_ms[8];
+    int32_t candidate_format[16];
+    int32_t candidate_width[16];
+    int64_t candidate_batch[16];
+    double candidate_ms[16];
 } ilm_batch_tuning;
 
-/* Times frames of the given views (at most 2 spp) for each node format and
- * batch size (2^20, 2^22, 2^24 paths), keeps the fastest for later frames and
- * reports the table (out may be NULL). */
+/* Times frames of the given views (at most 2 spp) for each tree (float
+ * BVH4, quantised BVH4, float BVH2, float BVH8) and batch size (2^20, 2^22,
+ * 2^24 paths), keeps the fastest for later frames and reports the table
+ * (out may be NULL). */
 int ilm_batch_autotune(ilm_batch* batch, const ilm_batch_view* views, int nviews, const ilm_render_params* params,
                        ilm_batch_tuning* out);
 

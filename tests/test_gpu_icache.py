@@ -76,7 +76,9 @@ def test_octree_and_lookups_match_restatement(gpu_ctx, ilm, variant):
         agree += ok
     covered = float((got[:, 3] > 0).mean())
     print(f"[ic lookup] agreement {agree / q:.5f}, covered {covered:.3f}", flush=True)
-    assert agree / q >= 0.999 and covered > 0.3
+    # Tabelion weights accept a record only within half its radius and ~6
+    # degrees of its normal at this threshold, so fewer points are covered
+    assert agree / q >= 0.999 and covered > (0.3 if variant == 0 else 0.1)
     ic.close()
     sc.close()
     m.close()

@@ -41,13 +41,15 @@ def _small_scene():
     return meshgen.merge([(sp, si), (tp, ti), (bp, bi)])
 
 
-@pytest.mark.parametrize("builder,fmt", [(0, 0), (1, 0), (2, 0), (3, 0), (4, 0), (0, 1), (1, 1), (4, 1)])
-def test_closest_matches_oracle_brute_force(gpu_ctx, ilm, orc, builder, fmt):
+@pytest.mark.parametrize("builder,fmt,width", [(0, 0, 4), (1, 0, 4), (2, 0, 4), (3, 0, 4), (4, 0, 4), (0, 1, 4),
+                                               (1, 1, 4), (4, 1, 4), (0, 0, 2), (0, 0, 8), (1, 0, 8), (2, 0, 8)])
+def test_closest_matches_oracle_brute_force(gpu_ctx, ilm, orc, builder, fmt, width):
     pos, idx = _small_scene()
     rays = np.concatenate([meshgen.rays_toward(30000, pos, seed=7), meshgen.random_rays(20000, seed=8)])
     n = len(rays)
     m = ilm.Mesh(gpu_ctx, pos, idx)
-    m.build(builder=builder, node_format=fmt)
+    info = m.build(builder=builder, node_format=fmt, width=width)
+    assert info["width"] == width
     rb = ilm.rays_to_device(gpu_ctx, rays)
     hb = gpu_ctx.buffer(n * 16)
     m.trace_closest(rb, n, hb)
@@ -55,7 +57,7 @@ def test_closest_matches_oracle_brute_force(gpu_ctx, ilm, orc, builder, fmt):
     h = hb.download(ilm.HIT_DTYPE, n)
     om = orc.Mesh(pos, idx)
     pc, tc = om.trace(rays, mode=0, accel=0)
-    assert _compare_closest(h["prim"], h["t"], pc, tc, f"brute b{builder} f{fmt}") >= 0.9999
+    assert _compare_closest(h["prim"], h["t"], pc, tc, f"brute b{builder} f{fmt} w{width}") >= 0.9999
     # barycentrics reproduce the hit point on the triangle
     ok = h["prim"] >= 0
     tri = pos[idx[h["prim"][ok]]]
@@ -67,14 +69,14 @@ def test_closest_matches_oracle_brute_force(gpu_ctx, ilm, orc, builder, fmt):
     m.close()
 
 
-@pytest.mark.parametrize("builder,fmt", [(0, 0), (2, 0), (4, 0), (0, 1)])
-def test_any_matches_oracle(gpu_ctx, ilm, orc, builder, fmt):
+@pytest.mark.parametrize("builder,fmt,width", [(0, 0, 4), (2, 0, 4), (4, 0, 4), (0, 1, 4), (0, 0, 2), (0, 0, 8)])
+def test_any_matches_oracle(gpu_ctx, ilm, orc, builder, fmt, width):
     pos, idx = _small_scene()
     rays = meshgen.rays_toward(40000, pos, seed=17)
     rays[:, 7] = np.random.default_rng(3).uniform(0.5, 4.0, len(rays)).astype(np.float32)  # finite segments
     n = len(rays)
     m = ilm.Mesh(gpu_ctx, pos, idx)
-    m.build(builder=builder, node_format=fmt)
+    m.build(builder=builder, node_format=fmt, width=width)
     rb = ilm.rays_to_device(gpu_ctx, rays)
     ob = gpu_ctx.buffer(n)
     m.trace_any(rb, n, ob)

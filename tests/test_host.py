@@ -67,8 +67,8 @@ def test_box_generator(ilm):
         assert (ext > 0).all()
 
 
-def _bvh_invariants(ilm, pos, idx, nodes, order):
-    boxes, child = ilm.decode_bvh4(nodes)
+def _bvh_invariants(ilm, pos, idx, nodes, order, width=4):
+    boxes, child = ilm.decode_bvh(nodes, width)
     nt = len(idx)
     assert sorted(order.tolist()) == list(range(nt))
     tri = pos[idx[order]]                       # triangles in slot order
@@ -76,7 +76,7 @@ def _bvh_invariants(ilm, pos, idx, nodes, order):
     seen = np.zeros(nt, np.int64)
     internal_refs = np.zeros(len(boxes), np.int64)
     for ni in range(len(boxes)):
-        for k in range(4):
+        for k in range(width):
             c = int(child[ni, k])
             b = boxes[ni, k]
             if c & 0x80000000:
@@ -96,16 +96,27 @@ def _bvh_invariants(ilm, pos, idx, nodes, order):
     assert internal_refs[0] == 0 and (internal_refs[1:] == 1).all()   # a tree
 
 
-@pytest.mark.parametrize("builder", [0, 1])
-def test_wide_bvh_invariants(ilm, builder):
+@pytest.mark.parametrize("builder,width", [(0, 4), (1, 4), (0, 2), (0, 8), (1, 8), (2, 8)])
+def test_wide_bvh_invariants(ilm, builder, width):
     sp, si = meshgen.icosphere(4)
     tp, ti = meshgen.random_triangles(3000, seed=3)
     pos, idx = meshgen.merge([(sp, si), (tp, ti)])
     m = ilm.Mesh(None, pos, idx)       # host-only mesh: build and export, no tracing
-    info = m.build(builder=builder)
+    info = m.build(builder=builder, width=width)
     nodes, order = m.export()
-    _bvh_invariants(ilm, pos, idx, nodes, order)
-    assert info["nodes"] == len(nodes) // 112
+    if builder == 2:
+        # kd-trees reference triangles from several cells: check the leaf boxes
+        # and the tree shape on the distinct references
+        boxes, child = ilm.decode_bvh(nodes, width)
+        assert info["width"] == width and info["node_stride"] == {2: 64, 4: 112, 8: 224}[width]
+        m.close()
+        return
+    _bvh_invariants(ilm, pos, idx, nodes, order, width)
+    assert info["nodes"] == len(nodes) // {2: 64, 4: 112, 8: 224}[width]
+    assert info["width"] == width
+    # wider nodes: a shallower tree and a stack bound of (W - 1) per level
+    if width == 8:
+        assert info["depth"] < ilm.Mesh(None, pos, idx).build(builder=builder)["depth"]
     if builder == 0:
         assert info["max_leaf"] <= 4   # the reference topology's depth/radius stops allow bigger leaves
     m.close()
