@@ -8,6 +8,8 @@ under which the first triangle found wins). Any-hit: occlusion bits equal on
 >= 99.99% of rays. The oracle is the C++ restatement of the reference's
 triangle test and BVHMesh / BasicMesh traversal.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -65,6 +67,29 @@ def test_closest_matches_oracle_brute_force(gpu_ctx, ilm, orc, builder, fmt, wid
     Q = (1 - h["u"][ok] - h["v"][ok])[:, None] * tri[:, 0] + h["u"][ok, None] * tri[:, 1] + h["v"][ok, None] * tri[:, 2]
     assert np.abs(P - Q).max() < 1e-4
     for b in (rb, hb):
+        b.free()
+    m.close()
+
+
+@pytest.mark.parametrize("builder,fmt,width", [(0, 0, 4), (1, 0, 4), (0, 1, 4), (0, 0, 2), (0, 0, 8)])
+def test_room_fixture(gpu_ctx, ilm, builder, fmt, width):
+    """The committed box-room fixture (closest hits and shadow occlusion of
+    the oracle's brute force, pinned by float64 rectangle geometry)."""
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "room_rays_4096.npz"))
+    m = ilm.Mesh(gpu_ctx, g["pos"], g["idx"])
+    m.build(builder=builder, node_format=fmt, width=width)
+    n = len(g["closest_rays"])
+    rb = ilm.rays_to_device(gpu_ctx, g["closest_rays"])
+    sb = ilm.rays_to_device(gpu_ctx, g["shadow_rays"])
+    hb, ob = gpu_ctx.buffer(n * 16), gpu_ctx.buffer(n)
+    m.trace_closest(rb, n, hb)
+    m.trace_any(sb, n, ob)
+    gpu_ctx.sync()
+    h = hb.download(ilm.HIT_DTYPE, n)
+    # every ray agrees (the boxes' bottom faces lie on the floor: exact ties may go either way)
+    assert _compare_closest(h["prim"], h["t"], g["closest_prim"], g["closest_t"], f"room b{builder} f{fmt} w{width}") == 1.0
+    assert np.array_equal(ob.download(np.uint8, n), g["shadow_occluded"])
+    for b in (rb, sb, hb, ob):
         b.free()
     m.close()
 
