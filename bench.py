@@ -587,7 +587,7 @@ def main():
     # idle-lane refill threshold per set: the library default (32) for the
     # incoherent and shadow sets, 64 (refill when the whole wave is idle) for
     # coherent camera rays, as the path integrator traces them
-    refill = {"incoherent": 32, "primary": 64, "shadow": 32}
+    refill = {"incoherent": 32, "primary": 64, "shadow": 40}
     sets = {
         "incoherent": (lambda: mesh.trace_closest(rs.incoherent, n, hits), rs.incoherent, False),
         "primary": (lambda: mesh.trace_closest(rs.primary, n, hits), rs.primary, False),

@@ -11,7 +11,11 @@
 // oldest entries to a global area only when deeper than 16. Nodes are wide
 // (BVH2/4/8 with float boxes, 64/112/224 bytes, or 64-byte quantised BVH4),
 // loaded as 16-byte vectors; children are sorted by entry distance with a
-// comparator network and the nearest is visited first.
+// comparator network and the nearest is visited first. Traversal is
+// speculative (Aila and Laine, "Understanding the Efficiency of Ray Traversal
+// on GPUs", HPG 2009): a lane that reaches a leaf parks it and keeps
+// descending until every lane of the wave holds a leaf, so that the wave
+// tests triangles together instead of idling lanes at each leaf.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -158,6 +162,7 @@ __global__ void __launch_bounds__(kBlock) k_trace4(TraceArgs a) {
     int64_t ray_id = 0;
     RayState r;
     uint32_t cur = kNoNode;
+    uint32_t stash = kNoNode;  // a leaf parked during speculative descent
     int sp = 0, lcnt = 0;
     float best_t = 0.f, best_u = 0.f, best_v = 0.f;
     int32_t best_prim = -1;
@@ -218,6 +223,7 @@ __global__ void __launch_bounds__(kBlock) k_trace4(TraceArgs a) {
                     cur = a.rooted ? __float_as_uint(o4.w) : 0u;
                     sp = 0;
                     lcnt = 0;
+                    stash = kNoNode;
                     best_prim = -1;
                     best_t = best_u = best_v = 0.f;
                     n_nodes = n_tris = 0;
@@ -227,8 +233,9 @@ __global__ void __launch_bounds__(kBlock) k_trace4(TraceArgs a) {
         }
         if (!__any(active)) break;
 
-        // inner nodes
-        while (active && !is_leaf(cur)) {
+        // inner nodes; a lane that meets a leaf parks it and keeps descending
+        // until every lane in the loop holds a parked leaf
+        while (active && cur != kNoNode && !is_leaf(cur)) {
             if (STATS) ++n_nodes;
             float d[W];
             uint32_t c[W];
@@ -237,16 +244,16 @@ __global__ void __launch_bounds__(kBlock) k_trace4(TraceArgs a) {
 #pragma unroll
             for (int k = W - 1; k >= 1; --k)
                 if (d[k] < INFINITY) push(c[k], d[k]);
-            if (d[0] < INFINITY) {
-                cur = c[0];
-            } else {
+            cur = d[0] < INFINITY ? c[0] : pop();
+            if (cur != kNoNode && is_leaf(cur) && stash == kNoNode) {
+                stash = cur;
                 cur = pop();
-                if (cur == kNoNode) finish();
             }
+            if (__all(stash != kNoNode)) break;
         }
-        // leaves
-        while (active && is_leaf(cur)) {
-            uint32_t first = leaf_first(cur), cnt = leaf_count(cur);
+        // parked leaves, then any leaves that follow on the stack
+        while (active && stash != kNoNode) {
+            uint32_t first = leaf_first(stash), cnt = leaf_count(stash);
             bool done = false;
             for (uint32_t k = 0; k < cnt; ++k) {
                 const float4* tp = a.tris + 3 * (int64_t)(first + k);
@@ -262,10 +269,15 @@ __global__ void __launch_bounds__(kBlock) k_trace4(TraceArgs a) {
                     if (ANY) { done = true; break; }
                 }
             }
-            if (done) { finish(); break; }
-            cur = pop();
-            if (cur == kNoNode) finish();
+            if (done) { stash = kNoNode; finish(); break; }
+            if (cur != kNoNode && is_leaf(cur)) {
+                stash = cur;
+                cur = pop();
+            } else {
+                stash = kNoNode;
+            }
         }
+        if (active && cur == kNoNode && stash == kNoNode) finish();
     }
 
     // the last wave of the grid resets the fetch counters for the next launch
