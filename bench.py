@@ -584,10 +584,10 @@ def main():
     hits = ctx.buffer(n * HIT_BYTES)
     occ = ctx.buffer(n)
     visits = ctx.buffer(n * 8)
-    # idle-lane refill threshold per set: the library default (32) for the
+    # idle-lane refill threshold per set: the library default (40) for the
     # incoherent and shadow sets, 64 (refill when the whole wave is idle) for
     # coherent camera rays, as the path integrator traces them
-    refill = {"incoherent": 32, "primary": 64, "shadow": 40}
+    refill = {"incoherent": 40, "primary": 64, "shadow": 40}
     sets = {
         "incoherent": (lambda: mesh.trace_closest(rs.incoherent, n, hits), rs.incoherent, False),
         "primary": (lambda: mesh.trace_closest(rs.primary, n, hits), rs.primary, False),
@@ -619,7 +619,7 @@ def main():
             "refill_idle_lanes": refill[name],
         }
         log(f"{name}: {res[name]}")
-    ctx.set_trace_refill(32)
+    ctx.set_trace_refill(40)
     ctx.sync()
     if "incoherent" not in res:
         print(json.dumps(res), flush=True)

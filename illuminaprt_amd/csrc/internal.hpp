@@ -40,7 +40,7 @@ struct ilm_ctx {
     unsigned long long* counters = nullptr;  // [0] next ray, [1] finished waves, [2] error flags
     uint32_t* spill = nullptr;
     size_t spill_words = 0;
-    int trace_refill = 32;  // idle lanes of a wave that trigger its ray refill
+    int trace_refill = 40;  // idle lanes of a wave that trigger its ray refill
     std::mutex mu;
 };
 

@@ -5,7 +5,7 @@
 // refills idle lanes with new rays from a global counter (one atomic per
 // wave per refill, prefix via ballot/mbcnt), so incoherent rays of very
 // different traversal length keep SIMD lanes busy. A wave refills once at
-// least refill_min of its lanes are idle (32 by default; 64, i.e. only when
+// least refill_min of its lanes are idle (40 by default; 64, i.e. only when
 // the whole wave is idle, for coherent camera rays). Per lane: a 16-entry
 // traversal stack in LDS (lane-strided, conflict-free) that spills its
 // oldest entries to a global area only when deeper than 16. Nodes are wide
