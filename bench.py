@@ -28,6 +28,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Mrays/s incoherent @1M tris; time-to-0.05 relRMSE, neural cache vs plain PT"
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_FLOPS = 157.3e12   # MI355X FP32 vector peak
 TRI_BYTES, RAY_BYTES, HIT_BYTES = 48, 32, 16
 
 
@@ -659,8 +660,15 @@ def main():
             "device": ctx.name,
             "cus": ctx.cu_count,
         },
+        # per ray class: roofline fraction by algorithmic bytes, and the VALU
+        # fraction of the FP32 vector peak by the estimate 12 FP32 operations
+        # per box test (a node tests one box per child slot) + 30 per triangle
         "rays": {k: {"mrays_s": round(v["mrays_s"], 2), "kernel_ms": round(v["kernel_ms"], 4),
-                     "nodes_per_ray": round(v["nodes_per_ray"], 3), "tris_per_ray": round(v["tris_per_ray"], 3)}
+                     "nodes_per_ray": round(v["nodes_per_ray"], 3), "tris_per_ray": round(v["tris_per_ray"], 3),
+                     "bytes_per_ray": round(v["bytes_per_ray"], 1),
+                     "roofline_frac": round(v["bytes_per_ray"] * v["mrays_s"] * 1e6 / (HBM_PEAK_GBS * 1e9), 4),
+                     "valu_frac_est": round((12 * v["nodes_per_ray"] * info["width"] + 30 * v["tris_per_ray"])
+                                            * v["mrays_s"] * 1e6 / FP32_PEAK_FLOPS, 4)}
                  for k, v in res.items()},
         "roofline": {
             "bound": "hbm",
@@ -669,7 +677,7 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic_from_profiles(),
-            "kernel": f"ilm::k_trace4<false, false, {'true' if args.nodes == 'quantized' else 'false'}>",
+            "kernel": f"ilm::k_trace4<false, false, {1 if args.nodes == 'quantized' else {4: 0, 2: 2, 8: 3}[args.width]}>",
             "node_bytes": info["node_stride"],
             "bytes_per_ray": round(inc["bytes_per_ray"], 1),
         },
