@@ -585,10 +585,10 @@ def main():
     hits = ctx.buffer(n * HIT_BYTES)
     occ = ctx.buffer(n)
     visits = ctx.buffer(n * 8)
-    # idle-lane refill threshold per set: the library default (40) for the
-    # incoherent and shadow sets, 64 (refill when the whole wave is idle) for
-    # coherent camera rays, as the path integrator traces them
-    refill = {"incoherent": 40, "primary": 64, "shadow": 40}
+    # idle-lane refill threshold per set: the library default (24) for the
+    # incoherent and shadow sets, 56 for coherent camera rays, as the path
+    # integrator traces them
+    refill = {"incoherent": 24, "primary": 56, "shadow": 24}
     sets = {
         "incoherent": (lambda: mesh.trace_closest(rs.incoherent, n, hits), rs.incoherent, False),
         "primary": (lambda: mesh.trace_closest(rs.primary, n, hits), rs.primary, False),
@@ -620,7 +620,7 @@ def main():
             "refill_idle_lanes": refill[name],
         }
         log(f"{name}: {res[name]}")
-    ctx.set_trace_refill(40)
+    ctx.set_trace_refill(24)
     ctx.sync()
     if "incoherent" not in res:
         print(json.dumps(res), flush=True)

@@ -73,8 +73,8 @@ int ilm_ctx_create(ilm_ctx** out, int device) {
     c->cu_count = prop.multiProcessorCount;
     c->name = prop.gcnArchName;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipMalloc(&c->counters, 8 * sizeof(unsigned long long));
-    if (e == hipSuccess) e = hipMemset(c->counters, 0, 8 * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMalloc(&c->counters, ilm::kCounterWords * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMemset(c->counters, 0, ilm::kCounterWords * sizeof(unsigned long long));
     if (e != hipSuccess) {
         int rc = ilm::fail_hip(e, "context setup");
         if (c->counters) (void)hipFree(c->counters);

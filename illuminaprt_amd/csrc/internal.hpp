@@ -27,6 +27,13 @@ int fail_hip(hipError_t e, const char* what);
 
 }  // namespace ilm
 
+namespace ilm {
+constexpr int kTraceSegs = 8;           // ray segments of a traversal launch (one per XCD)
+constexpr int kSegCounter0 = 64;        // first segment counter in ilm_ctx::counters
+constexpr int kSegCounterStride = 64;   // 512 bytes between segment counters
+constexpr int kCounterWords = kSegCounter0 + kTraceSegs * kSegCounterStride;
+}  // namespace ilm
+
 struct ilm_ctx {
     int device = 0;
     int cu_count = 0;
@@ -37,10 +44,13 @@ struct ilm_ctx {
     int next_event = 0;
     // per-context scratch for traversal: ray-fetch counters and the
     // traversal-stack spill area, grown on demand
-    unsigned long long* counters = nullptr;  // [0] next ray, [1] finished waves, [2] error flags
+    // [1] finished waves, [2] error flags, [4] post-process scratch; the
+    // ray-fetch counters of the kTraceSegs ray segments at
+    // kSegCounter0 + k * kSegCounterStride (each on its own 512-byte line)
+    unsigned long long* counters = nullptr;
     uint32_t* spill = nullptr;
     size_t spill_words = 0;
-    int trace_refill = 40;  // idle lanes of a wave that trigger its ray refill
+    int trace_refill = 24;  // idle lanes of a wave that trigger its ray refill
     std::mutex mu;
 };
 

@@ -903,11 +903,11 @@ int render_views(ilm_scene* s, ilm_nrc* nrc, const std::vector<ilm::View>& views
             hipLaunchKernelGGL(ilm::k_generate, dim3((unsigned)((a.npaths + 255) / 256)), dim3(256), 0, st, a);
             int cur = 0;
             for (int b = 0; b < max_bounces; ++b) {
-                // camera rays are coherent: their waves refill only when every
-                // lane is idle (measured fastest for primary rays); later
+                // camera rays are coherent: their waves refill only when 56
+                // lanes are idle (measured fastest for primary rays); later
                 // bounces use the context's threshold
                 int r2 = ilm::trace_queue(s->mesh, a.rays[cur], a.counts + cur, cap, a.hits, nullptr, st, true,
-                                          b == 0 ? 64 : 0);
+                                          b == 0 ? 56 : 0);
                 if (r2) return r2;
                 if (cache)
                     hipLaunchKernelGGL(ilm::k_shade<true>, dim3(grid), dim3(ilm::kPtBlock), 0, st, a, b, cur);

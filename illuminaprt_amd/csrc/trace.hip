@@ -2,11 +2,17 @@
 // the C ABI.
 //
 // Design: persistent workgroups sized to the chip's occupancy; each wave
-// refills idle lanes with new rays from a global counter (one atomic per
-// wave per refill, prefix via ballot/mbcnt), so incoherent rays of very
-// different traversal length keep SIMD lanes busy. A wave refills once at
-// least refill_min of its lanes are idle (40 by default; 64, i.e. only when
-// the whole wave is idle, for coherent camera rays). Per lane: a 16-entry
+// refills idle lanes with new rays from a fetch counter (one atomic per wave
+// per refill, prefix via ballot/mbcnt), so incoherent rays of very different
+// traversal length keep SIMD lanes busy. The rays are split into eight
+// contiguous segments with a counter each; a workgroup starts on segment
+// blockIdx % 8, which under the round-robin dispatch of workgroups to the
+// eight XCDs is its XCD's own, and moves on once a segment is spent. Each
+// XCD's L2 then serves a narrow window of neighbouring (coherent) rays, and
+// the fetch atomics spread over eight counters. Only the first ceil(n / 64)
+// waves of the grid take part, so short launches stay cheap. A wave refills once at
+// least refill_min of its lanes are idle (24 by default; 56 for coherent
+// camera rays, whose lanes finish together). Per lane: a 16-entry
 // traversal stack in LDS (lane-strided, conflict-free) that spills its
 // oldest entries to a global area only when deeper than 16. Nodes are wide
 // (BVH2/4/8 with float boxes, 64/112/224 bytes, or 64-byte quantised BVH4),
@@ -156,9 +162,19 @@ __global__ void __launch_bounds__(kBlock) k_trace4(TraceArgs a) {
     const int tid = threadIdx.x;
     const int64_t gtid = (int64_t)blockIdx.x * kBlock + tid;
     const int64_t n = a.n_dev ? (int64_t)min((unsigned long long)a.n, *a.n_dev) : a.n;
+    // only the first ceil(n / 64) waves can receive rays: the others leave at
+    // once, without touching the counters, so that a short launch (the late
+    // bounces of a frame) costs few atomics
+    const unsigned long long wave_id = (unsigned long long)gtid >> 6;
+    const unsigned long long part = min((unsigned long long)a.total_waves, ((unsigned long long)n + 63ull) >> 6);
+    if (wave_id >= part) return;
+    const unsigned long long seg_len = ((unsigned long long)n + kTraceSegs - 1) / kTraceSegs;
 
     bool active = false;
     bool exhausted = false;  // uniform across the wave
+    // the segment the wave fetches from (it starts on its XCD's) and the
+    // number of segments it has found spent; uniform across the wave
+    unsigned int seg = blockIdx.x % kTraceSegs, spent = 0;
     int64_t ray_id = 0;
     RayState r;
     uint32_t cur = kNoNode;
@@ -209,13 +225,31 @@ __global__ void __launch_bounds__(kBlock) k_trace4(TraceArgs a) {
     for (;;) {
         unsigned long long idle = __ballot(!active);
         if (!exhausted && idle != 0ull && (__popcll(idle) >= a.refill_min || idle == ~0ull)) {
-            unsigned long long base = 0;
-            if (__lane_id() == 0) base = atomicAdd(&a.counters[0], (unsigned long long)__popcll(idle));
-            base = __shfl(base, 0);
-            if (base + (unsigned long long)__popcll(idle) >= (unsigned long long)n) exhausted = true;
+            // idle lanes take the next ray ids of the current segment; a spent
+            // segment sends the wave on to the next one
+            const unsigned long long need = (unsigned long long)__popcll(idle);
+            const unsigned long long rank = (unsigned long long)__popcll(idle & __lanemask_lt());
+            unsigned long long got = 0, mine = ~0ull;
+            while (got < need && spent < (unsigned)kTraceSegs) {
+                const unsigned long long s0 = seg * seg_len;
+                const unsigned long long s1 = min(s0 + seg_len, (unsigned long long)n);
+                const unsigned long long size = s1 > s0 ? s1 - s0 : 0ull;
+                const unsigned long long want = need - got;
+                unsigned long long base = 0;
+                if (__lane_id() == 0) base = atomicAdd(&a.counters[kSegCounter0 + seg * kSegCounterStride], want);
+                base = __shfl(base, 0);
+                const unsigned long long take = base < size ? min(want, size - base) : 0ull;
+                if (rank >= got && rank < got + take) mine = s0 + base + (rank - got);
+                got += take;
+                if (base + want >= size) {
+                    seg = (seg + 1) % kTraceSegs;
+                    ++spent;
+                }
+            }
+            if (spent >= (unsigned)kTraceSegs) exhausted = true;
             if (!active) {
-                int64_t my = (int64_t)base + __popcll(idle & __lanemask_lt());
-                if (my < n) {
+                const int64_t my = (int64_t)mine;
+                if (mine < (unsigned long long)n) {
                     ray_id = my;
                     const float4* rp = reinterpret_cast<const float4*>(a.rays + my);
                     float4 o4 = rp[0], d4 = rp[1];
@@ -284,8 +318,8 @@ __global__ void __launch_bounds__(kBlock) k_trace4(TraceArgs a) {
     // the last wave of the grid resets the fetch counters for the next launch
     if (__lane_id() == 0) {
         unsigned long long prev = atomicAdd(&a.counters[1], 1ull);
-        if (prev == (unsigned long long)a.total_waves - 1ull) {
-            atomicExch(&a.counters[0], 0ull);
+        if (prev == part - 1ull) {
+            for (int k = 0; k < kTraceSegs; ++k) atomicExch(&a.counters[kSegCounter0 + k * kSegCounterStride], 0ull);
             atomicExch(&a.counters[1], 0ull);
         }
     }

@@ -173,7 +173,7 @@ int ilm_bvh_export(ilm_mesh* mesh, void* nodes, int32_t* tri_order);
 
 /* ---------------------------------------------------------------- tracing */
 
-/* Sets how many idle lanes (1..64, default 40) a traversal wave waits for
+/* Sets how many idle lanes (1..64, default 24) a traversal wave waits for
  * before it fetches new rays for them: fewer means more refills (one atomic
  * and a divergent ray load each), more means idle lanes for longer. Applies
  * to later traversal launches of the context. No reference counterpart. */
