@@ -127,6 +127,7 @@ struct PtArgs {
     const float4* lights;    // (Le rgb, area)
     const int2* lrange;      // (first emitter triangle, count)
     int nlights;             // lights of all views' scenes
+    int max_vl;              // shadow-queue slots per path vertex: the most lights of any view
     // views, and the batch of global paths / pixels this launch covers
     const View* views;
     int nviews;
@@ -149,7 +150,8 @@ struct PtArgs {
     float4* scontrib2;       // contribution to the cache target of a training path (rgb)
     uint8_t* socc;
     // [0], [1] ray queues, [2] shadow queue, [3] extension total, [4] shadow total,
-    // [5] cache queries of this batch, [6] training samples of this frame
+    // [5] cache queries of this batch, [6] training samples of this frame,
+    // [7] shadow segments written this bounce (the shadow queue's used slots)
     unsigned long long* counts;
     double* accum;
     // radiance cache (nrc_prefix < 0: plain path tracing)
@@ -253,6 +255,11 @@ __global__ void __launch_bounds__(kPtBlock) k_shade(PtArgs a, int bounce, int cu
     const int nxt = cur ^ 1;
     const int64_t count = (int64_t)a.counts[cur];
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    // the shadow queue has max_vl fixed slots per queue entry (slot j: the
+    // view's light j); unused slots hold an empty segment, so the queue needs
+    // no append atomics. Its length is known up front.
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.counts[2] = (unsigned long long)count * (unsigned long long)a.max_vl;
+    uint32_t nshadow = 0;  // shadow segments this lane wrote (for the frame's ray count)
     // iterate in whole-wave steps so that every lane of a wave reaches the
     // queue appends together
     for (int64_t base = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) & ~63ll; base < count; base += stride) {
@@ -371,17 +378,15 @@ __global__ void __launch_bounds__(kPtBlock) k_shade(PtArgs a, int bounce, int cu
         // next-event estimation over every light except the one hit
         // (SampleAllLights / EstimateDirectLighting, Integrator.cpp:114-185)
         Basis gb = basis_from_w(Ng);
-        // the view's scene's lights (uniform bounds: the loop's wave appends
-        // must be reached by every lane together)
-        const int lbeg = 0, lend = a.nlights;
+        // the view's scene's lights: light j of the view fills shadow slot j
         const int vl0 = a.views[vi].l0, vl1 = a.views[vi].l1;
         const uint32_t root = a.views[vi].root;
-        for (int l = lbeg; l < lend; ++l) {
-            if (__ballot(live && l >= vl0 && l < vl1) == 0ull) continue;
+        for (int j = 0; j < a.max_vl; ++j) {
+            const int l = vl0 + j;
             bool want = false;
             ilm_ray sray;
             f3 contrib = mk(0, 0, 0), contrib2 = mk(0, 0, 0);
-            if (live && l != light_id && l >= vl0 && l < vl1) {
+            if (live && l != light_id && l < vl1) {
                 Block8 lb = sample_block((uint32_t)a.frame[0], (uint64_t)pixel, sample, (uint32_t)bounce,
                                          1u + (uint32_t)(l - vl0));
                 int2 rg = a.lrange[l];
@@ -418,11 +423,23 @@ __global__ void __launch_bounds__(kPtBlock) k_shade(PtArgs a, int bounce, int cu
                     }
                 }
             }
-            int64_t slot = wave_append(&a.counts[2], want);
-            if (want) {
-                a.srays[slot] = sray;
-                a.scontrib[slot] = make_float4(contrib.x, contrib.y, contrib.z, __int_as_float(p));
-                if (CACHE && a.nrc_prefix >= 0) a.scontrib2[slot] = make_float4(contrib2.x, contrib2.y, contrib2.z, 0.f);
+            if (k < count) {
+                const int64_t slot = k * a.max_vl + j;
+                if (want) {
+                    a.srays[slot] = sray;
+                    a.scontrib[slot] = make_float4(contrib.x, contrib.y, contrib.z, __int_as_float(p));
+                    if (CACHE && a.nrc_prefix >= 0)
+                        a.scontrib2[slot] = make_float4(contrib2.x, contrib2.y, contrib2.z, 0.f);
+                    ++nshadow;
+                } else {
+                    // an empty segment (t window below 0: no node is entered)
+                    // and no path to credit
+                    ilm_ray e;
+                    e.ox = 0.f; e.oy = 0.f; e.oz = 0.f; e.tmin = __uint_as_float(0u);
+                    e.dx = 0.f; e.dy = 0.f; e.dz = 1.f; e.tmax = -1.0f;
+                    a.srays[slot] = e;
+                    a.scontrib[slot] = make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
+                }
             }
         }
         // BSDF sampling in the shading frame (PathIntegrator.cpp:138-177)
@@ -466,6 +483,9 @@ __global__ void __launch_bounds__(kPtBlock) k_shade(PtArgs a, int bounce, int cu
             if (measure) a.tbeta[p] = make_float4(tbeta.x, tbeta.y, tbeta.z, 0.f);
         }
     }
+    // one atomic per wave for the frame's shadow-ray count
+    for (int off = 32; off > 0; off >>= 1) nshadow += __shfl_xor(nshadow, off);
+    if (__lane_id() == 0 && nshadow) atomicAdd(&a.counts[7], (unsigned long long)nshadow);
 }
 
 // Cache predictions at the query vertices: L += throughput * max(prediction, 0).
@@ -507,6 +527,7 @@ __global__ void k_shadow_accum(PtArgs a) {
         if (a.socc[k]) continue;
         float4 c = a.scontrib[k];
         int64_t p = __float_as_int(c.w);
+        if (p < 0) continue;  // an unused slot
         atomicAdd(&a.L[3 * p], c.x);
         atomicAdd(&a.L[3 * p + 1], c.y);
         atomicAdd(&a.L[3 * p + 2], c.z);
@@ -530,9 +551,10 @@ __global__ void k_clear_counts(unsigned long long* counts) {
 __global__ void k_end_bounce(unsigned long long* counts, int cur) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         counts[3] += counts[cur];
-        counts[4] += counts[2];
+        counts[4] += counts[7];
         counts[cur] = 0ull;
         counts[2] = 0ull;
+        counts[7] = 0ull;
     }
 }
 
@@ -867,6 +889,7 @@ int render_views(ilm_scene* s, ilm_nrc* nrc, const std::vector<ilm::View>& views
     a.lights = s->d_lights;
     a.lrange = s->d_lrange;
     a.nlights = s->nlights;
+    a.max_vl = max_vl;
     a.views = s->d_views;
     a.nviews = (int)views.size();
     a.max_depth = p.max_depth;
