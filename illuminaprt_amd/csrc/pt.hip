@@ -521,23 +521,38 @@ __global__ void k_nrc_clear(PtArgs a) {
 }
 
 __global__ void k_shadow_accum(PtArgs a) {
-    const int64_t count = (int64_t)a.counts[2];
+    // one thread per path vertex: its max_vl shadow slots are adjacent, and a
+    // path has one vertex per bounce, so its radiance needs no atomics
+    const int64_t count = (int64_t)(a.counts[2] / (unsigned long long)a.max_vl);
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < count; k += stride) {
-        if (a.socc[k]) continue;
-        float4 c = a.scontrib[k];
-        int64_t p = __float_as_int(c.w);
-        if (p < 0) continue;  // an unused slot
-        atomicAdd(&a.L[3 * p], c.x);
-        atomicAdd(&a.L[3 * p + 1], c.y);
-        atomicAdd(&a.L[3 * p + 2], c.z);
-        if (a.nrc_prefix >= 0) {
-            float4 t = a.scontrib2[k];
-            if (t.x != 0.f || t.y != 0.f || t.z != 0.f) {
-                atomicAdd(&a.T[3 * p], t.x);
-                atomicAdd(&a.T[3 * p + 1], t.y);
-                atomicAdd(&a.T[3 * p + 2], t.z);
+        int64_t p = -1;
+        float lx = 0.f, ly = 0.f, lz = 0.f, tx = 0.f, ty = 0.f, tz = 0.f;
+        for (int j = 0; j < a.max_vl; ++j) {
+            const int64_t slot = k * a.max_vl + j;
+            if (a.socc[slot]) continue;
+            const float4 c = a.scontrib[slot];
+            const int32_t pj = __float_as_int(c.w);
+            if (pj < 0) continue;  // an unused slot
+            p = pj;
+            lx += c.x;
+            ly += c.y;
+            lz += c.z;
+            if (a.nrc_prefix >= 0) {
+                const float4 t = a.scontrib2[slot];
+                tx += t.x;
+                ty += t.y;
+                tz += t.z;
             }
+        }
+        if (p < 0) continue;
+        a.L[3 * p] += lx;
+        a.L[3 * p + 1] += ly;
+        a.L[3 * p + 2] += lz;
+        if (a.nrc_prefix >= 0 && (tx != 0.f || ty != 0.f || tz != 0.f)) {
+            a.T[3 * p] += tx;
+            a.T[3 * p + 1] += ty;
+            a.T[3 * p + 2] += tz;
         }
     }
 }
