@@ -260,10 +260,15 @@ __global__ void __launch_bounds__(kPtBlock) k_shade(PtArgs a, int bounce, int cu
     // no append atomics. Its length is known up front.
     if (blockIdx.x == 0 && threadIdx.x == 0) a.counts[2] = (unsigned long long)count * (unsigned long long)a.max_vl;
     uint32_t nshadow = 0;  // shadow segments this lane wrote (for the frame's ray count)
-    // iterate in whole-wave steps so that every lane of a wave reaches the
-    // queue appends together
-    for (int64_t base = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) & ~63ll; base < count; base += stride) {
-        const int64_t k = base + (int64_t)__lane_id();
+    // per-wave counts and the block's base of the extension-ray append,
+    // double-buffered by iteration parity
+    __shared__ uint32_t s_cnt[2][kPtBlock / 64];
+    __shared__ unsigned long long s_base[2];
+    // iterate in whole-block steps so that every thread of the block reaches
+    // the queue appends (and their barriers) together
+    int par = 0;
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < count; base += stride, par ^= 1) {
+        const int64_t k = base + (int64_t)threadIdx.x;
         bool live = k < count;
         int32_t p = 0;
         ilm_hit h;
@@ -475,7 +480,23 @@ __global__ void __launch_bounds__(kPtBlock) k_shade(PtArgs a, int bounce, int cu
                 }
             }
         }
-        int64_t slot = wave_append(&a.counts[nxt], cont);
+        // one atomic per block for the continuing paths
+        int64_t slot;
+        {
+            const unsigned long long m = __ballot(cont);
+            const int wv = threadIdx.x >> 6;
+            if (__lane_id() == 0) s_cnt[par][wv] = (uint32_t)__popcll(m);
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                unsigned long long tot = 0;
+                for (int w = 0; w < kPtBlock / 64; ++w) tot += s_cnt[par][w];
+                s_base[par] = tot ? atomicAdd(&a.counts[nxt], tot) : 0ull;
+            }
+            __syncthreads();
+            unsigned long long off = s_base[par];
+            for (int w = 0; w < wv; ++w) off += s_cnt[par][w];
+            slot = cont ? (int64_t)(off + (unsigned long long)__popcll(m & __lanemask_lt())) : -1;
+        }
         if (cont) {
             a.rays[nxt][slot] = nray;
             a.qpath[nxt][slot] = p;
