@@ -91,15 +91,25 @@ __device__ __forceinline__ f3 bsdf_sample(const Material& m, f3 wo, f3& wi, floa
 __device__ __forceinline__ float luminance(f3 s) { return 0.212671f * s.x + 0.715160f * s.y + 0.072169f * s.z; }
 __device__ __forceinline__ bool is_black(f3 s) { return s.x == 0.f && s.y == 0.f && s.z == 0.f; }
 
-// One atomic per wave: lanes with `want` get consecutive slots.
-__device__ __forceinline__ int64_t wave_append(unsigned long long* counter, bool want) {
-    unsigned long long m = __ballot(want);
-    if (m == 0ull) return -1;
-    int leader = __ffsll((long long)m) - 1;
-    unsigned long long base = 0;
-    if ((int)__lane_id() == leader) base = atomicAdd(counter, (unsigned long long)__popcll(m));
-    base = __shfl(base, leader);
-    return want ? (int64_t)(base + __popcll(m & __lanemask_lt())) : -1;
+
+// Appends the block's wanting threads to a queue with one atomic and returns
+// each one's slot (-1 for the others). Every thread of the block calls it
+// together; cnt (one entry per wave) and base are the call site's own LDS.
+__device__ __forceinline__ int64_t block_append(unsigned long long* counter, bool want, uint32_t* cnt,
+                                                unsigned long long* base) {
+    const unsigned long long m = __ballot(want);
+    const int wv = threadIdx.x >> 6;
+    if (__lane_id() == 0) cnt[wv] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long tot = 0;
+        for (int w = 0; w < kPtBlock / 64; ++w) tot += cnt[w];
+        *base = tot ? atomicAdd(counter, tot) : 0ull;
+    }
+    __syncthreads();
+    unsigned long long off = *base;
+    for (int w = 0; w < wv; ++w) off += cnt[w];
+    return want ? (int64_t)(off + (unsigned long long)__popcll(m & __lanemask_lt())) : -1;
 }
 
 struct CamK {
@@ -260,10 +270,11 @@ __global__ void __launch_bounds__(kPtBlock) k_shade(PtArgs a, int bounce, int cu
     // no append atomics. Its length is known up front.
     if (blockIdx.x == 0 && threadIdx.x == 0) a.counts[2] = (unsigned long long)count * (unsigned long long)a.max_vl;
     uint32_t nshadow = 0;  // shadow segments this lane wrote (for the frame's ray count)
-    // per-wave counts and the block's base of the extension-ray append,
-    // double-buffered by iteration parity
-    __shared__ uint32_t s_cnt[2][kPtBlock / 64];
-    __shared__ unsigned long long s_base[2];
+    // LDS of the block appends (0 continuing paths, 1 cache queries, 2
+    // training records), double-buffered by iteration parity: a wave may
+    // start the next iteration while another still reads this one's
+    __shared__ uint32_t s_cnt[3][2][kPtBlock / 64];
+    __shared__ unsigned long long s_base[3][2];
     // iterate in whole-block steps so that every thread of the block reaches
     // the queue appends (and their barriers) together
     int par = 0;
@@ -351,15 +362,15 @@ __global__ void __launch_bounds__(kPtBlock) k_shade(PtArgs a, int bounce, int cu
             qrec[13] = mat.type == 1 ? mat.ks.z : 0.f;
             qrec[14] = 0.f; qrec[15] = 0.f;
         }
-        if (CACHE) {
-            int64_t qs = wave_append(&a.counts[5], to_cache);
+        if (CACHE && a.nrc_prefix >= 0 && bounce == a.nrc_prefix) {  // uniform: the appends hold barriers
+            int64_t qs = block_append(&a.counts[5], to_cache, s_cnt[1][par], &s_base[1][par]);
             if (to_cache) {
                 float4* d = reinterpret_cast<float4*>(a.queries + 16 * qs);
                 for (int k = 0; k < 4; ++k) d[k] = make_float4(qrec[4 * k], qrec[4 * k + 1], qrec[4 * k + 2], qrec[4 * k + 3]);
                 a.qbeta[qs] = make_float4(beta.x, beta.y, beta.z, __int_as_float(p));
                 live = false;
             }
-            int64_t ts = wave_append(&a.counts[6], new_train);
+            int64_t ts = block_append(&a.counts[6], new_train, s_cnt[2][par], &s_base[2][par]);
             if (new_train) {
                 if (ts < a.train_cap) {
                     float4* d = reinterpret_cast<float4*>(a.tq + 16 * ts);
@@ -481,22 +492,7 @@ __global__ void __launch_bounds__(kPtBlock) k_shade(PtArgs a, int bounce, int cu
             }
         }
         // one atomic per block for the continuing paths
-        int64_t slot;
-        {
-            const unsigned long long m = __ballot(cont);
-            const int wv = threadIdx.x >> 6;
-            if (__lane_id() == 0) s_cnt[par][wv] = (uint32_t)__popcll(m);
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                unsigned long long tot = 0;
-                for (int w = 0; w < kPtBlock / 64; ++w) tot += s_cnt[par][w];
-                s_base[par] = tot ? atomicAdd(&a.counts[nxt], tot) : 0ull;
-            }
-            __syncthreads();
-            unsigned long long off = s_base[par];
-            for (int w = 0; w < wv; ++w) off += s_cnt[par][w];
-            slot = cont ? (int64_t)(off + (unsigned long long)__popcll(m & __lanemask_lt())) : -1;
-        }
+        const int64_t slot = block_append(&a.counts[nxt], cont, s_cnt[0][par], &s_base[0][par]);
         if (cont) {
             a.rays[nxt][slot] = nray;
             a.qpath[nxt][slot] = p;
