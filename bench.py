@@ -677,7 +677,7 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic_from_profiles(),
-            "kernel": f"ilm::k_trace4<false, false, {1 if args.nodes == 'quantized' else {4: 0, 2: 2, 8: 3}[args.width]}>",
+            "kernel": f"ilm::k_trace4<false, false, {1 if args.nodes == 'quantized' else {4: 0, 2: 2, 8: 3}[args.width]}, false>",
             "node_bytes": info["node_stride"],
             "bytes_per_ray": round(inc["bytes_per_ray"], 1),
         },

@@ -106,6 +106,12 @@ int render_paths(ilm_scene* s, const ilm_render_params& p, const PathMode& mode,
                  ilm_render_stats* stats);
 int trace_queue(ilm_mesh* m, const ilm_ray* rays, const unsigned long long* n_dev, int64_t capacity, ilm_hit* hits,
                 uint8_t* occ, hipStream_t stream, bool rooted = false, int refill = 0);
+// One launch over two queues: closest hits of (rays, n_dev) into hits and any
+// hits of (rays2, n2_dev) into occ2 (a bounce's shadow segments traced
+// together with the next bounce's rays).
+int trace_queue_dual(ilm_mesh* m, const ilm_ray* rays, const unsigned long long* n_dev, int64_t capacity,
+                     ilm_hit* hits, const ilm_ray* rays2, const unsigned long long* n2_dev, int64_t capacity2,
+                     uint8_t* occ2, hipStream_t stream, bool rooted = false, int refill = 0);
 // Builds m's BVH4 on its device (linear BVH + collapse); fills the device
 // arrays, info and host copies as ilm_bvh_build does.
 int build_lbvh(ilm_mesh* m, int max_leaf, int node_format);

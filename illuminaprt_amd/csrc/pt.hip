@@ -957,22 +957,29 @@ int render_views(ilm_scene* s, ilm_nrc* nrc, const std::vector<ilm::View>& views
             a.npaths = bt.npaths;
             hipLaunchKernelGGL(ilm::k_generate, dim3((unsigned)((a.npaths + 255) / 256)), dim3(256), 0, st, a);
             int cur = 0;
+            // camera rays are coherent: their waves refill only when 56 lanes
+            // are idle (measured fastest for primary rays); later launches use
+            // the context's threshold
+            int r2 = ilm::trace_queue(s->mesh, a.rays[0], a.counts + 0, cap, a.hits, nullptr, st, true, 56);
+            if (r2) return r2;
             for (int b = 0; b < max_bounces; ++b) {
-                // camera rays are coherent: their waves refill only when 56
-                // lanes are idle (measured fastest for primary rays); later
-                // bounces use the context's threshold
-                int r2 = ilm::trace_queue(s->mesh, a.rays[cur], a.counts + cur, cap, a.hits, nullptr, st, true,
-                                          b == 0 ? 56 : 0);
-                if (r2) return r2;
                 if (cache)
                     hipLaunchKernelGGL(ilm::k_shade<true>, dim3(grid), dim3(ilm::kPtBlock), 0, st, a, b, cur);
                 else
                     hipLaunchKernelGGL(ilm::k_shade<false>, dim3(grid), dim3(ilm::kPtBlock), 0, st, a, b, cur);
-                if (s->nlights > 0) {
+                // this bounce's shadow segments and the next bounce's rays come
+                // from the same shading pass: one launch traces both
+                const bool next = b + 1 < max_bounces;
+                if (s->nlights > 0 && next)
+                    r2 = ilm::trace_queue_dual(s->mesh, a.rays[cur ^ 1], a.counts + (cur ^ 1), cap, a.hits, a.srays,
+                                               a.counts + 2, cap * max_vl, a.socc, st, true);
+                else if (s->nlights > 0)
                     r2 = ilm::trace_queue(s->mesh, a.srays, a.counts + 2, cap * max_vl, nullptr, a.socc, st, true);
-                    if (r2) return r2;
-                    hipLaunchKernelGGL(ilm::k_shadow_accum, dim3(grid), dim3(256), 0, st, a);
-                }
+                else if (next)
+                    r2 = ilm::trace_queue(s->mesh, a.rays[cur ^ 1], a.counts + (cur ^ 1), cap, a.hits, nullptr, st,
+                                          true);
+                if (r2) return r2;
+                if (s->nlights > 0) hipLaunchKernelGGL(ilm::k_shadow_accum, dim3(grid), dim3(256), 0, st, a);
                 hipLaunchKernelGGL(ilm::k_end_bounce, dim3(1), dim3(1), 0, st, a.counts, cur);
                 cur ^= 1;
             }

@@ -51,6 +51,12 @@ struct TraceArgs {
     int64_t grid_threads;
     unsigned long long* counters;
     uint32_t total_waves;
+    // dual launches: a second queue traced for any hits in the same launch,
+    // as ids n1 .. n1 + n2 of the combined range (n1 = the first queue's count)
+    const ilm_ray* __restrict__ rays2;
+    int64_t n2;                        // its count, or capacity when n2_dev is set
+    const unsigned long long* n2_dev;
+    uint8_t* __restrict__ occ2;
 };
 
 __device__ __forceinline__ float slab1(float lox, float hix, float loy, float hiy, float loz, float hiz,
@@ -155,13 +161,14 @@ __device__ __forceinline__ void node_hits(const float4* __restrict__ nodes, uint
     }
 }
 
-template <bool ANY, bool STATS, int KIND>
+template <bool ANY, bool STATS, int KIND, bool DUAL>
 __global__ void __launch_bounds__(kBlock) k_trace4(TraceArgs a) {
     constexpr int W = Kind<KIND>::W;
     __shared__ uint32_t lds[kLds * kBlock];
     const int tid = threadIdx.x;
     const int64_t gtid = (int64_t)blockIdx.x * kBlock + tid;
-    const int64_t n = a.n_dev ? (int64_t)min((unsigned long long)a.n, *a.n_dev) : a.n;
+    const int64_t n1 = a.n_dev ? (int64_t)min((unsigned long long)a.n, *a.n_dev) : a.n;
+    const int64_t n = DUAL ? n1 + (a.n2_dev ? (int64_t)min((unsigned long long)a.n2, *a.n2_dev) : a.n2) : n1;
     // only the first ceil(n / 64) waves can receive rays: the others leave at
     // once, without touching the counters, so that a short launch (the late
     // bounces of a frame) costs few atomics
@@ -176,6 +183,7 @@ __global__ void __launch_bounds__(kBlock) k_trace4(TraceArgs a) {
     // number of segments it has found spent; uniform across the wave
     unsigned int seg = blockIdx.x % kTraceSegs, spent = 0;
     int64_t ray_id = 0;
+    bool any_ray = ANY;  // this lane's ray wants any hit (dual launches: the second queue's rays)
     RayState r;
     uint32_t cur = kNoNode;
     uint32_t stash = kNoNode;  // a leaf parked during speculative descent
@@ -211,6 +219,8 @@ __global__ void __launch_bounds__(kBlock) k_trace4(TraceArgs a) {
     auto finish = [&]() {
         if (ANY) {
             a.occ[ray_id] = best_prim >= 0 ? 1 : 0;
+        } else if (DUAL && any_ray) {
+            a.occ2[ray_id - n1] = best_prim >= 0 ? 1 : 0;
         } else {
             float4 h = make_float4(best_t, __int_as_float(best_prim), best_u, best_v);
             reinterpret_cast<float4*>(a.hits)[ray_id] = h;
@@ -251,7 +261,8 @@ __global__ void __launch_bounds__(kBlock) k_trace4(TraceArgs a) {
                 const int64_t my = (int64_t)mine;
                 if (mine < (unsigned long long)n) {
                     ray_id = my;
-                    const float4* rp = reinterpret_cast<const float4*>(a.rays + my);
+                    any_ray = ANY || (DUAL && my >= n1);
+                    const float4* rp = reinterpret_cast<const float4*>(DUAL && my >= n1 ? a.rays2 + (my - n1) : a.rays + my);
                     float4 o4 = rp[0], d4 = rp[1];
                     ray_setup(r, mk(o4.x, o4.y, o4.z), mk(d4.x, d4.y, d4.z), d4.w);
                     cur = a.rooted ? __float_as_uint(o4.w) : 0u;
@@ -301,7 +312,7 @@ __global__ void __launch_bounds__(kBlock) k_trace4(TraceArgs a) {
                     best_v = v;
                     best_prim = __float_as_int(A.w);
                     r.thi = tmax_window(t);
-                    if (ANY) { done = true; break; }
+                    if (ANY || (DUAL && any_ray)) { done = true; break; }
                 }
             }
             if (done) { stash = kNoNode; finish(); break; }
@@ -325,15 +336,27 @@ __global__ void __launch_bounds__(kBlock) k_trace4(TraceArgs a) {
     }
 }
 
-template <bool ANY, bool STATS>
+// The second queue of a dual launch (any hits).
+struct AnyQueue {
+    const ilm_ray* rays;
+    int64_t capacity;
+    const unsigned long long* n_dev;
+    uint8_t* occ;
+};
+
+template <bool ANY, bool STATS, bool DUAL = false>
 static int launch_trace(ilm_mesh* m, const ilm_ray* rays, int64_t n, ilm_hit* hits, uint8_t* occ, int32_t* stats,
                         void* stream, const unsigned long long* n_dev = nullptr, bool rooted = false,
-                        int refill = 0) {
+                        int refill = 0, const AnyQueue* q2 = nullptr) {
+    static_assert(!DUAL || (!ANY && !STATS), "dual launches trace closest hits plus a queue of any hits");
     if (!m) return fail(ILM_ERR_INVALID, "null mesh");
     if (!m->built) return fail(ILM_ERR_STATE, "BVH not built");
     if (!m->ctx) return fail(ILM_ERR_STATE, "mesh was created without a device context");
     if (n < 0 || (n > 0 && !rays)) return fail(ILM_ERR_INVALID, "bad ray buffer");
-    if (n == 0) return ILM_OK;
+    if (DUAL && (!q2 || q2->capacity < 0 || (q2->capacity > 0 && (!q2->rays || !q2->occ))))
+        return fail(ILM_ERR_INVALID, "bad second queue");
+    const int64_t total = n + (DUAL ? q2->capacity : 0);
+    if (total == 0) return ILM_OK;
     if (!ANY && !hits) return fail(ILM_ERR_INVALID, "null hits");
     if (ANY && !occ) return fail(ILM_ERR_INVALID, "null occlusion output");
     if (STATS && !stats) return fail(ILM_ERR_INVALID, "null stats");
@@ -343,15 +366,15 @@ static int launch_trace(ilm_mesh* m, const ilm_ray* rays, int64_t n, ilm_hit* hi
                      : m->width == 2 ? kFloat2 : m->width == 8 ? kFloat8 : kFloat4;
     if (m->node_format == ILM_NODES_QUANTIZED && m->width != 4) return fail(ILM_ERR_STATE, "quantised nodes are BVH4");
     if (m->width != 2 && m->width != 4 && m->width != 8) return fail(ILM_ERR_STATE, "BVH width must be 2, 4 or 8");
-    const void* fn = kind == kQuant4   ? (const void*)k_trace4<ANY, STATS, kQuant4>
-                     : kind == kFloat2 ? (const void*)k_trace4<ANY, STATS, kFloat2>
-                     : kind == kFloat8 ? (const void*)k_trace4<ANY, STATS, kFloat8>
-                                       : (const void*)k_trace4<ANY, STATS, kFloat4>;
+    const void* fn = kind == kQuant4   ? (const void*)k_trace4<ANY, STATS, kQuant4, DUAL>
+                     : kind == kFloat2 ? (const void*)k_trace4<ANY, STATS, kFloat2, DUAL>
+                     : kind == kFloat8 ? (const void*)k_trace4<ANY, STATS, kFloat8, DUAL>
+                                       : (const void*)k_trace4<ANY, STATS, kFloat4, DUAL>;
     int per_cu = 0;
     ILM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kBlock, 0));
     per_cu = std::max(per_cu, 1);
     int64_t blocks = (int64_t)per_cu * ctx->cu_count;
-    blocks = std::min<int64_t>(blocks, (n + kBlock - 1) / kBlock);
+    blocks = std::min<int64_t>(blocks, (total + kBlock - 1) / kBlock);
     blocks = std::max<int64_t>(blocks, 1);
     TraceArgs a;
     a.nodes = reinterpret_cast<const float4*>(m->d_nodes);
@@ -371,15 +394,19 @@ static int launch_trace(ilm_mesh* m, const ilm_ray* rays, int64_t n, ilm_hit* hi
     a.spill = ctx->spill;
     a.counters = ctx->counters;
     a.total_waves = (uint32_t)(blocks * (kBlock / 64));
+    a.rays2 = DUAL ? q2->rays : nullptr;
+    a.n2 = DUAL ? q2->capacity : 0;
+    a.n2_dev = DUAL ? q2->n_dev : nullptr;
+    a.occ2 = DUAL ? q2->occ : nullptr;
     hipStream_t st = pick_stream(ctx, stream);
     if (kind == kQuant4)
-        hipLaunchKernelGGL((k_trace4<ANY, STATS, kQuant4>), dim3((unsigned)blocks), dim3(kBlock), 0, st, a);
+        hipLaunchKernelGGL((k_trace4<ANY, STATS, kQuant4, DUAL>), dim3((unsigned)blocks), dim3(kBlock), 0, st, a);
     else if (kind == kFloat2)
-        hipLaunchKernelGGL((k_trace4<ANY, STATS, kFloat2>), dim3((unsigned)blocks), dim3(kBlock), 0, st, a);
+        hipLaunchKernelGGL((k_trace4<ANY, STATS, kFloat2, DUAL>), dim3((unsigned)blocks), dim3(kBlock), 0, st, a);
     else if (kind == kFloat8)
-        hipLaunchKernelGGL((k_trace4<ANY, STATS, kFloat8>), dim3((unsigned)blocks), dim3(kBlock), 0, st, a);
+        hipLaunchKernelGGL((k_trace4<ANY, STATS, kFloat8, DUAL>), dim3((unsigned)blocks), dim3(kBlock), 0, st, a);
     else
-        hipLaunchKernelGGL((k_trace4<ANY, STATS, kFloat4>), dim3((unsigned)blocks), dim3(kBlock), 0, st, a);
+        hipLaunchKernelGGL((k_trace4<ANY, STATS, kFloat4, DUAL>), dim3((unsigned)blocks), dim3(kBlock), 0, st, a);
     ILM_HIP(hipGetLastError());
     return ILM_OK;
 }
@@ -601,6 +628,14 @@ int trace_queue(ilm_mesh* m, const ilm_ray* rays, const unsigned long long* n_de
                 uint8_t* occ, hipStream_t stream, bool rooted, int refill) {
     if (occ) return launch_trace<true, false>(m, rays, capacity, nullptr, occ, nullptr, stream, n_dev, rooted, refill);
     return launch_trace<false, false>(m, rays, capacity, hits, nullptr, nullptr, stream, n_dev, rooted, refill);
+}
+
+int trace_queue_dual(ilm_mesh* m, const ilm_ray* rays, const unsigned long long* n_dev, int64_t capacity,
+                     ilm_hit* hits, const ilm_ray* rays2, const unsigned long long* n2_dev, int64_t capacity2,
+                     uint8_t* occ2, hipStream_t stream, bool rooted, int refill) {
+    const AnyQueue q2{rays2, capacity2, n2_dev, occ2};
+    return launch_trace<false, false, true>(m, rays, capacity, hits, nullptr, nullptr, stream, n_dev, rooted, refill,
+                                            &q2);
 }
 }  // namespace ilm
 
