@@ -532,9 +532,9 @@ def main():
     ap.add_argument("--cache-target", type=float, default=0.05)
     ap.add_argument("--cache-max-spp", type=int, default=1024)
     ap.add_argument("--cache-max-frames", type=int, default=400)
-    ap.add_argument("--cache-skip", type=int, default=4, help="neural-cache frames rendered before accumulating")
+    ap.add_argument("--cache-skip", type=int, default=0, help="neural-cache frames rendered before accumulating")
     ap.add_argument("--cache-lr", type=float, default=1e-2)
-    ap.add_argument("--cache-train-steps", type=int, default=4, help="Adam steps per neural-cache frame")
+    ap.add_argument("--cache-train-steps", type=int, default=3, help="Adam steps per neural-cache frame")
     ap.add_argument("--no-ic", action="store_true", help="skip the irradiance-cache comparator")
     ap.add_argument("--ic-threshold", type=float, default=0.3, help="irradiance-cache error threshold")
     args = ap.parse_args()
