@@ -41,6 +41,8 @@
 #include <cstring>
 #include <vector>
 
+#include <hipcub/device/device_select.hpp>
+
 #include "geom.hpp"
 #include "internal.hpp"
 
@@ -112,12 +114,16 @@ __device__ float ic_lookup(const IcArgs& c, f3 p, f3 n, f3& e) {
     return den;
 }
 
-// Lookups of the samples in `list` (indices into the G-buffer); miss[i] = 1
-// for reflector samples no record covers.
-__global__ void k_ic_probe(IcArgs c, const float4* gbuf, const int32_t* list, int64_t n, uint8_t* miss) {
+// Lookups of a placement pass's candidates: with stride > 0 the first sample
+// of every stride-th pixel of every stride-th row (row-major), with stride 0
+// every sample. Candidate i's sample (an index into the G-buffer) goes to
+// list[i], and miss[i] = 1 for a reflector sample that no record covers.
+__global__ void k_ic_probe_pass(IcArgs c, const float4* gbuf, int stride, int width, int spp, int64_t nx, int64_t nc,
+                                int32_t* list, uint8_t* miss) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int64_t g = list[i];
+    if (i >= nc) return;
+    const int64_t g = stride > 0 ? (((i / nx) * stride) * (int64_t)width + (i % nx) * stride) * spp : i;
+    list[i] = (int32_t)g;
     const float4 a = gbuf[3 * g], b = gbuf[3 * g + 1];
     uint8_t m = 0;
     if (a.w == 1.f) {
@@ -264,7 +270,12 @@ struct ilm_ic {
     size_t cap_nodes = 0, cap_refs = 0, cap_recs = 0;
     uint32_t passes = 0;
     unsigned long long* d_count = nullptr;
-    std::vector<void*> scratch;  // per-frame buffers
+    // frame buffers, grown on demand and kept across frames
+    struct Buf {
+        void* p = nullptr;
+        size_t bytes = 0;
+    };
+    Buf gbuf, own, list, miss, sel, nsel, tmp, rays, L, ft, newrecs;
 };
 
 namespace {
@@ -448,7 +459,9 @@ int ilm_ic_destroy(ilm_ic* ic) {
     if (!ic) return ILM_OK;
     for (void* p : {(void*)ic->d_nodes, (void*)ic->d_refs, (void*)ic->d_recs, (void*)ic->d_count})
         if (p) ilm_free(ic->ctx, p);
-    for (void* p : ic->scratch) if (p) ilm_free(ic->ctx, p);
+    for (ilm_ic::Buf* b : {&ic->gbuf, &ic->own, &ic->list, &ic->miss, &ic->sel, &ic->nsel, &ic->tmp, &ic->rays, &ic->L,
+                           &ic->ft, &ic->newrecs})
+        if (b->p) ilm_free(ic->ctx, b->p);
     delete ic;
     return ILM_OK;
 }
@@ -545,21 +558,29 @@ int ilm_ic_render(ilm_ic* ic, const ilm_render_params* prm, double* accum, ilm_i
     const int64_t npix = (int64_t)prm->width * prm->height, nsamp = npix * prm->spp;
     if (nsamp > INT32_MAX) return fail(ILM_ERR_INVALID, "more than 2^31 - 1 samples in one frame");
     const int mn = ic->prm.azimuth_strata * ic->prm.altitude_strata;
-    for (void* p : ic->scratch) if (p) ilm_free(ctx, p);
-    ic->scratch.clear();
-    auto alloc = [&](size_t bytes, void** p) {
-        int rc = ilm_alloc(ctx, bytes, p);
-        if (!rc) ic->scratch.push_back(*p);
+    auto grow = [&](ilm_ic::Buf& b, size_t bytes) -> int {
+        if (b.bytes >= bytes && b.p) return ILM_OK;
+        if (b.p) ilm_free(ctx, b.p);
+        b.p = nullptr;
+        b.bytes = 0;
+        const size_t want = std::max<size_t>(bytes, 256);
+        int rc = ilm_alloc(ctx, want, &b.p);
+        if (!rc) b.bytes = want;
         return rc;
     };
-    float* gbuf = nullptr;
-    int32_t *own = nullptr, *list = nullptr;
-    uint8_t* miss = nullptr;
-    int rc = alloc((size_t)nsamp * 48, (void**)&gbuf);
-    if (!rc) rc = alloc((size_t)nsamp * 4, (void**)&own);
-    if (!rc) rc = alloc((size_t)nsamp * 4, (void**)&list);
-    if (!rc) rc = alloc((size_t)nsamp, (void**)&miss);
+    int rc = grow(ic->gbuf, (size_t)nsamp * 48);
+    if (!rc) rc = grow(ic->own, (size_t)nsamp * 4);
+    if (!rc) rc = grow(ic->list, (size_t)nsamp * 4);
+    if (!rc) rc = grow(ic->miss, (size_t)nsamp);
+    if (!rc) rc = grow(ic->sel, (size_t)nsamp * 4);
+    if (!rc) rc = grow(ic->nsel, 8);
     if (rc) return rc;
+    float* gbuf = (float*)ic->gbuf.p;
+    int32_t* own = (int32_t*)ic->own.p;
+    int32_t* list = (int32_t*)ic->list.p;
+    int32_t* slist = (int32_t*)ic->sel.p;
+    uint8_t* miss = (uint8_t*)ic->miss.p;
+    int32_t* nsel = (int32_t*)ic->nsel.p;
     // 1. direct light (PathIntegrator with one vertex: emission at the first
     //    hit and next-event estimation there) and the first-hit records
     ilm_render_params p1 = *prm;
@@ -571,81 +592,70 @@ int ilm_ic_render(ilm_ic* ic, const ilm_render_params* prm, double* accum, ilm_i
     if (rc) return rc;
     hipLaunchKernelGGL(ilm::k_ic_fill_i32, dim3(ilm::grid_of(nsamp)), dim3(256), 0, st, own, nsamp, -1);
     // 2. record placement passes: every stride-th pixel (first sample), the
-    //    stride halving down to 1, then every sample
+    //    stride halving down to 1, then every sample. The samples that no
+    //    record covers are selected on the device in candidate order, which
+    //    is the order their records are inserted in.
     int64_t created = 0, gather = 0;
-    std::vector<int32_t> cand, add_list;
-    std::vector<uint8_t> hmiss;
     std::vector<int> strides;
     for (int s = ic->prm.placement_stride; s >= 1; s /= 2) strides.push_back(s);
     strides.push_back(0);
     for (int stride : strides) {
-        cand.clear();
-        if (stride > 0) {
-            for (int64_t y = 0; y < prm->height; y += stride)
-                for (int64_t x = 0; x < prm->width; x += stride) cand.push_back((int32_t)((y * prm->width + x) * prm->spp));
-        } else {
-            for (int64_t g = 0; g < nsamp; ++g) cand.push_back((int32_t)g);
-        }
-        const int64_t nc = (int64_t)cand.size();
-        rc = ilm_upload(ctx, list, cand.data(), cand.size() * 4);
-        if (rc) return rc;
-        hipLaunchKernelGGL(ilm::k_ic_probe, dim3(ilm::grid_of(nc)), dim3(256), 0, st, args_of(ic),
-                           reinterpret_cast<const float4*>(gbuf), list, nc, miss);
+        const int64_t nx = stride > 0 ? (prm->width + stride - 1) / stride : 0;
+        const int64_t ny = stride > 0 ? (prm->height + stride - 1) / stride : 0;
+        const int64_t nc = stride > 0 ? nx * ny : nsamp;
+        hipLaunchKernelGGL(ilm::k_ic_probe_pass, dim3(ilm::grid_of(nc)), dim3(256), 0, st, args_of(ic),
+                           reinterpret_cast<const float4*>(gbuf), stride, prm->width, prm->spp, nx, nc, list, miss);
         ILM_HIP(hipGetLastError());
-        hmiss.resize(nc);
-        rc = ilm_download(ctx, hmiss.data(), miss, (size_t)nc);
+        size_t tmp_bytes = 0;
+        ILM_HIP(hipcub::DeviceSelect::Flagged(nullptr, tmp_bytes, list, miss, slist, nsel, (int)nc, st));
+        rc = grow(ic->tmp, tmp_bytes);
         if (rc) return rc;
-        add_list.clear();
-        for (int64_t i = 0; i < nc; ++i)
-            if (hmiss[i]) add_list.push_back(cand[i]);
-        if (add_list.empty()) continue;
-        const int64_t nrec = (int64_t)add_list.size(), nray = nrec * mn;
-        void *rays = nullptr, *L = nullptr, *ft = nullptr, *recs = nullptr, *slist = nullptr;
-        rc = ilm_alloc(ctx, (size_t)nray * sizeof(ilm_ray), &rays);
-        if (!rc) rc = ilm_alloc(ctx, (size_t)nray * 24, &L);
-        if (!rc) rc = ilm_alloc(ctx, (size_t)nray * 4, &ft);
-        if (!rc) rc = ilm_alloc(ctx, (size_t)nrec * 48, &recs);
-        if (!rc) rc = ilm_alloc(ctx, (size_t)nrec * 4, &slist);
-        if (!rc) rc = ilm_upload(ctx, slist, add_list.data(), add_list.size() * 4);
-        if (!rc) rc = ilm_memset(ctx, L, 0, (size_t)nray * 24);
-        if (!rc) {
-            hipLaunchKernelGGL(ilm::k_ic_gather_rays, dim3(ilm::grid_of(nray)), dim3(256), 0, st,
-                               reinterpret_cast<const float4*>(gbuf), (const int32_t*)slist, nrec,
-                               ic->prm.azimuth_strata, ic->prm.altitude_strata,
-                               ic->prm.variant == ILM_IC_TABELION ? 0.f : ic->prm.eps, prm->seed, ic->passes,
-                               (ilm_ray*)rays);
-            // PathLi for every gather ray: one row of nray paths
-            ilm_render_params pg = *prm;
-            pg.width = (int32_t)nray;
-            pg.height = 1;
-            pg.spp = 1;
-            pg.spp_offset = (int32_t)ic->passes;
-            pg.max_depth = ic->prm.ray_depth;
-            pg.rr_depth = 3;
-            pg.eps = ic->prm.eps;
-            pg.seed = prm->seed ^ 0x9e3779b9u;
-            if (nray > INT32_MAX) rc = fail(ILM_ERR_INVALID, "too many gather rays in one pass");
-            ilm::PathMode mode;
-            mode.gather = (const ilm_ray*)rays;
-            mode.first_t = (float*)ft;
-            mode.rr_rule = 1;
-            mode.emit0 = false;
-            if (!rc) rc = ilm::render_paths(ic->scene, pg, mode, (double*)L, nullptr);
-        }
-        std::vector<float> hrec;
-        if (!rc) {
-            hipLaunchKernelGGL(ilm::k_ic_reduce, dim3(ilm::grid_of(nrec)), dim3(256), 0, st,
-                               reinterpret_cast<const float4*>(gbuf), (const int32_t*)slist, nrec, mn,
-                               (const double*)L, (const float*)ft, ic->rmin, ic->rmax, ic->prm.variant,
-                               (float4*)recs);
-            // the sample that made a record takes that record's irradiance
-            // (GetIrradiance returns it, ICIntegrator.cpp:487-493)
-            hipLaunchKernelGGL(ilm::k_ic_own, dim3(ilm::grid_of(nrec)), dim3(256), 0, st, (const int32_t*)slist, nrec,
-                               (int32_t)ic->recs.size(), own);
-            hrec.resize((size_t)nrec * 12);
-            rc = ilm_download(ctx, hrec.data(), recs, hrec.size() * 4);
-        }
-        for (void* q : {rays, L, ft, recs, slist}) if (q) ilm_free(ctx, q);
+        ILM_HIP(hipcub::DeviceSelect::Flagged(ic->tmp.p, tmp_bytes, list, miss, slist, nsel, (int)nc, st));
+        int32_t hsel = 0;
+        rc = ilm_download(ctx, &hsel, nsel, sizeof(hsel));
+        if (rc) return rc;
+        if (hsel == 0) continue;
+        const int64_t nrec = hsel, nray = nrec * mn;
+        if (nray > INT32_MAX) return fail(ILM_ERR_INVALID, "too many gather rays in one pass");
+        rc = grow(ic->rays, (size_t)nray * sizeof(ilm_ray));
+        if (!rc) rc = grow(ic->L, (size_t)nray * 24);
+        if (!rc) rc = grow(ic->ft, (size_t)nray * 4);
+        if (!rc) rc = grow(ic->newrecs, (size_t)nrec * 48);
+        if (rc) return rc;
+        void *rays = ic->rays.p, *L = ic->L.p, *ft = ic->ft.p, *recs = ic->newrecs.p;
+        rc = ilm_memset(ctx, L, 0, (size_t)nray * 24);
+        if (rc) return rc;
+        hipLaunchKernelGGL(ilm::k_ic_gather_rays, dim3(ilm::grid_of(nray)), dim3(256), 0, st,
+                           reinterpret_cast<const float4*>(gbuf), (const int32_t*)slist, nrec,
+                           ic->prm.azimuth_strata, ic->prm.altitude_strata,
+                           ic->prm.variant == ILM_IC_TABELION ? 0.f : ic->prm.eps, prm->seed, ic->passes,
+                           (ilm_ray*)rays);
+        // PathLi for every gather ray: one row of nray paths
+        ilm_render_params pg = *prm;
+        pg.width = (int32_t)nray;
+        pg.height = 1;
+        pg.spp = 1;
+        pg.spp_offset = (int32_t)ic->passes;
+        pg.max_depth = ic->prm.ray_depth;
+        pg.rr_depth = 3;
+        pg.eps = ic->prm.eps;
+        pg.seed = prm->seed ^ 0x9e3779b9u;
+        ilm::PathMode mode;
+        mode.gather = (const ilm_ray*)rays;
+        mode.first_t = (float*)ft;
+        mode.rr_rule = 1;
+        mode.emit0 = false;
+        rc = ilm::render_paths(ic->scene, pg, mode, (double*)L, nullptr);
+        if (rc) return rc;
+        hipLaunchKernelGGL(ilm::k_ic_reduce, dim3(ilm::grid_of(nrec)), dim3(256), 0, st,
+                           reinterpret_cast<const float4*>(gbuf), (const int32_t*)slist, nrec, mn,
+                           (const double*)L, (const float*)ft, ic->rmin, ic->rmax, ic->prm.variant, (float4*)recs);
+        // the sample that made a record takes that record's irradiance
+        // (GetIrradiance returns it, ICIntegrator.cpp:487-493)
+        hipLaunchKernelGGL(ilm::k_ic_own, dim3(ilm::grid_of(nrec)), dim3(256), 0, st, (const int32_t*)slist, nrec,
+                           (int32_t)ic->recs.size(), own);
+        std::vector<float> hrec((size_t)nrec * 12);
+        rc = ilm_download(ctx, hrec.data(), recs, hrec.size() * 4);
         if (rc) return rc;
         std::vector<ilm_ic::Rec> add((size_t)nrec);
         for (int64_t r = 0; r < nrec; ++r) {
