@@ -599,7 +599,13 @@ int ilm_ic_render(ilm_ic* ic, const ilm_render_params* prm, double* accum, ilm_i
     std::vector<int> strides;
     for (int s = ic->prm.placement_stride; s >= 1; s /= 2) strides.push_back(s);
     strides.push_back(0);
+    const bool has_stride1 = std::find(strides.begin(), strides.end(), 1) != strides.end();
     for (int stride : strides) {
+        // with one sample per pixel the every-sample pass probes the stride-1
+        // pass's candidates again, and each of those that was uncovered has
+        // since made a record that covers it (weight at its own record: w > 0),
+        // so the pass cannot select anything
+        if (stride == 0 && prm->spp == 1 && has_stride1) continue;
         const int64_t nx = stride > 0 ? (prm->width + stride - 1) / stride : 0;
         const int64_t ny = stride > 0 ? (prm->height + stride - 1) / stride : 0;
         const int64_t nc = stride > 0 ? nx * ny : nsamp;
