@@ -385,9 +385,11 @@ def cache_convergence(ctx, ilm, workloads, args):
     plain_ms, spp, trace = 0.0, 0, []
     check = 1
     plain_hit = None
+    plain_rays = 0
     while spp < args.cache_max_spp:
         st = sc.render(cam, W, H, 1, buf, max_depth=depth, seed=7001, spp_offset=spp)
         plain_ms += st["ms"]
+        plain_rays += st["camera_rays"] + st["extension_rays"] + st["shadow_rays"]
         spp += 1
         if spp == check:
             e, raw = _corrected(buf.download(np.float64, n3).reshape(H, W, 3) / spp, ref, noise2)
@@ -401,7 +403,8 @@ def cache_convergence(ctx, ilm, workloads, args):
     spp_fit = s_last * (e_last / target) ** 2
     plain_out = {"ms_per_spp": round(plain_ms / spp, 3), "spp_to_target": plain_hit[0] if plain_hit else None,
                  "ms_to_target": round(plain_hit[1], 1) if plain_hit else None,
-                 "spp_fit": round(spp_fit, 1), "ms_fit": round(spp_fit * plain_ms / spp, 1), "trace": trace}
+                 "spp_fit": round(spp_fit, 1), "ms_fit": round(spp_fit * plain_ms / spp, 1),
+                 "mrays_s": round(plain_rays / plain_ms / 1e3, 1), "trace": trace}
     log(f"cache plain: {plain_out}")
 
     # Neural cache: 1-spp frames with one training step each; the image is the
@@ -489,12 +492,31 @@ def cache_convergence(ctx, ilm, workloads, args):
     p_ms = plain_out["ms_to_target"] or plain_out["ms_fit"]
     if nrc_out["ms_to_target"] and p_ms:
         speedup = round(p_ms / nrc_out["ms_to_target"], 3)
+    # The CPU restatement of the reference path (BVHMesh topology, unordered
+    # traversal, PathIntegrator) on a fixed window of 1/256 of the image at
+    # 1 spp: a rate, beside the GPU's plain frames
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        s = oracle.Scene(pos, nrm, idx, tm, tl, mats, lights)
+        ww, wh = W // 16, H // 16
+        x0, y0 = (W - ww) // 2, (H - wh) // 2
+        t0 = time.perf_counter()
+        s.render(workloads.cache_camera(W / H).as_list(), W, H, 1, max_depth=depth, seed=7,
+                 window=(x0, y0, x0 + ww, y0 + wh))
+        dt = time.perf_counter() - t0
+        rays = int(s.last_rays.sum())
+        cpu = {"window": [x0, y0, x0 + ww, y0 + wh], "seconds": round(dt, 2), "rays": rays,
+               "mrays_s": round(rays / dt / 1e6, 4), "cores": int(s.last_threads),
+               "gpu_plain_mrays_s": round(plain_out.get("mrays_s", 0.0), 1)}
+        del s
+        log(f"cache cpu: {cpu}")
     return {"workload": f"radiance-cache scene {len(idx)} tris, {W}x{H}, depth {depth} "
                         f"(cache after {prefix} bounces)",
             "target_rel_rmse": target, "reference_spp": 2 * args.cache_ref_spp,
             "reference_noise_rel_rmse": round(float(np.sqrt(noise2)), 5),
             "plain_pt": plain_out, "neural_cache": nrc_out, "irradiance_cache": ic_out,
-            "speedup_time_to_target": speedup, "mlp": mlp}
+            "speedup_time_to_target": speedup, "mlp": mlp, "cpu_reference_path": cpu}
 
 
 def main():
