@@ -21,6 +21,10 @@
  *    Utility/Sandbox/Multithreaded.h:207-263; batch those calls instead).
  *  - stream == NULL means the context's own stream. Calls are asynchronous
  *    on that stream unless documented otherwise.
+ *  - Traversal launches of one context share its ray-fetch counters and
+ *    stack-spill area: launches given other streams must still be ordered
+ *    after one another (same stream, or events). Launches that overlap
+ *    would trace some of their rays twice and skip others.
  */
 #ifndef ILM_GPU_H
 #define ILM_GPU_H
