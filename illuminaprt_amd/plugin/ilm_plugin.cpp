@@ -127,6 +127,10 @@ bool ArgumentMap::GetArgument(const std::string& name, std::vector<Vector3>& v) 
 // ---------------------------------------------------------------- context
 static std::mutex g_ctx_lock;
 static ilm_ctx* g_ctx = nullptr;
+// Calls that stage data in an object's buffer and launch work on the shared
+// context, one at a time: the reference calls Intersects from several tile
+// threads at once, and a context's traversal launches must be ordered.
+static std::mutex g_call_lock;
 
 ilm_ctx* GpuContext() {
     std::lock_guard<std::mutex> g(g_ctx_lock);
@@ -197,6 +201,7 @@ static void FillSurface(const Ray& r, const ilm_hit& h, DifferentialSurface& s) 
 }
 
 bool GpuBVHMesh::Intersects(const Ray& ray, DifferentialSurface& surface) {
+    std::lock_guard<std::mutex> lock(g_call_lock);
     if (!m_mesh) throw new Exception("GpuBVHMesh: Compile() has not been called");
     ilm_ray q = ToRay(ray);
     ilm_hit h;
@@ -211,6 +216,7 @@ bool GpuBVHMesh::Intersects(const Ray& ray, DifferentialSurface& surface) {
 }
 
 bool GpuBVHMesh::Intersects(const Ray& ray) {
+    std::lock_guard<std::mutex> lock(g_call_lock);
     if (!m_mesh) throw new Exception("GpuBVHMesh: Compile() has not been called");
     ilm_ray q = ToRay(ray);
     uint8_t occ = 0;
@@ -224,6 +230,7 @@ bool GpuBVHMesh::Intersects(const Ray& ray) {
 
 // Host-array batch: one upload, one traversal launch, one download.
 void GpuBVHMesh::IntersectBatch(const std::vector<Ray>& rays, std::vector<DifferentialSurface>& out) {
+    std::lock_guard<std::mutex> lock(g_call_lock);
     if (!m_mesh) throw new Exception("GpuBVHMesh: Compile() has not been called");
     const int64_t n = (int64_t)rays.size();
     out.assign(rays.size(), DifferentialSurface());
@@ -245,6 +252,7 @@ void GpuBVHMesh::IntersectBatch(const std::vector<Ray>& rays, std::vector<Differ
 }
 
 void GpuBVHMesh::OccludedBatch(const std::vector<Ray>& rays, std::vector<uint8_t>& out) {
+    std::lock_guard<std::mutex> lock(g_call_lock);
     if (!m_mesh) throw new Exception("GpuBVHMesh: Compile() has not been called");
     const int64_t n = (int64_t)rays.size();
     out.assign(rays.size(), 0);
@@ -297,6 +305,7 @@ bool GpuSpace::Build() {
 }
 
 bool GpuSpace::Intersects(const Ray& ray, Intersection& intersection) {
+    std::lock_guard<std::mutex> lock(g_call_lock);
     if (!m_mesh) throw new Exception("GpuSpace: Build() has not been called");
     ilm_ray q = ToRay(ray);
     ilm_ray* dr = (ilm_ray*)m_stage;
@@ -318,6 +327,7 @@ bool GpuSpace::Intersects(const Ray& ray, Intersection& intersection) {
 }
 
 bool GpuSpace::Intersects(const Ray& ray) {
+    std::lock_guard<std::mutex> lock(g_call_lock);
     if (!m_mesh) throw new Exception("GpuSpace: Build() has not been called");
     ilm_ray q = ToRay(ray);
     uint8_t occ = 0;
@@ -330,6 +340,7 @@ bool GpuSpace::Intersects(const Ray& ray) {
 }
 
 void GpuSpace::IntersectBatch(const std::vector<Ray>& rays, std::vector<Intersection>& out) {
+    std::lock_guard<std::mutex> lock(g_call_lock);
     if (!m_mesh) throw new Exception("GpuSpace: Build() has not been called");
     const int64_t n = (int64_t)rays.size();
     out.assign(rays.size(), Intersection());
@@ -388,6 +399,7 @@ void GpuPathTracing::Initialise(GpuBVHMesh* mesh, const std::vector<float>& vert
 
 void GpuPathTracing::RenderRegion(const CameraArgs& cam, int width, int height, int spp, uint32_t seed,
                                   std::vector<float>& image) {
+    std::lock_guard<std::mutex> lock(g_call_lock);
     if (!m_scene) throw new Exception("GpuPathTracing: Initialise() has not been called");
     if (width <= 0 || height <= 0 || spp <= 0) throw new Exception("GpuPathTracing: bad image size or spp");
     ilm_render_params p;
