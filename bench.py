@@ -219,7 +219,7 @@ def large_render_leg(ctx, ilm, workloads, args):
     cameras, depth 12, rendered as one batch (one scene, eight views): plain
     frames and neural-cache frames (2-bounce prefix, 10-bounce training
     suffix), 1 spp per view per frame (a 64-spp frame is 64 of them). The
-    plain frames' per-view means judge the cache frames' (after `skip`
+    plain frames' per-view means check the cache frames' (after `skip`
     training frames). HBM footprint = device memory in use after the frames
     minus before the scene was created."""
     free0, total = ctx.memory()
@@ -562,7 +562,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
-              "(the traversal shards by ray ranges; see DESIGN.md)", file=sys.stderr)
+              "(a multi-GPU run would shard the rays by ranges and replicate the BVH)", file=sys.stderr)
         sys.exit(2)
 
     import illuminaprt_amd as ilm

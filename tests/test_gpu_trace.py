@@ -2,7 +2,7 @@
 
 Bar (closest hit): the GPU hit index equals the oracle's on >= 99.99% of rays,
 where a differing index counts as agreeing only at a near-tie, i.e. both hit
-and |t_gpu - t_cpu| <= 1e-6 * t (the tolerance of the north-star gate) or
+and |t_gpu - t_cpu| <= 1e-6 * t (the agreement tolerance of this path) or
 within the reference's own 1e-5 absolute slop (IndexedTriangle.cpp:114-115,
 under which the first triangle found wins). Any-hit: occlusion bits equal on
 >= 99.99% of rays. The oracle is the C++ restatement of the reference's
