@@ -376,7 +376,8 @@ __global__ void __launch_bounds__(256) k_dw(DwArgs a) {
     const int64_t n = a.n, ld = a.ld;
     const int64_t s0 = (int64_t)blockIdx.x * a.chunk, s1 = min(n, s0 + a.chunk);
     float* part = a.partial + (int64_t)blockIdx.x * kParams;
-    for (int tile = wave; tile < 92; tile += 4) {
+    // the 92 output tiles are shared by the gridDim.y workgroups of a chunk
+    for (int tile = (int)blockIdx.y * 4 + wave; tile < 92; tile += 4 * (int)gridDim.y) {
         int layer, to, ti;
         tile_of(tile, layer, to, ti);
         // A rows: G_{layer+1} output features; B rows: H_layer input features
@@ -707,7 +708,8 @@ int ilm_nrc_train_step(ilm_nrc* r, const float* queries, const float* targets, i
     d.ld = (n + 31) / 32 * 32;
     d.chunk = chunk;
     d.partial = r->partial;
-    hipLaunchKernelGGL(k_dw, dim3((unsigned)dw_blocks), dim3(256), 0, st, d);
+    // four workgroups per chunk of samples, each on a quarter of the tiles
+    hipLaunchKernelGGL(k_dw, dim3((unsigned)dw_blocks, 4), dim3(256), 0, st, d);
     ILM_HIP(hipGetLastError());
     r->step += 1;
     float c1 = 1.f - powf(r->b1, (float)r->step), c2 = 1.f - powf(r->b2, (float)r->step);
