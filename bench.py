@@ -479,11 +479,26 @@ def cache_convergence(ctx, ilm, workloads, args):
     _, per = time_launches(ctx, lambda: net.infer(qd, nq, od), 10)
     inf_ms = float(np.mean(per))
     tflops = nq * MLP_FLOP_PER_QUERY / (inf_ms * 1e-3) / 1e12
+    # One training step (fused forward + backward, weight-gradient reduction,
+    # Adam and bf16 repack) on a frame's 65,536 samples: about 3x the forward
+    # FLOP per sample
+    nt = 65536
+    t = np.concatenate([rng.random((nt, 3)), np.ones((nt, 1))], axis=1).astype(np.float32)
+    td = ctx.buffer(t.nbytes)
+    td.upload(t)
+    for _ in range(2):
+        net.train_step(qd, td, nt)
+    _, per = time_launches(ctx, lambda: net.train_step(qd, td, nt), 10)
+    tr_ms = float(np.mean(per))
+    tr_tflops = nt * 3 * MLP_FLOP_PER_QUERY / (tr_ms * 1e-3) / 1e12
     mlp = {"queries": nq, "kernel_ms": round(inf_ms, 4), "tflops": round(tflops, 1),
            "flop_per_query": MLP_FLOP_PER_QUERY, "roofline_frac": round(tflops / MFMA_BF16_PEAK_TFLOPS, 4),
-           "peak_tflops": MFMA_BF16_PEAK_TFLOPS, "kernel": "ilm::k_infer"}
+           "peak_tflops": MFMA_BF16_PEAK_TFLOPS, "kernel": "ilm::k_infer",
+           "train_step": {"samples": nt, "ms": round(tr_ms, 4), "tflops": round(tr_tflops, 1),
+                          "flop_per_sample": 3 * MLP_FLOP_PER_QUERY,
+                          "roofline_frac": round(tr_tflops / MFMA_BF16_PEAK_TFLOPS, 4)}}
     log(f"cache mlp inference: {mlp}")
-    for b in (qd, od, buf):
+    for b in (qd, od, td, buf):
         b.free()
     net.close()
     sc.close()
