@@ -32,6 +32,12 @@ FP32_PEAK_FLOPS = 157.3e12   # MI355X FP32 vector peak
 TRI_BYTES, RAY_BYTES, HIT_BYTES = 48, 32, 16
 
 
+def train_fraction(args, paths):
+    """Fraction of a frame's paths extended as training paths: as many as
+    the 65,536-sample training batch holds, at most --cache-train-frac."""
+    return min(args.cache_train_frac, 65536 / max(1, paths))
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -251,8 +257,8 @@ def large_render_leg(ctx, ilm, workloads, args):
     skip, cache = args.large_skip, []
     for i in range(skip + args.large_frames):
         cache.append(b.render_nrc(net, views, acc if i >= skip else None, seed=400 + i, prefix_depth=prefix,
-                                  suffix_depth=depth - prefix, train_fraction=1 / 16, train_batch=65536,
-                                  train_steps=args.large_train_steps))
+                                  suffix_depth=depth - prefix, train_fraction=train_fraction(args, npix),
+                                  train_batch=65536, train_steps=args.large_train_steps))
     cache_means = view_means() / args.large_frames
     free1, _ = ctx.memory()
     net.close()
@@ -271,6 +277,7 @@ def large_render_leg(ctx, ilm, workloads, args):
            "hbm_footprint_gib": round((free0 - free1) / 2**30, 2), "hbm_total_gib": round(total / 2**30, 1),
            "plain": summary(plain), "neural_cache": summary(cache[skip:]),
            "cache_training_frames": skip, "cache_train_steps_per_frame": args.large_train_steps,
+           "cache_train_fraction": round(train_fraction(args, npix), 6),
            "cache_over_plain_view_means": [round(float(c / p), 4) for c, p in zip(cache_means, plain_means)]}
     log(f"large render: {out}")
     return out
@@ -416,7 +423,7 @@ def cache_convergence(ctx, ilm, workloads, args):
     skip = args.cache_skip
     while frames < args.cache_max_frames:
         st = sc.render_nrc(net, cam, W, H, 1, buf if frames >= skip else None, seed=20001 + frames,
-                           prefix_depth=prefix, suffix_depth=depth - prefix, train_fraction=1 / 16,
+                           prefix_depth=prefix, suffix_depth=depth - prefix, train_fraction=train_fraction(args, W * H),
                            train_batch=65536, train_steps=args.cache_train_steps)
         nrc_ms += st["ms"]
         frames += 1
@@ -430,7 +437,8 @@ def cache_convergence(ctx, ilm, workloads, args):
                     break
     nrc_out = {"ms_per_frame": round(nrc_ms / frames, 3), "frames_to_target": nrc_hit[0] if nrc_hit else None,
                "ms_to_target": round(nrc_hit[1], 1) if nrc_hit else None, "skip_frames": skip,
-               "train_steps_per_frame": args.cache_train_steps, "trace": ntrace}
+               "train_steps_per_frame": args.cache_train_steps,
+               "train_fraction": round(train_fraction(args, W * H), 5), "trace": ntrace}
     log(f"cache nrc: {nrc_out}")
 
     # Irradiance cache (the reference's comparator): 1-spp frames, the cache
@@ -571,6 +579,8 @@ def main():
     ap.add_argument("--cache-max-frames", type=int, default=400)
     ap.add_argument("--cache-skip", type=int, default=0, help="neural-cache frames rendered before accumulating")
     ap.add_argument("--cache-lr", type=float, default=1e-2)
+    ap.add_argument("--cache-train-frac", type=float, default=1 / 16,
+                    help="largest fraction of paths extended as training paths (fewer when 65,536 fill the batch)")
     ap.add_argument("--cache-train-steps", type=int, default=3, help="Adam steps per neural-cache frame")
     ap.add_argument("--no-ic", action="store_true", help="skip the irradiance-cache comparator")
     ap.add_argument("--ic-threshold", type=float, default=0.3, help="irradiance-cache error threshold")
