@@ -871,6 +871,10 @@ int render_views(ilm_scene* s, ilm_nrc* nrc, const std::vector<ilm::View>& views
         max_spp = std::max(max_spp, v.spp);
         max_vl = std::max(max_vl, v.l1 - v.l0);
     }
+    // extend no more training paths than the batch keeps: the records beyond
+    // train_cap would be dropped, and which ones are kept would follow the
+    // order of the queue appends
+    if (cache && total_paths > 0) train_frac = std::min(train_frac, (float)((double)train_cap / (double)total_paths));
     const int64_t want = std::max<int64_t>(p.batch_paths > 0 ? p.batch_paths : (int64_t(1) << 22), max_spp);
     struct Batch { int64_t pix0, npix, path0, npaths; };
     std::vector<Batch> batches;

@@ -611,7 +611,9 @@ int ilm_nrc_train_step(ilm_nrc* nrc, const float* queries, const float* targets,
 typedef struct {
     int32_t prefix_depth;   /* depth of the vertex answered by the cache (default 2, >= 1) */
     int32_t suffix_depth;   /* further bounces of training paths (default 8) */
-    float train_fraction;   /* fraction of paths extended as training paths (default 1/16) */
+    float train_fraction;   /* largest fraction of paths extended as training paths (default
+                               1/16); lowered to train_batch / (the frame's paths) when that is
+                               smaller, so that every training path's record is kept */
     int32_t train_batch;    /* training samples kept per frame (default 65536) */
     int32_t train_steps;    /* Adam steps per frame (0 = no training, negative = 1) */
     int32_t accumulate;     /* add this frame's radiance to accum */
