@@ -69,8 +69,19 @@ def test_room_same_samples_matches_oracle(gpu_ctx, ilm, orc):
     W = H = 128
     for depth in (1, 5):
         g, st = _gpu_render(gpu_ctx, ilm, sa, cam, W, H, 16, max_depth=depth)
-        c = _cpu_render(orc, sa, cam.as_list(), W, H, 16, max_depth=depth)
+        s = orc.Scene(*sa)
+        c = s.render(cam.as_list(), W, H, 16, max_depth=depth, rr_depth=4, seed=1234)
         _same_sample_agreement(g, c, 16, f"room depth {depth}")
+        # ray counts of the same samples: the oracle counts camera and
+        # extension rays together, and a shadow segment whenever the light
+        # sample is non-black; the GPU traces only segments whose BSDF-cosine
+        # contribution is non-zero and never counts its queue's unused slots
+        c_ext, c_shd = (int(x) for x in s.last_rays)
+        g_ext = st["camera_rays"] + st["extension_rays"]
+        print(f"[room depth {depth}] rays gpu {g_ext} + {st['shadow_rays']} shadow, cpu {c_ext} + {c_shd}",
+              flush=True)
+        assert abs(g_ext - c_ext) <= 0.01 * c_ext
+        assert 0.5 * c_shd <= st["shadow_rays"] <= c_shd
     assert st["camera_rays"] == W * H * 16 and st["extension_rays"] > 0 and st["shadow_rays"] > 0
 
 
