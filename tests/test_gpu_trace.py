@@ -215,6 +215,40 @@ def test_empty_and_tiny_inputs(gpu_ctx, ilm):
     m.close()
 
 
+def test_every_ray_traced_once_for_any_count(gpu_ctx, ilm, orc):
+    """Launches split their rays into eight segments with their own fetch
+    counters (one per XCD) and only the first ceil(n / 64) waves take part:
+    for counts around those boundaries every ray is traced (no hit record
+    keeps its sentinel) and hits equal brute force; any hits too."""
+    pos, idx = _small_scene()
+    rays_all = meshgen.rays_toward(5000, pos, seed=21)
+    m = ilm.Mesh(gpu_ctx, pos, idx)
+    m.build()
+    om = orc.Mesh(pos, idx)
+    pc_all, tc_all = om.trace(rays_all, mode=0, accel=0)
+    oc_all, _ = om.trace(rays_all, mode=1, accel=0)
+    rb = ilm.rays_to_device(gpu_ctx, rays_all)
+    hb = gpu_ctx.buffer(len(rays_all) * 16)
+    ob = gpu_ctx.buffer(len(rays_all))
+    sentinel = np.zeros(len(rays_all), ilm.HIT_DTYPE)
+    sentinel["prim"] = -7
+    for n in (1, 2, 7, 8, 9, 15, 63, 64, 65, 127, 129, 511, 512, 513, 1000, 4095, 4097, 5000):
+        hb.upload(sentinel)
+        ob.upload(np.full(len(rays_all), 9, np.uint8))
+        m.trace_closest(rb, n, hb)
+        m.trace_any(rb, n, ob)
+        gpu_ctx.sync()
+        h = hb.download(ilm.HIT_DTYPE, len(rays_all))
+        occ = ob.download(np.uint8, len(rays_all))
+        assert (h["prim"][:n] != -7).all() and (h["prim"][n:] == -7).all(), n
+        assert (occ[:n] != 9).all() and (occ[n:] == 9).all(), n
+        assert _compare_closest(h["prim"][:n], h["t"][:n], pc_all[:n], tc_all[:n], f"n={n}") >= 0.999
+        assert (occ[:n] == (oc_all[:n] >= 0)).mean() >= 0.999
+    for b in (rb, hb, ob):
+        b.free()
+    m.close()
+
+
 @pytest.fixture(scope="module")
 def c2(gpu_ctx, ilm):
     from illuminaprt_amd import workloads
