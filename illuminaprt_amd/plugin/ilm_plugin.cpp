@@ -4,6 +4,7 @@
 
 #include <cctype>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <sstream>
 
@@ -507,19 +508,64 @@ private:
 
 }  // namespace IlluminaGpu
 
-// The instance lives until DisposePlugInSingleton; the shared device context
-// is released there, after the engine has released the plug-in's instances.
+// Plug-in objects: the singleton (GetPlugInSingleton / DisposePlugInSingleton)
+// and the multi-instance objects (CreatePlugInInstance ... DisposePlugInInstances,
+// keyed by handle). The device context they share is released once the last
+// of them is disposed; the engine releases the plug-in's shapes, spaces and
+// integrators before it disposes the plug-in.
+static std::mutex g_plugin_lock;
 static IlluminaGpu::GpuAccelPlugIn* g_plugin = nullptr;
+static std::map<IlluminaGpu::PlugInHandle, IlluminaGpu::GpuAccelPlugIn*> g_instances;
+static IlluminaGpu::PlugInHandle g_next_handle = 1;
+
+// Caller holds g_plugin_lock.
+static void ReleaseContextIfUnused() {
+    if (!g_plugin && g_instances.empty()) IlluminaGpu::ReleaseGpuContext();
+}
 
 extern "C" IlluminaGpu::IPlugIn* GetPlugInSingleton(void) {
+    std::lock_guard<std::mutex> g(g_plugin_lock);
     if (!g_plugin) g_plugin = new IlluminaGpu::GpuAccelPlugIn();
     return g_plugin;
 }
 
-extern "C" void DisposePlugInSingleton(IlluminaGpu::IPlugIn* plugin) {
-    if (plugin && plugin == g_plugin) {
-        delete g_plugin;
-        g_plugin = nullptr;
-        IlluminaGpu::ReleaseGpuContext();
-    }
+extern "C" void DisposePlugInSingleton(void) {
+    std::lock_guard<std::mutex> g(g_plugin_lock);
+    delete g_plugin;
+    g_plugin = nullptr;
+    ReleaseContextIfUnused();
+}
+
+extern "C" IlluminaGpu::PlugInHandle CreatePlugInInstance(void) {
+    std::lock_guard<std::mutex> g(g_plugin_lock);
+    const IlluminaGpu::PlugInHandle h = g_next_handle++;
+    g_instances[h] = new IlluminaGpu::GpuAccelPlugIn();
+    return h;
+}
+
+extern "C" IlluminaGpu::IPlugIn* GetPlugInInstance(IlluminaGpu::PlugInHandle handle) {
+    std::lock_guard<std::mutex> g(g_plugin_lock);
+    auto it = g_instances.find(handle);
+    return it == g_instances.end() ? nullptr : it->second;
+}
+
+extern "C" void DisposePlugInInstance(IlluminaGpu::PlugInHandle handle) {
+    std::lock_guard<std::mutex> g(g_plugin_lock);
+    auto it = g_instances.find(handle);
+    if (it == g_instances.end()) return;
+    delete it->second;
+    g_instances.erase(it);
+    ReleaseContextIfUnused();
+}
+
+extern "C" void DisposePlugInInstances(void) {
+    std::lock_guard<std::mutex> g(g_plugin_lock);
+    for (auto& kv : g_instances) delete kv.second;
+    g_instances.clear();
+    ReleaseContextIfUnused();
+}
+
+extern "C" int IlluminaGpuContextHeld(void) {
+    std::lock_guard<std::mutex> g(IlluminaGpu::g_ctx_lock);
+    return IlluminaGpu::g_ctx != nullptr;
 }

@@ -270,6 +270,9 @@ public:
     virtual void Unregister(EngineKernel* engine) = 0;
 };
 
+// Handle of a multi-instance plug-in object (Core/System/LibrarySymbols.h:15).
+typedef unsigned long PlugInHandle;
+
 // The device context the plug-in's objects share (created on first use, device 0).
 ilm_ctx* GpuContext();
 
@@ -278,6 +281,13 @@ ilm_ctx* GpuContext();
 // C symbols the plug-in exports (Core/System/LibrarySymbols.h:15-23,
 // Plugin/Dummy/DummyLib.cpp:27-77).
 extern "C" {
-IlluminaGpu::IPlugIn* GetPlugInSingleton(void);
-void DisposePlugInSingleton(IlluminaGpu::IPlugIn* plugin);
+IlluminaGpu::IPlugIn* GetPlugInSingleton(void);                            // SYMBOL_SINGLE_GET
+void DisposePlugInSingleton(void);                                         // SYMBOL_SINGLE_DISPOSE
+IlluminaGpu::PlugInHandle CreatePlugInInstance(void);                      // SYMBOL_MULTI_CREATE
+IlluminaGpu::IPlugIn* GetPlugInInstance(IlluminaGpu::PlugInHandle handle);  // SYMBOL_MULTI_GET; null if unknown
+void DisposePlugInInstance(IlluminaGpu::PlugInHandle handle);              // SYMBOL_MULTI_DISPOSE
+void DisposePlugInInstances(void);                                         // SYMBOL_MULTI_DISPOSEALL
+// 1 while the plug-in objects hold their shared device context (a
+// diagnostic of this plug-in; the reference has no counterpart).
+int IlluminaGpuContextHeld(void);
 }

@@ -9,6 +9,11 @@
 //   plugin_sandbox <plugin.so> host
 //       host-side checks (argument parsing, factory errors, registration);
 //       needs no GPU. Prints "host ok" on success.
+//   plugin_sandbox <plugin.so> multi
+//       resolves all six plug-in symbols through the loader's types and
+//       checks singleton / multi-instance lifetimes and the release of the
+//       shared device context (without a device: the bookkeeping only).
+//       Prints "multi ok" on success.
 //   plugin_sandbox <plugin.so> trace <in.bin> <out.bin> "<shape args>"
 //       in:  int64 nv, nt, n, nper; float pos[3nv]; int32 idx[3nt]; float rays[8n]
 //       out: batched closest hits (n x {t, prim, u, v}), batched occlusion
@@ -52,8 +57,14 @@
 
 using namespace IlluminaGpu;
 
-typedef IPlugIn* (*GetPlugInFn)(void);
-typedef void (*DisposePlugInFn)(IPlugIn*);
+// The loader's symbol types (Core/System/LibrarySymbols.h:17-23).
+typedef IPlugIn* (*SYMBOL_SINGLE_GET)(void);
+typedef void (*SYMBOL_SINGLE_DISPOSE)(void);
+typedef PlugInHandle (*SYMBOL_MULTI_CREATE)(void);
+typedef IPlugIn* (*SYMBOL_MULTI_GET)(PlugInHandle);
+typedef void (*SYMBOL_MULTI_DISPOSE)(PlugInHandle);
+typedef void (*SYMBOL_MULTI_DISPOSEALL)(void);
+typedef int (*ContextHeldFn)(void);
 
 template <class T>
 static void read_into(FILE* f, std::vector<T>& v, size_t n) {
@@ -134,6 +145,65 @@ static int run_host(IPlugIn* plugin) {
     plugin->Unregister(&kernel);
     ok &= !kernel.GetShapeManager().QueryFactory("GpuBVHMesh");
     printf(ok ? "host ok\n" : "host FAILED\n");
+    return ok ? 0 : 1;
+}
+
+// Resolves all six plug-in symbols through the loader's types and checks the
+// lifetime rules: handles name distinct objects until disposed, and the shared
+// device context is released (with a device) once the singleton and every
+// instance are gone. The singleton is disposed with no argument, as
+// LibraryManager::DisposePlugInSingleton calls it (LibraryManager.cpp:80-90).
+static int run_multi(void* lib) {
+    auto get = (SYMBOL_SINGLE_GET)dlsym(lib, "GetPlugInSingleton");
+    auto dispose = (SYMBOL_SINGLE_DISPOSE)dlsym(lib, "DisposePlugInSingleton");
+    auto create = (SYMBOL_MULTI_CREATE)dlsym(lib, "CreatePlugInInstance");
+    auto get_inst = (SYMBOL_MULTI_GET)dlsym(lib, "GetPlugInInstance");
+    auto dispose_inst = (SYMBOL_MULTI_DISPOSE)dlsym(lib, "DisposePlugInInstance");
+    auto dispose_all = (SYMBOL_MULTI_DISPOSEALL)dlsym(lib, "DisposePlugInInstances");
+    auto held = (ContextHeldFn)dlsym(lib, "IlluminaGpuContextHeld");
+    if (!get || !dispose || !create || !get_inst || !dispose_inst || !dispose_all || !held) {
+        fprintf(stderr, "multi: a plug-in symbol is missing\n");
+        return 1;
+    }
+    int ndev = 0;
+    ilm_device_count(&ndev);
+    bool ok = true;
+    auto check = [&](bool c, const char* what) {
+        if (!c) fprintf(stderr, "multi: %s\n", what);
+        ok &= c;
+    };
+    IPlugIn* s = get();
+    check(s && s == get() && s->GetName() == "GpuAccel", "singleton");
+    if (ndev) {
+        s->Initialise();
+        check(held() == 1, "singleton holds the context after Initialise");
+    }
+    dispose();
+    check(held() == 0, "no-argument DisposePlugInSingleton releases the context");
+    const PlugInHandle h1 = create(), h2 = create();
+    IPlugIn* p1 = get_inst(h1);
+    IPlugIn* p2 = get_inst(h2);
+    check(h1 != h2 && p1 && p2 && p1 != p2, "instances are distinct");
+    check(get_inst(h1) == p1 && get_inst(h1 + h2 + 100) == nullptr, "handles name their instance");
+    EngineKernel kernel;
+    p1->Register(&kernel);
+    check(kernel.GetShapeManager().QueryFactory("GpuBVHMesh"), "an instance registers its factories");
+    p1->Unregister(&kernel);
+    if (ndev) {
+        p1->Initialise();
+        check(held() == 1, "instance holds the context");
+    }
+    dispose_inst(h1);
+    check(get_inst(h1) == nullptr && get_inst(h2) == p2, "DisposePlugInInstance disposes one instance");
+    if (ndev) check(held() == 1, "the context stays while an instance remains");
+    dispose_inst(h1);  // an unknown handle is ignored
+    dispose_all();
+    check(get_inst(h2) == nullptr, "DisposePlugInInstances disposes every instance");
+    check(held() == 0, "the context is released with the last instance");
+    IPlugIn* s2 = get();
+    check(s2 && s2->GetName() == "GpuAccel", "a new singleton after disposal");
+    dispose();
+    printf(ok ? "multi ok\n" : "multi FAILED\n");
     return ok ? 0 : 1;
 }
 
@@ -381,7 +451,7 @@ static int run_ilm_render(const char* script, const char* seed, const char* out)
 
 int main(int argc, char** argv) {
     if (argc < 3) {
-        fprintf(stderr, "usage: %s <plugin.so> host | trace|render <in> <out> <args>\n", argv[0]);
+        fprintf(stderr, "usage: %s <plugin.so> host | multi | trace|render <in> <out> <args>\n", argv[0]);
         return 2;
     }
     void* lib = dlopen(argv[1], RTLD_NOW | RTLD_LOCAL);
@@ -389,14 +459,25 @@ int main(int argc, char** argv) {
         fprintf(stderr, "dlopen: %s\n", dlerror());
         return 2;
     }
-    GetPlugInFn get = (GetPlugInFn)dlsym(lib, "GetPlugInSingleton");
-    DisposePlugInFn dispose = (DisposePlugInFn)dlsym(lib, "DisposePlugInSingleton");
+    auto get = (SYMBOL_SINGLE_GET)dlsym(lib, "GetPlugInSingleton");
+    auto dispose = (SYMBOL_SINGLE_DISPOSE)dlsym(lib, "DisposePlugInSingleton");
     if (!get || !dispose) {
         fprintf(stderr, "plug-in symbols missing\n");
         return 2;
     }
-    IPlugIn* plugin = get();
     const std::string mode = argv[2];
+    if (mode == "multi") {
+        int rc = 3;
+        try {
+            rc = run_multi(lib);
+        } catch (Exception* e) {
+            fprintf(stderr, "exception: %s\n", e->GetMessage().c_str());
+            delete e;
+        }
+        dlclose(lib);
+        return rc;
+    }
+    IPlugIn* plugin = get();
     int rc = 2;
     try {
         if (mode == "host")
@@ -418,7 +499,7 @@ int main(int argc, char** argv) {
         delete e;
         rc = 3;
     }
-    dispose(plugin);
+    dispose();
     dlclose(lib);
     return rc;
 }

@@ -39,8 +39,25 @@ def test_plugin_host_checks():
 def test_plugin_exports_entry_symbols():
     import ctypes
     lib = ctypes.CDLL(PLUGIN)
-    for s in ("GetPlugInSingleton", "DisposePlugInSingleton"):
+    # the six symbols of Core/System/LibrarySymbols.h:17-23
+    for s in ("GetPlugInSingleton", "DisposePlugInSingleton", "CreatePlugInInstance", "GetPlugInInstance",
+              "DisposePlugInInstance", "DisposePlugInInstances"):
         assert hasattr(lib, s), s
+
+
+def test_plugin_multi_instance_symbols_host():
+    # Without a device only the handle bookkeeping is checked; the context
+    # release is checked by the gpu variant below.
+    r = _run("multi")
+    assert r.returncode == 0 and "multi ok" in r.stdout
+
+
+@pytest.mark.gpu
+def test_plugin_dispose_releases_device_context(gpu_ctx):
+    # Initialise creates the shared context; the no-argument
+    # DisposePlugInSingleton and the last DisposePlugInInstance release it.
+    r = _run("multi")
+    assert r.returncode == 0 and "multi ok" in r.stdout
 
 
 def test_plugin_reports_errors_as_exceptions():
