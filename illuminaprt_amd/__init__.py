@@ -117,7 +117,7 @@ class IcStats(C.Structure):
 
 class RenderStats(C.Structure):
     _fields_ = [("camera_rays", C.c_int64), ("extension_rays", C.c_int64), ("shadow_rays", C.c_int64),
-                ("ms", C.c_double)]
+                ("ms", C.c_double), ("train_records", C.c_int64)]
 
 
 # name -> (restype, argtypes); every exported symbol of include/ilm_gpu.h

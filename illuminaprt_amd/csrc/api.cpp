@@ -38,6 +38,7 @@ int ensure_spill(ilm_ctx* ctx, size_t words) {
     }
     ILM_HIP(hipMalloc(&ctx->spill, words * sizeof(uint32_t)));
     ctx->spill_words = words;
+    ++ctx->spill_generation;
     return ILM_OK;
 }
 
@@ -174,7 +175,12 @@ int ilm_sync(ilm_ctx* ctx) {
     ILM_HIP(hipStreamSynchronize(ctx->stream));
     unsigned long long flags = 0;
     ILM_HIP(hipMemcpy(&flags, ctx->counters + 2, sizeof(flags), hipMemcpyDeviceToHost));
-    if (flags) return fail(ILM_ERR_STATE, "a traversal exceeded its stack bound; results are incomplete");
+    if (flags) {
+        // reported once: cleared for the launches that follow
+        const unsigned long long zero = 0;
+        ILM_HIP(hipMemcpy(ctx->counters + 2, &zero, sizeof(zero), hipMemcpyHostToDevice));
+        return fail(ILM_ERR_STATE, "a traversal exceeded its stack bound; results are incomplete");
+    }
     return ILM_OK;
 }
 

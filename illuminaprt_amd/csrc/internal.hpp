@@ -50,6 +50,7 @@ struct ilm_ctx {
     unsigned long long* counters = nullptr;
     uint32_t* spill = nullptr;
     size_t spill_words = 0;
+    uint64_t spill_generation = 0;  // counts reallocations of spill (graphs key on it)
     int trace_refill = 24;  // idle lanes of a wave that trigger its ray refill
     std::mutex mu;
 };

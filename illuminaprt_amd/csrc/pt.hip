@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -168,7 +169,9 @@ struct PtArgs {
     int nrc_prefix;          // depth of the vertex whose outgoing radiance the cache supplies
     int max_depth_train;     // depth limit of the training paths (prefix + suffix)
     float train_frac;        // fraction of paths extended as training paths
-    int64_t train_cap;       // training samples kept per frame
+    int64_t train_cap;       // training record slots (train_count rounded up to whole training batches)
+    int64_t total_paths;     // paths of the frame (all batches)
+    int64_t train_count;     // training paths of the frame: floor(total_paths * train_frac)
     float4* tbeta;           // throughput of a training path relative to its query vertex
     float* T;                // 3 floats per path: radiance leaving the query vertex (training paths)
     int32_t* tslot;          // training-sample slot of a path, or -1
@@ -204,6 +207,24 @@ __device__ __forceinline__ void path_pixel(const PtArgs& a, int64_t p, int& vi, 
     const int spp = a.views[vi].spp;
     pixel = a.views[vi].pix0 + local / spp;
     sample = (uint32_t)(a.frame[1] + (int)(local % spp));
+}
+
+// Training paths of the radiance cache: exactly train_count of the frame's
+// total_paths paths, spread evenly over the global path order with a
+// per-frame offset (from the frame seed): path g trains when
+// floor(((g + 1) K + off) / T) > floor((g K + off) / T), and its record goes
+// to slot floor((g K + off) / T), so every training path owns one slot and
+// the slots do not depend on scheduling.
+__device__ __forceinline__ int64_t train_offset(const PtArgs& a) {
+    const Block8 b = sample_block((uint32_t)a.frame[0], ~0ull, 0u, 0u, 15u);
+    return min((int64_t)((double)b.v[0] * (double)a.total_paths), a.total_paths - 1);
+}
+__device__ __forceinline__ bool is_training_path(const PtArgs& a, int64_t g, int64_t off) {
+    if (a.train_count <= 0) return false;
+    return ((g + 1) * a.train_count + off) / a.total_paths > (g * a.train_count + off) / a.total_paths;
+}
+__device__ __forceinline__ int64_t training_slot(const PtArgs& a, int64_t g, int64_t off) {
+    return (g * a.train_count + off) / a.total_paths;
 }
 
 __global__ void k_generate(PtArgs a) {
@@ -248,8 +269,8 @@ __global__ void k_generate(PtArgs a) {
     r.dx = d.x; r.dy = d.y; r.dz = d.z; r.tmax = FLT_MAX;
     a.rays[0][p] = r;
     a.qpath[0][p] = (int32_t)p;
-    // beta.w flags a training path of the radiance cache (sample 6 of the block)
-    float train = (a.nrc_prefix >= 0 && b.v[6] < a.train_frac) ? 1.f : 0.f;
+    // beta.w flags a training path of the radiance cache
+    float train = (a.nrc_prefix >= 0 && is_training_path(a, a.bpath0 + p, train_offset(a))) ? 1.f : 0.f;
     a.beta[p] = make_float4(1.f, 1.f, 1.f, train);
     if (a.nrc_prefix >= 0) a.tslot[p] = -1;
     a.L[3 * p] = 0.f;
@@ -370,8 +391,10 @@ __global__ void __launch_bounds__(kPtBlock) k_shade(PtArgs a, int bounce, int cu
                 a.qbeta[qs] = make_float4(beta.x, beta.y, beta.z, __int_as_float(p));
                 live = false;
             }
-            int64_t ts = block_append(&a.counts[6], new_train, s_cnt[2][par], &s_base[2][par]);
+            // counts the frame's training records; the record's slot is the path's own
+            (void)block_append(&a.counts[6], new_train, s_cnt[2][par], &s_base[2][par]);
             if (new_train) {
+                const int64_t ts = training_slot(a, a.bpath0 + p, train_offset(a));
                 if (ts < a.train_cap) {
                     float4* d = reinterpret_cast<float4*>(a.tq + 16 * ts);
                     for (int k = 0; k < 4; ++k) d[k] = make_float4(qrec[4 * k], qrec[4 * k + 1], qrec[4 * k + 2], qrec[4 * k + 3]);
@@ -627,6 +650,7 @@ struct ilm_scene {
     int state_nl = 0;
     bool cache_ready = false;  // radiance-cache buffers allocated for `cap` paths
     int64_t train_cap = 0;
+    uint64_t train_round = 0;  // Adam steps taken by cache frames (picks the next training batch)
     std::vector<void*> bufs;
     ilm::PtArgs a{};
     // frame constants (device) and their host copy
@@ -845,14 +869,15 @@ int render_views(ilm_scene* s, ilm_nrc* nrc, const std::vector<ilm::View>& views
     if (views_in.empty() || p.max_depth <= 0) return fail(ILM_ERR_INVALID, "no views or max_depth not positive");
     const bool cache = nrc != nullptr;
     int prefix = -1, max_depth_train = p.max_depth, train_steps = 0;
-    int64_t train_cap = 0;
+    int64_t train_cap = 0, train_batch = 0, train_count = 0;
     float train_frac = 0.f;
     if (cache) {
         if (!np) return fail(ILM_ERR_INVALID, "cache rendering needs ilm_nrc_render_params");
         prefix = np->prefix_depth > 0 ? np->prefix_depth : 2;
         max_depth_train = prefix + (np->suffix_depth > 0 ? np->suffix_depth : 8);
         train_frac = np->train_fraction > 0 ? np->train_fraction : 1.f / 16.f;
-        train_cap = np->train_batch > 0 ? np->train_batch : 65536;
+        train_batch = np->train_batch > 0 ? np->train_batch : 65536;
+        if (train_frac > 1.f) return fail(ILM_ERR_INVALID, "train_fraction above 1");
         train_steps = np->train_steps >= 0 ? np->train_steps : 1;
     }
     ilm_ctx* ctx = s->ctx;
@@ -871,10 +896,14 @@ int render_views(ilm_scene* s, ilm_nrc* nrc, const std::vector<ilm::View>& views
         max_spp = std::max(max_spp, v.spp);
         max_vl = std::max(max_vl, v.l1 - v.l0);
     }
-    // extend no more training paths than the batch keeps: the records beyond
-    // train_cap would be dropped, and which ones are kept would follow the
-    // order of the queue appends
-    if (cache && total_paths > 0) train_frac = std::min(train_frac, (float)((double)train_cap / (double)total_paths));
+    // exactly floor(total * fraction) training paths, each with its own record
+    // slot; the slots are grouped into training batches of train_batch records
+    if (cache) {
+        train_count = (int64_t)std::floor((double)total_paths * (double)train_frac);
+        const int64_t nb = std::max<int64_t>(1, (train_count + train_batch - 1) / train_batch);
+        train_cap = nb * train_batch;
+        if (train_cap > (int64_t(1) << 31) - 1) return fail(ILM_ERR_INVALID, "too many training records");
+    }
     const int64_t want = std::max<int64_t>(p.batch_paths > 0 ? p.batch_paths : (int64_t(1) << 22), max_spp);
     struct Batch { int64_t pix0, npix, path0, npaths; };
     std::vector<Batch> batches;
@@ -917,6 +946,8 @@ int render_views(ilm_scene* s, ilm_nrc* nrc, const std::vector<ilm::View>& views
     a.max_depth_train = max_depth_train;
     a.train_frac = train_frac;
     a.train_cap = train_cap;
+    a.total_paths = total_paths;
+    a.train_count = train_count;
     const bool do_accum = !cache || np->accumulate;
     const int max_bounces = cache ? max_depth_train : p.max_depth;
     a.shade = s->d_shade;
@@ -1005,17 +1036,25 @@ int render_views(ilm_scene* s, ilm_nrc* nrc, const std::vector<ilm::View>& views
         rc = launch_frame();
         if (rc) return rc;
     } else {
-        // everything the captured launches bake in
-        std::vector<int64_t> key = {
-            (int64_t)(uintptr_t)nrc, (int64_t)(uintptr_t)accum, p.max_depth, p.rr_depth,
-            (int64_t)ilm::float_bits(p.eps), cap, do_accum, prefix, max_depth_train, (int64_t)ilm::float_bits(train_frac),
-            train_cap, (int64_t)(uintptr_t)s->mesh->d_nodes, s->mesh->node_format, want, (int64_t)views.size()};
-        for (const ilm::View& v : views) {
-            key.push_back(v.W);
-            key.push_back(v.H);
-            key.push_back(v.spp);
-        }
-        if (s->graph && s->graph_key == key) {
+        // everything the captured launches bake in, including the context's
+        // traversal spill area (a later launch that needs more reallocates it)
+        // and the mesh's device arrays and stack bound
+        auto make_key = [&]() {
+            std::vector<int64_t> key = {
+                (int64_t)(uintptr_t)nrc, (int64_t)(uintptr_t)accum, p.max_depth, p.rr_depth,
+                (int64_t)ilm::float_bits(p.eps), cap, do_accum, prefix, max_depth_train,
+                (int64_t)ilm::float_bits(train_frac), train_cap, (int64_t)(uintptr_t)s->mesh->d_nodes,
+                (int64_t)(uintptr_t)s->mesh->d_tris, s->mesh->node_format, s->mesh->width, s->mesh->info.stack_need,
+                (int64_t)(uintptr_t)ctx->spill, (int64_t)ctx->spill_words, (int64_t)ctx->spill_generation,
+                ctx->trace_refill, want, (int64_t)views.size()};
+            for (const ilm::View& v : views) {
+                key.push_back(v.W);
+                key.push_back(v.H);
+                key.push_back(v.spp);
+            }
+            return key;
+        };
+        if (s->graph && s->graph_key == make_key()) {
             ILM_HIP(hipGraphLaunch(s->graph, st));
         } else {
             // first frame with these settings: run it, then capture the same
@@ -1038,12 +1077,19 @@ int render_views(ilm_scene* s, ilm_nrc* nrc, const std::vector<ilm::View>& views
                 s->graph = nullptr;
                 return ilm::fail_hip(e, "frame graph instantiation");
             }
-            s->graph_key = key;
+            s->graph_key = make_key();
         }
     }
-    for (int k = 0; k < train_steps; ++k) {
-        rc = ilm_nrc_train_step(nrc, a.tq, reinterpret_cast<const float*>(a.tt), train_cap, nullptr, st);
-        if (rc) return rc;
+    // Adam steps of train_batch records each; successive steps (across
+    // frames) take the frame's batches in turn
+    if (cache) {
+        const int64_t nb = train_cap / train_batch;
+        for (int k = 0; k < train_steps; ++k) {
+            const int64_t bi = (int64_t)(s->train_round++ % (uint64_t)nb);
+            rc = ilm_nrc_train_step(nrc, a.tq + 16 * bi * train_batch,
+                                    reinterpret_cast<const float*>(a.tt + bi * train_batch), train_batch, nullptr, st);
+            if (rc) return rc;
+        }
     }
     rc = ilm_timer_mark(ctx, &ev1);
     if (rc) return rc;
@@ -1057,6 +1103,7 @@ int render_views(ilm_scene* s, ilm_nrc* nrc, const std::vector<ilm::View>& views
         stats->extension_rays = (int64_t)cnt[3] - stats->camera_rays;
         stats->shadow_rays = (int64_t)cnt[4];
         stats->ms = ms;
+        stats->train_records = cache ? (int64_t)cnt[6] : 0;
     }
     return ilm_sync(ctx);
 }

@@ -373,6 +373,8 @@ typedef struct {
 typedef struct {
     int64_t camera_rays, extension_rays, shadow_rays;
     double ms;            /* GPU time of the call (events on the stream) */
+    int64_t train_records; /* cache frames: training records made (training paths that reached
+                              the query vertex); 0 otherwise */
 } ilm_render_stats;
 
 /* Renders spp samples per pixel with the wavefront path integrator and
@@ -611,20 +613,23 @@ int ilm_nrc_train_step(ilm_nrc* nrc, const float* queries, const float* targets,
 typedef struct {
     int32_t prefix_depth;   /* depth of the vertex answered by the cache (default 2, >= 1) */
     int32_t suffix_depth;   /* further bounces of training paths (default 8) */
-    float train_fraction;   /* largest fraction of paths extended as training paths (default
-                               1/16); lowered to train_batch / (the frame's paths) when that is
-                               smaller, so that every training path's record is kept */
-    int32_t train_batch;    /* training samples kept per frame (default 65536) */
-    int32_t train_steps;    /* Adam steps per frame (0 = no training, negative = 1) */
+    float train_fraction;   /* fraction of the frame's paths extended as training paths (default
+                               1/16): exactly floor(paths * fraction) of them, spread evenly over
+                               the path order with a per-frame offset from the seed; every
+                               training path's record is kept, in a slot of its own */
+    int32_t train_batch;    /* samples per Adam step (default 65536); the frame's records form
+                               ceil(records / train_batch) batches */
+    int32_t train_steps;    /* Adam steps per frame (0 = no training, negative = 1); successive
+                               steps, also across frames, take the batches in turn */
     int32_t accumulate;     /* add this frame's radiance to accum */
 } ilm_nrc_render_params;
 
 /* Renders like ilm_render, but paths that are not training paths stop at
  * prefix_depth and take their remaining radiance from the cache (batched
  * inference at the end of each wavefront batch); training paths (a
- * counter-based fraction) are traced prefix_depth + suffix_depth deep, and the
+ * fixed fraction) are traced prefix_depth + suffix_depth deep, and the
  * radiance they measure leaving their query vertex trains the cache
- * (train_steps Adam steps on train_batch samples after the frame). The
+ * (train_steps Adam steps on train_batch samples each after the frame). The
  * image adds the training paths' own estimates and the cache's answers.
  * Synchronous; accum may be NULL when accumulate is 0. */
 int ilm_render_nrc(ilm_scene* scene, ilm_nrc* nrc, const ilm_render_params* params,

@@ -61,6 +61,8 @@ def lib():
     L.orc_cosine_hemisphere.argtypes = [C.c_float, C.c_float, _f32p]
     L.orc_uniform_triangle.argtypes = [C.c_float, C.c_float, fp3, fp3]
     L.orc_camera_ray.argtypes = [_f32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, _f32p, _f32p]
+    L.orc_bsdf_eval.argtypes = [_f32p, C.c_int64, _f32p, _f32p, _f32p, _f32p]
+    L.orc_bsdf_sample.argtypes = [_f32p, C.c_int64, _f32p, _f32p, _f32p, _f32p, _f32p]
     L.orc_mesh_create.argtypes = [_f32p, C.c_int64, _i32p, C.c_int64]
     L.orc_mesh_create.restype = C.c_void_p
     L.orc_mesh_destroy.argtypes = [C.c_void_p]
@@ -141,6 +143,33 @@ def camera_ray(cam, w, h, px, py, u, v):
     d = np.zeros(3, np.float32)
     lib().orc_camera_ray(np.ascontiguousarray(cam, np.float32), w, h, px, py, u, v, o, d)
     return o, d
+
+
+def _mat8(kd, ks, exponent, kind):
+    return np.array([*kd, *ks, exponent, kind], np.float32)
+
+
+def bsdf_eval(kd, ks, exponent, kind, wo, wi):
+    """f (n x 3) and SampleF's pdf (n) of a material in its local frame."""
+    wo = np.ascontiguousarray(wo, np.float32).reshape(-1, 3)
+    wi = np.ascontiguousarray(wi, np.float32).reshape(-1, 3)
+    n = len(wo)
+    f = np.zeros((n, 3), np.float32)
+    pdf = np.zeros(n, np.float32)
+    lib().orc_bsdf_eval(_mat8(kd, ks, exponent, kind), n, wo, wi, f, pdf)
+    return f, pdf
+
+
+def bsdf_sample(kd, ks, exponent, kind, wo, uvw):
+    """BSDF::SampleF for n outgoing directions: wi (n x 3), f (n x 3), pdf (n)."""
+    wo = np.ascontiguousarray(wo, np.float32).reshape(-1, 3)
+    uvw = np.ascontiguousarray(uvw, np.float32).reshape(-1, 3)
+    n = len(wo)
+    wi = np.zeros((n, 3), np.float32)
+    f = np.zeros((n, 3), np.float32)
+    pdf = np.zeros(n, np.float32)
+    lib().orc_bsdf_sample(_mat8(kd, ks, exponent, kind), n, wo, uvw, wi, f, pdf)
+    return wi, f, pdf
 
 
 class Mesh:

@@ -1251,4 +1251,34 @@ int orc_render(const orc_scene* s, const orc_render_params* p, double* accum, in
     return nt;
 }
 
+
+// Local-frame BSDF of a material record (kd rgb, ks rgb, exponent, type):
+// f, and the pdf BSDF::SampleF reports for a direction it sampled
+// (BSDF.cpp:141-211: the chosen lobe's pdf plus the other lobe's Pdf,
+// divided by the lobe count; LambertianBxDF.cpp:52-55).
+void orc_bsdf_eval(const float mat[8], int64_t n, const float* wo, const float* wi, float* f, float* pdf) {
+    orc_scene::Mat m{V3(mat[0], mat[1], mat[2]), V3(mat[3], mat[4], mat[5]), mat[6], (int)mat[7]};
+    for (int64_t k = 0; k < n; ++k) {
+        V3 o(wo[3 * k], wo[3 * k + 1], wo[3 * k + 2]), i(wi[3 * k], wi[3 * k + 1], wi[3 * k + 2]);
+        V3 v = bsdf_f(m, o, i);
+        f[3 * k] = v.x; f[3 * k + 1] = v.y; f[3 * k + 2] = v.z;
+        float pl = std::fabs(i.z) * kInvPi;
+        pdf[k] = m.type == 1 ? 0.5f * (pl + phong_pdf(m, o, i)) : pl;
+    }
+}
+
+// BSDF::SampleF over n (wo, u, v, w): wi, f and pdf as the integrator uses them.
+void orc_bsdf_sample(const float mat[8], int64_t n, const float* wo, const float* uvw, float* wi, float* f,
+                     float* pdf) {
+    orc_scene::Mat m{V3(mat[0], mat[1], mat[2]), V3(mat[3], mat[4], mat[5]), mat[6], (int)mat[7]};
+    for (int64_t k = 0; k < n; ++k) {
+        V3 o(wo[3 * k], wo[3 * k + 1], wo[3 * k + 2]), i;
+        float pd = 0.f;
+        V3 v = bsdf_sample(m, o, i, uvw[3 * k], uvw[3 * k + 1], uvw[3 * k + 2], pd);
+        wi[3 * k] = i.x; wi[3 * k + 1] = i.y; wi[3 * k + 2] = i.z;
+        f[3 * k] = v.x; f[3 * k + 1] = v.y; f[3 * k + 2] = v.z;
+        pdf[k] = pd;
+    }
+}
+
 }  // extern "C"

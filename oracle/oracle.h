@@ -156,6 +156,20 @@ typedef struct {
 int orc_render(const orc_scene* s, const orc_render_params* p, double* accum,
                int64_t ray_counts[2], int nthreads);
 
+/* ---- materials -------------------------------------------------------- */
+
+/* The BSDF of a material record (8 floats: kd rgb, ks rgb, Phong exponent,
+ * type 0 Lambertian / 1 Lambertian + Phong) in its local frame (z = normal):
+ * f(wo, wi) as BSDF::F (Core/Material/BSDF.cpp:215-237, reflection only) and
+ * the pdf BSDF::SampleF reports for a direction it sampled (BSDF.cpp:141-211,
+ * LambertianBxDF.cpp:27-55). The Phong lobe (normalised modified Phong,
+ * f = ks (n+2)/(2 pi) cos^n a about the mirror direction, sampled by cos^n)
+ * has no reference counterpart. n directions each. */
+void orc_bsdf_eval(const float mat[8], int64_t n, const float* wo, const float* wi, float* f, float* pdf);
+/* BSDF::SampleF for n outgoing directions and (u, v, w) triples: wi, f, pdf. */
+void orc_bsdf_sample(const float mat[8], int64_t n, const float* wo, const float* uvw, float* wi, float* f,
+                     float* pdf);
+
 #ifdef __cplusplus
 }
 #endif

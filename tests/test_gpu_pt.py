@@ -124,6 +124,24 @@ def test_graph_replays_match_eager(gpu_ctx, ilm):
             other.zero()
             sc2.render(cam, W, H, 2, other, seed=99 + rep)
             gpu_ctx.sync()
+    # a deeper tree traced on the same context grows the context's traversal
+    # spill area (freeing the one the captured frame used): the next replay
+    # must not use the freed area, and still equals the eager frame
+    dpos, didx = meshgen.icosphere(6)
+    deep = ilm.Mesh(gpu_ctx, dpos, didx)
+    deep.build(builder=ilm.ILM_BUILDER_REFERENCE, max_leaf=1, max_depth=60)
+    drays = meshgen.rays_toward(50000, dpos, seed=4)
+    drb = ilm.rays_to_device(gpu_ctx, drays)
+    dhb = gpu_ctx.buffer(len(drays) * 16)
+    deep.trace_closest(drb, len(drays), dhb)
+    gpu_ctx.sync()
+    for sd in seeds[:2]:
+        acc.zero()
+        sc.render(cam, W, H, 4, acc, seed=sd, spp_offset=sd)
+        assert np.array_equal(acc.download(np.float64, W * H * 3), eager[sd]), ("after spill growth", sd)
+    for b in (drb, dhb):
+        b.free()
+    deep.close()
     # cache renderer, no training: replays equal eager frames
     net = ilm.Nrc(gpu_ctx, (0, 0, 0), (1, 1, 1), seed=3)
     ne = {}
