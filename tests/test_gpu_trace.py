@@ -2,9 +2,11 @@
 
 Bar (closest hit): the GPU hit index equals the oracle's on >= 99.99% of rays,
 where a differing index counts as agreeing only at a near-tie, i.e. both hit
-and |t_gpu - t_cpu| <= 1e-6 * t (the agreement tolerance of this path) or
-within the reference's own 1e-5 absolute slop (IndexedTriangle.cpp:114-115,
-under which the first triangle found wins). Any-hit: occlusion bits equal on
+and |t_gpu - t_cpu| <= 1e-6 * t (the agreement tolerance of this path). On
+the small test meshes the reference's own 1e-5 absolute slop
+(IndexedTriangle.cpp:114-115, under which the first triangle found wins) also
+counts as a tie; the 1.33M-triangle sets are held to the 1e-6 relative rule
+alone on all of their rays, and report the rest by cause. Any-hit: occlusion bits equal on
 >= 99.99% of rays. The oracle is the C++ restatement of the reference's
 triangle test and BVHMesh / BasicMesh traversal.
 """
@@ -262,14 +264,51 @@ def c2(gpu_ctx, ilm):
     m.close()
 
 
-def _sample(n, k, seed):
-    return np.sort(np.random.default_rng(seed).choice(n, k, replace=False))
+@pytest.fixture(scope="module")
+def c2_oracle(c2, orc):
+    """The oracle's BVHMesh over the 1,334,720 triangles (reference topology:
+    128-bin count heuristic, leaves of at most 10, depth 35)."""
+    pos, idx, _, _ = c2
+    om = orc.Mesh(pos, idx)
+    om.build(10, 35)
+    return om
+
+
+def _classify_disagreements(orc, pos, idx, rays, ph, th, pc, tc):
+    """Rays whose index differs and are not near-ties under the 1e-6 relative
+    rule, by cause: 'slop' both hit within the reference's 1e-5 absolute
+    window (IndexedTriangle.cpp:114-115: a later triangle within 1e-5 of the
+    current hit cannot replace it, so the first one found wins); 'leak' the
+    GPU's triangle is one the reference's Shirley test misses for this ray
+    (a ray through a shared edge or vertex that the Shirley test lets
+    through, while the watertight test counts it); 'other' the rest."""
+    both = (ph >= 0) & (pc >= 0)
+    dt = np.abs(th.astype(np.float64) - tc.astype(np.float64))
+    ns = (ph == pc) | (both & (dt <= 1e-6 * np.maximum(np.abs(tc.astype(np.float64)), 1e-30)))
+    out = {"slop": 0, "leak": 0, "other": 0}
+    others = []
+    for i in np.nonzero(~ns)[0]:
+        if both[i] and dt[i] <= 1e-5:
+            out["slop"] += 1
+            continue
+        if ph[i] >= 0:
+            tri = pos[idx[ph[i]]]
+            hit = orc.tri_intersect(tri[0], tri[1], tri[2], rays[i, 0:3], rays[i, 4:7], rays[i, 7])
+            if not hit[0]:
+                out["leak"] += 1
+                continue
+        out["other"] += 1
+        others.append(int(i))
+    return ns.mean(), out, others
 
 
 @pytest.mark.parametrize("kind", ["primary", "incoherent"])
-def test_c2_closest_matches_oracle_bvhmesh(c2, gpu_ctx, ilm, orc, kind):
+def test_c2_closest_matches_oracle_bvhmesh(c2, c2_oracle, gpu_ctx, ilm, orc, kind):
     """1,334,720 triangles: GPU closest hit vs the oracle's BVHMesh (reference
-    topology, unordered recursive traversal) on a seeded sample of the ray set."""
+    topology, unordered recursive traversal) on every one of the 4,194,304
+    rays of the set. The bar: index equal, or both hit with
+    |t_gpu - t_cpu| <= 1e-6 t, on >= 99.99% of rays; the rest are counted by
+    cause."""
     pos, idx, m, rs = c2
     buf = rs.primary if kind == "primary" else rs.incoherent
     n = rs.n
@@ -280,11 +319,13 @@ def test_c2_closest_matches_oracle_bvhmesh(c2, gpu_ctx, ilm, orc, kind):
     rays = buf.download(np.float32, 8 * n).reshape(n, 8)
     if kind == "primary":
         assert (h["prim"] >= 0).all()   # the primary set is made of hitting rays
-    sel = _sample(n, 200000, 5)
-    om = orc.Mesh(pos, idx)
-    om.build(10, 35)
-    pc, tc = om.trace(rays[sel], mode=0, accel=1)
-    assert _compare_closest(h["prim"][sel], h["t"][sel], pc, tc, f"c2 {kind}") >= 0.9999
+    pc, tc = c2_oracle.trace(rays, mode=0, accel=1)
+    frac, cls, others = _classify_disagreements(orc, pos, idx, rays, h["prim"], h["t"], pc, tc)
+    loose = _compare_closest(h["prim"], h["t"], pc, tc, f"c2 {kind} (all rays)")
+    print(f"[c2 {kind}] 1e-6-relative rule: {frac:.7f}; the rest by cause {cls}", flush=True)
+    for i in others[:5]:
+        print(f"   other ray {i}: gpu ({h['prim'][i]}, {h['t'][i]!r}) cpu ({pc[i]}, {tc[i]!r})", flush=True)
+    assert frac >= 0.9999 and loose >= frac
     hb.free()
 
 
@@ -326,7 +367,7 @@ def test_c2_lbvh_matches_sah(c2, gpu_ctx, ilm):
     ml.close()
 
 
-def test_c2_shadow_matches_oracle_and_closest(c2, gpu_ctx, ilm, orc):
+def test_c2_shadow_matches_oracle_and_closest(c2, c2_oracle, gpu_ctx, ilm, orc):
     pos, idx, m, rs = c2
     n = rs.n
     ob = gpu_ctx.buffer(n)
@@ -343,12 +384,18 @@ def test_c2_shadow_matches_oracle_and_closest(c2, gpu_ctx, ilm, orc):
     end = rays[:, 0:3] + rays[:, 4:7]
     assert np.abs(end[:, 1] - 2.5).max() < 1e-5
     assert (np.abs(end[:, 0]) <= 0.25 + 1e-5).all() and (np.abs(end[:, 2]) <= 0.25 + 1e-5).all()
-    sel = _sample(n, 200000, 6)
-    om = orc.Mesh(pos, idx)
-    om.build(10, 35)
-    pc, _ = om.trace(rays[sel], mode=1, accel=1)
-    agree = (occ[sel].astype(bool) == (pc >= 0)).mean()
-    print(f"[c2 shadow] agree={agree:.6f} occluded={occ.mean():.4f}", flush=True)
+    # every one of the 4,194,304 segments against the oracle's any hit
+    pc, _ = c2_oracle.trace(rays, mode=1, accel=1)
+    diff = occ.astype(bool) != (pc >= 0)
+    agree = 1.0 - diff.mean()
+    # a differing segment: does the oracle's Shirley test miss the triangle the GPU found?
+    leak = 0
+    for i in np.nonzero(diff & (h["prim"] >= 0))[0]:
+        tri = pos[idx[h["prim"][i]]]
+        leak += not orc.tri_intersect(tri[0], tri[1], tri[2], rays[i, 0:3], rays[i, 4:7], rays[i, 7])[0]
+    print(f"[c2 shadow] all {n} segments: agree={agree:.7f} occluded={occ.mean():.4f} differing={int(diff.sum())} "
+          f"(gpu occluded, cpu not: {int((diff & occ.astype(bool)).sum())}, of which the oracle's test misses "
+          f"the gpu's triangle: {leak}; cpu occluded, gpu not: {int((diff & ~occ.astype(bool)).sum())})", flush=True)
     assert agree >= 0.9999
     ob.free()
     hb.free()
