@@ -147,6 +147,7 @@ SIGNATURES = {
     "ilm_trace_closest": (C.c_int, [_vp, _vp, _i64, _vp, _vp]),
     "ilm_trace_any": (C.c_int, [_vp, _vp, _i64, _vp, _vp]),
     "ilm_trace_stats": (C.c_int, [_vp, _vp, _i64, C.c_int, _vp, _vp]),
+    "ilm_bvh_visit_counts": (C.c_int, [_vp, _vp, _i64, C.c_int, _vp, C.c_int]),
     "ilm_raygen_camera": (C.c_int, [_vp, C.POINTER(Camera), C.c_int, C.c_int, C.c_int, _u32, _u32, _vp, _vp]),
     "ilm_raygen_camera_list": (C.c_int, [_vp, C.POINTER(Camera), C.c_int, C.c_int, _vp, _i64, _i64, _u32, _vp, _vp]),
     "ilm_raygen_secondary": (C.c_int, [_vp, _vp, _vp, _vp, _i64, C.c_int, _vp, _f32, _u32, _vp, _vp]),
@@ -386,6 +387,14 @@ class Mesh:
 
     def trace_any(self, rays: DeviceBuffer, n: int, occ: DeviceBuffer):
         check(load().ilm_trace_any(self.h, rays.ptr, n, occ.ptr, None))
+
+    def visit_counts(self, rays: np.ndarray, any_hit: bool = False, threads: int = 0) -> np.ndarray:
+        """Node visits and triangle tests per ray (n x 2 int32) of the ordered,
+        non-speculative traversal of this tree, counted on the host."""
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        out = np.zeros((len(rays), 2), np.int32)
+        check(load().ilm_bvh_visit_counts(self.h, _ptr(rays), len(rays), int(any_hit), _ptr(out), threads))
+        return out
 
     def trace_stats(self, rays: DeviceBuffer, n: int, any_hit: bool, visits: DeviceBuffer):
         check(load().ilm_trace_stats(self.h, rays.ptr, n, int(any_hit), visits.ptr, None))

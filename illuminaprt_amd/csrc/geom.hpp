@@ -210,6 +210,30 @@ ILM_HD float slab(const WideNode<W>& n, int i, const RayState& r) {
     return (tn <= tf * 1.00000048f) ? tn : INFINITY;
 }
 
+ILM_HD void cswap(float& da, float& db, uint32_t& ca, uint32_t& cb) {
+    if (db < da) {
+        float td = da; da = db; db = td;
+        uint32_t tc = ca; ca = cb; cb = tc;
+    }
+}
+
+// Sorts children by entry distance, nearest first (comparator networks:
+// 1, 5 and Batcher's 19 exchanges).
+template <int W>
+ILM_HD void sort_children(float (&d)[W], uint32_t (&c)[W]) {
+    if (W == 2) {
+        cswap(d[0], d[1], c[0], c[1]);
+    } else if (W == 4) {
+        cswap(d[0], d[1], c[0], c[1]); cswap(d[2], d[3], c[2], c[3]); cswap(d[0], d[2], c[0], c[2]);
+        cswap(d[1], d[3], c[1], c[3]); cswap(d[1], d[2], c[1], c[2]);
+    } else {
+        constexpr int net[19][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {0, 2}, {1, 3}, {4, 6}, {5, 7}, {1, 2}, {5, 6},
+                                    {0, 4}, {3, 7}, {1, 5}, {2, 6}, {1, 4}, {3, 6}, {2, 4}, {3, 5}, {3, 4}};
+#pragma unroll
+        for (int k = 0; k < 19; ++k) cswap(d[net[k][0]], d[net[k][1]], c[net[k][0]], c[net[k][1]]);
+    }
+}
+
 // ------------------------------------------------------------------ triangle test
 // Watertight ray/triangle test (Woop, Benthin, Wald 2013). The edge functions
 // are evaluated without fused multiply-add so that the two triangles sharing

@@ -195,6 +195,18 @@ int ilm_trace_closest(ilm_mesh* mesh, const ilm_ray* rays, int64_t n, ilm_hit* h
  * (Scene/Visibility.cpp:33-37). */
 int ilm_trace_any(ilm_mesh* mesh, const ilm_ray* rays, int64_t n, uint8_t* occluded, void* stream);
 
+/* Host-side traversal statistics (no GPU needed, no hits produced): per ray
+ * (visits, host, 2 int32 each) the node visits and triangle tests of a
+ * deterministic ordered traversal of the mesh's flattened float tree, the
+ * traversal kernels' without speculation (nearest child first by the
+ * kernels' exchange network, every child box tested, a closest hit shrinks
+ * the window, an any hit ends the ray). rays are host memory. threads 0 =
+ * all host cores. Prices a launch by algorithmic bytes per ray
+ * (32 + 16 + visits * node bytes + tests * 48); the measured counterpart of
+ * the speculative kernels is ilm_trace_stats. No reference counterpart
+ * (the reference's BVHMesh traversal is unordered, BVHMesh.cpp:169-198). */
+int ilm_bvh_visit_counts(ilm_mesh* mesh, const ilm_ray* rays, int64_t n, int any_hit, int32_t* visits,
+                         int threads);
 /* Same traversal with per-ray counters: visits[2*i] = wide nodes visited,
  * visits[2*i+1] = triangles tested. Used to state algorithmic bytes. */
 int ilm_trace_stats(ilm_mesh* mesh, const ilm_ray* rays, int64_t n, int any_hit,

@@ -248,3 +248,90 @@ def test_mesh_without_context_cannot_trace(ilm):
     rc = ilm.load().ilm_trace_closest(m.h, ctypes.c_void_p(16), 1, ctypes.c_void_p(16), None)
     assert rc == -4 and b"device context" in ilm.load().ilm_last_error()
     m.close()
+
+
+def _py_ordered_counts(pos, idx, boxes, child, order, ray, any_hit):
+    """Ordered traversal of an exported BVH4 in float64 (slab entry distances,
+    nearest child first, ties kept in slot order; Moller-Trumbore hits with
+    the reference's 1e-5 window), counting node visits and triangle tests."""
+    o, d, tmax = ray[0:3].astype(np.float64), ray[4:7].astype(np.float64), float(ray[7])
+    with np.errstate(divide="ignore", invalid="ignore"):
+        inv = 1.0 / np.where(d == 0, 1e-18, d)
+    lo_t, hi_t = 1e-5, tmax - 1e-5
+    stack, cur, nn, nt = [], 0, 0, 0
+    while True:
+        if not cur & 0x80000000:
+            nn += 1
+            bx = boxes[cur].astype(np.float64)
+            t0 = (bx[:, 0:3] - o) * inv
+            t1 = (bx[:, 3:6] - o) * inv
+            tn = np.maximum(np.minimum(t0, t1).max(axis=1), lo_t)
+            tf = np.minimum(np.maximum(t0, t1).min(axis=1), hi_t)
+            dist = np.where((tn <= tf) & (child[cur] != 0x80000000), tn, np.inf)
+            ordk = sorted(range(4), key=lambda k: dist[k])
+            for k in ordk[:0:-1]:
+                if np.isfinite(dist[k]):
+                    stack.append(int(child[cur][k]))
+            if np.isfinite(dist[ordk[0]]):
+                cur = int(child[cur][ordk[0]])
+                continue
+        else:
+            first, cnt = (cur & 0x7FFFFFFF) >> 5, cur & 31
+            for s in range(first, first + cnt):
+                nt += 1
+                p0, p1, p2 = (pos[i].astype(np.float64) for i in idx[order[s]])
+                e1, e2 = p1 - p0, p2 - p0
+                pv = np.cross(d, e2)
+                det = e1 @ pv
+                if det == 0:
+                    continue
+                tv = o - p0
+                u = (tv @ pv) / det
+                qv = np.cross(tv, e1)
+                v = (d @ qv) / det
+                t = (e2 @ qv) / det
+                if u >= 0 and v >= 0 and u + v <= 1 and lo_t <= t <= hi_t:
+                    hi_t = t - 1e-5
+                    if any_hit:
+                        return nn, nt
+        if not stack:
+            return nn, nt
+        cur = stack.pop()
+
+
+def test_visit_counts_of_the_ordered_traversal(ilm):
+    """ilm_bvh_visit_counts (host): equal to an independent float64 ordered
+    traversal of the exported tree on most rays (the two round box and hit
+    distances differently, which can move a cut by a node), and bounded by
+    the structure: a ray that misses the root's children visits one node and
+    tests nothing; an any-hit query never does more work than the closest-hit
+    query of the same ray."""
+    sp, si = meshgen.icosphere(3)
+    tp, ti = meshgen.random_triangles(300, seed=4, extent=1.2)
+    pos, idx = meshgen.merge([(sp, si), (tp, ti)])
+    m = ilm.Mesh(None, pos, idx)
+    m.build()
+    nodes, order = m.export()
+    boxes, child = ilm.decode_bvh(nodes, 4)
+    rays = meshgen.rays_toward(400, pos, seed=8)
+    miss = rays[:20].copy()
+    miss[:, 0:3] = (10.0, 10.0, 10.0)
+    miss[:, 4:7] = (0.0, 1.0, 0.0)                     # away from the mesh
+    rays = np.concatenate([rays, miss])
+    c = m.visit_counts(rays)
+    a = m.visit_counts(rays, any_hit=True)
+    assert (c[-20:] == [1, 0]).all()
+    assert (a <= c).all()
+    py = np.array([_py_ordered_counts(pos, idx, boxes, child, order, r, False) for r in rays])
+    eq = (py == c).all(axis=1).mean()
+    print(f"visit counts: host {c.mean(axis=0)} vs float64 restatement {py.mean(axis=0)}, equal on {eq:.4f}")
+    assert eq >= 0.97
+    np.testing.assert_allclose(c.mean(axis=0), py.mean(axis=0), rtol=0.01)
+    # any-hit counts against the float64 any-hit traversal
+    pya = np.array([_py_ordered_counts(pos, idx, boxes, child, order, r, True) for r in rays])
+    assert (pya == a).all(axis=1).mean() >= 0.97
+    with pytest.raises(ilm.IlmError):
+        mq = ilm.Mesh(None, pos, idx)
+        mq.build(node_format=ilm.ILM_NODES_QUANTIZED)
+        mq.visit_counts(rays)
+    m.close()
