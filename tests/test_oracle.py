@@ -126,6 +126,45 @@ def test_aabb_known_answers(orc):
     assert hit and a == 1.0 and b == 1.5  # exits through x = 1 at t = 1.5
 
 
+def zero_direction_fans():
+    """Two fans of 16 thin triangles in the plane z = 1, one on each side of
+    x = 0, sharing edges on that plane, and 16 rays along the plane
+    (origin x = 0, direction (0, 0, 1)) that hit those edges at t = 2."""
+    P, idx = [], []
+    for s in (-1.0, 1.0):
+        for k in range(16):
+            y0, y1 = k / 64, (k + 1) / 64
+            b = len(P)
+            P += [[0.0, y0, 1.0], [s * 0.5, y0, 1.0], [0.0, y1, 1.0]]
+            idx.append([b, b + 1, b + 2])
+    rays = np.zeros((16, 8), np.float32)
+    rays[:, 1] = (np.arange(16) + 0.3) / 64
+    rays[:, 2] = -1.0
+    rays[:, 6] = 1.0
+    rays[:, 7] = F32MAX
+    return np.array(P, np.float32), np.array(idx, np.int32), rays
+
+
+def test_bvhmesh_loses_hits_along_a_box_face(orc):
+    """The reference's BVHMesh misses what its BasicMesh hits when a ray with
+    a zero direction component runs in the plane of a node's box face: the
+    first split of the fans is the plane x = 0, both children's boxes have a
+    face there, and (0 - 0) * inf = NaN makes the slab test reject both
+    (BoundingBox.cpp:212-232, Maths.h:22-30). Moving the rays off the plane
+    by 1e-3 restores the hits."""
+    P, idx, rays = zero_direction_fans()
+    m = orc.Mesh(P, idx)
+    m.build(10, 35)
+    pb, tb = m.trace(rays, 0, 0)
+    assert (pb >= 0).all() and (pb < 16).all() and (tb == 2.0).all()   # BasicMesh: the first fan (index order)
+    pv, _ = m.trace(rays, 0, 1)
+    assert (pv == -1).all()                                             # BVHMesh: lost
+    off = rays.copy()
+    off[:, 0] = 1e-3
+    po, to = m.trace(off, 0, 1)
+    assert (po >= 16).all() and (to == 2.0).all()
+
+
 # ---------------------------------------------------------------- warps
 def test_cosine_hemisphere_moments(orc):
     rng = np.random.default_rng(0)
