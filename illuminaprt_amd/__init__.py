@@ -146,6 +146,7 @@ SIGNATURES = {
     "ilm_bvh_export": (C.c_int, [_vp, _vp, _vp]),
     "ilm_trace_closest": (C.c_int, [_vp, _vp, _i64, _vp, _vp]),
     "ilm_trace_any": (C.c_int, [_vp, _vp, _i64, _vp, _vp]),
+    "ilm_trace_closest_noalloc": (C.c_int, [_vp, _vp, _i64, _vp, _vp]),
     "ilm_trace_stats": (C.c_int, [_vp, _vp, _i64, C.c_int, _vp, _vp]),
     "ilm_bvh_visit_counts": (C.c_int, [_vp, _vp, _i64, C.c_int, _vp, C.c_int]),
     "ilm_raygen_camera": (C.c_int, [_vp, C.POINTER(Camera), C.c_int, C.c_int, C.c_int, _u32, _u32, _vp, _vp]),
@@ -387,6 +388,10 @@ class Mesh:
 
     def trace_any(self, rays: DeviceBuffer, n: int, occ: DeviceBuffer):
         check(load().ilm_trace_any(self.h, rays.ptr, n, occ.ptr, None))
+
+    def trace_closest_noalloc(self, rays: DeviceBuffer, n: int, hits: DeviceBuffer):
+        """trace_closest without any device allocation (fixed per-GPU scratch)."""
+        check(load().ilm_trace_closest_noalloc(self.h, rays.ptr, n, hits.ptr, None))
 
     def visit_counts(self, rays: np.ndarray, any_hit: bool = False, threads: int = 0) -> np.ndarray:
         """Node visits and triangle tests per ray (n x 2 int32) of the ordered,

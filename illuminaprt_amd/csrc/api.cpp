@@ -76,8 +76,8 @@ int ilm_ctx_create(ilm_ctx** out, int device) {
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc(&c->counters, ilm::kCounterWords * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMemset(c->counters, 0, ilm::kCounterWords * sizeof(unsigned long long));
-    if (e != hipSuccess) {
-        int rc = ilm::fail_hip(e, "context setup");
+    int rc = e != hipSuccess ? ilm::fail_hip(e, "context setup") : ilm::noalloc_setup(device, c->stream);
+    if (rc != ILM_OK) {
         if (c->counters) (void)hipFree(c->counters);
         if (c->stream) (void)hipStreamDestroy(c->stream);
         delete c;

@@ -75,6 +75,9 @@ struct ilm_mesh {
 namespace ilm {
 // Ensures ctx->spill holds at least `words` 32-bit words.
 int ensure_spill(ilm_ctx* ctx, size_t words);
+// Takes this device's address of the fixed scratch of ilm_trace_closest_noalloc
+// and zeroes its fetch counters (once per device; called at context creation).
+int noalloc_setup(int device, hipStream_t stream);
 hipStream_t pick_stream(ilm_ctx* ctx, void* stream);
 // PerspectiveCamera frame: out = centre, corner, across, up (12 floats).
 void camera_frame(const ilm_camera& c, float out[12]);

@@ -26,6 +26,8 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <mutex>
+#include <string>
 
 #include "geom.hpp"
 #include "internal.hpp"
@@ -320,10 +322,48 @@ struct AnyQueue {
     uint8_t* occ;
 };
 
+// ---------------------------------------------------------------- fixed scratch
+// The entry point that allocates nothing (ilm_trace_closest_noalloc) keeps
+// its ray-fetch counters and traversal-stack spill area in this array: the
+// first kScratchCounterBytes hold the counters (same layout as
+// ilm_ctx::counters), the rest the spill area. Each device holds its own copy
+// for the life of the process; its device address is taken once per device
+// at context creation (noalloc_setup), and host code only ever passes that
+// address, never the array's name.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+constexpr size_t kScratchCounterBytes = 8192;
+static_assert(kCounterWords * sizeof(unsigned long long) <= kScratchCounterBytes, "counter block too small");
+static __device__ __align__(256) uint32_t scratch[kScratchBytes / sizeof(uint32_t)];
+
+constexpr int kMaxDevices = 64;
+static void* g_scratch[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+__global__ void k_zero_u64(unsigned long long* p, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+int noalloc_setup(int device, hipStream_t stream) {
+    if (device < 0 || device >= kMaxDevices) return fail(ILM_ERR_INVALID, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (g_scratch[device]) return ILM_OK;
+    void* p = nullptr;
+    ILM_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(ILM_ERR_HIP, "no device address for the fixed scratch");
+    // the fetch counters must start at zero (each launch's last wave resets them)
+    hipLaunchKernelGGL(k_zero_u64, dim3((kCounterWords + 255) / 256), dim3(256), 0, stream,
+                       reinterpret_cast<unsigned long long*>(p), (int64_t)kCounterWords);
+    ILM_HIP(hipGetLastError());
+    ILM_HIP(hipStreamSynchronize(stream));
+    g_scratch[device] = p;
+    return ILM_OK;
+}
+
 template <bool ANY, bool STATS, bool DUAL = false>
 static int launch_trace(ilm_mesh* m, const ilm_ray* rays, int64_t n, ilm_hit* hits, uint8_t* occ, int32_t* stats,
                         void* stream, const unsigned long long* n_dev = nullptr, bool rooted = false,
-                        int refill = 0, const AnyQueue* q2 = nullptr) {
+                        int refill = 0, const AnyQueue* q2 = nullptr, bool fixed_scratch = false) {
     static_assert(!DUAL || (!ANY && !STATS), "dual launches trace closest hits plus a queue of any hits");
     if (!m) return fail(ILM_ERR_INVALID, "null mesh");
     if (!m->built) return fail(ILM_ERR_STATE, "BVH not built");
@@ -364,11 +404,28 @@ static int launch_trace(ilm_mesh* m, const ilm_ray* rays, int64_t n, ilm_hit* hi
     a.grid_threads = blocks * kBlock;
     a.rooted = rooted ? 1 : 0;
     a.refill_min = refill > 0 ? refill : ctx->trace_refill;
-    a.spill_depth = std::max(1, m->info.stack_need + 1);
-    int rc = ensure_spill(ctx, (size_t)a.spill_depth * (size_t)a.grid_threads);
-    if (rc) return rc;
-    a.spill = ctx->spill;
-    a.counters = ctx->counters;
+    if (fixed_scratch) {
+        // the spill area holds the entries beyond the kLds in LDS: a stack of
+        // at most stack_need entries spills at most stack_need - kLds of them
+        void* base = ctx->device < kMaxDevices ? g_scratch[ctx->device] : nullptr;
+        if (!base) return fail(ILM_ERR_STATE, "fixed scratch not set up on this device");
+        a.spill_depth = std::max(1, m->info.stack_need + 1 - kLds);
+        const int64_t per_block = (int64_t)a.spill_depth * kBlock * (int64_t)sizeof(uint32_t);
+        const int64_t fit = (int64_t)(kScratchBytes - kScratchCounterBytes) / per_block;
+        if (fit < std::min<int64_t>(blocks, ctx->cu_count))
+            return fail(ILM_ERR_INVALID, "tree too deep for the fixed scratch (stack need " +
+                                             std::to_string(m->info.stack_need) + ", 64 MiB scratch)");
+        blocks = std::min<int64_t>(blocks, fit);
+        a.grid_threads = blocks * kBlock;
+        a.counters = reinterpret_cast<unsigned long long*>(base);
+        a.spill = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(base) + kScratchCounterBytes);
+    } else {
+        a.spill_depth = std::max(1, m->info.stack_need + 1);
+        int rc = ensure_spill(ctx, (size_t)a.spill_depth * (size_t)a.grid_threads);
+        if (rc) return rc;
+        a.spill = ctx->spill;
+        a.counters = ctx->counters;
+    }
     a.total_waves = (uint32_t)(blocks * (kBlock / 64));
     a.rays2 = DUAL ? q2->rays : nullptr;
     a.n2 = DUAL ? q2->capacity : 0;
@@ -632,6 +689,11 @@ int ilm_trace_closest(ilm_mesh* mesh, const ilm_ray* rays, int64_t n, ilm_hit* h
 
 int ilm_trace_any(ilm_mesh* mesh, const ilm_ray* rays, int64_t n, uint8_t* occluded, void* stream) {
     return launch_trace<true, false>(mesh, rays, n, nullptr, occluded, nullptr, stream);
+}
+
+int ilm_trace_closest_noalloc(ilm_mesh* mesh, const ilm_ray* rays, int64_t n, ilm_hit* hits, void* stream) {
+    return launch_trace<false, false>(mesh, rays, n, hits, nullptr, nullptr, stream, nullptr, false, 0, nullptr,
+                                      true);
 }
 
 int ilm_trace_stats(ilm_mesh* mesh, const ilm_ray* rays, int64_t n, int any_hit, int32_t* visits, void* stream) {

@@ -195,6 +195,18 @@ int ilm_trace_closest(ilm_mesh* mesh, const ilm_ray* rays, int64_t n, ilm_hit* h
  * (Scene/Visibility.cpp:33-37). */
 int ilm_trace_any(ilm_mesh* mesh, const ilm_ray* rays, int64_t n, uint8_t* occluded, void* stream);
 
+/* ilm_trace_closest that allocates nothing: for callers that cannot call
+ * hipMalloc (code under HIP graph capture, tight serving loops). No
+ * reference counterpart. Same arguments and results as ilm_trace_closest;
+ * the ray-fetch counters and the traversal-stack spill area live in a fixed
+ * 64 MiB device array per GPU (held until the process ends) instead of the
+ * context's scratch, which ilm_trace_closest grows with hipMalloc. The
+ * persistent grid shrinks to what that spill area holds; a tree whose stack
+ * need (ilm_bvh_info.stack_need) leaves fewer than one workgroup per CU is
+ * rejected with ILM_ERR_INVALID. Every call on a GPU shares its array: calls
+ * that overlap on one GPU (different streams, threads or contexts) race. */
+int ilm_trace_closest_noalloc(ilm_mesh* mesh, const ilm_ray* rays, int64_t n, ilm_hit* hits, void* stream);
+
 /* Host-side traversal statistics (no GPU needed, no hits produced): per ray
  * (visits, host, 2 int32 each) the node visits and triangle tests of a
  * deterministic ordered traversal of the mesh's flattened float tree, the
