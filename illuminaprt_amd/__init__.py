@@ -137,6 +137,7 @@ SIGNATURES = {
     "ilm_memset": (C.c_int, [_vp, _vp, C.c_int, _sz]),
     "ilm_sync": (C.c_int, [_vp]),
     "ilm_device_copy": (C.c_int, [_vp, _vp, _vp, _sz, _vp]),
+    "ilm_read_probe": (C.c_int, [_vp, _vp, _sz, C.c_int, C.c_int, _vp, _vp]),
     "ilm_timer_mark": (C.c_int, [_vp, C.POINTER(C.c_int)]),
     "ilm_timer_elapsed": (C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(C.c_float)]),
     "ilm_mesh_create": (C.c_int, [_vp, _vp, _i64, _vp, _i64, C.POINTER(_vp)]),
@@ -157,6 +158,14 @@ SIGNATURES = {
     "ilm_scenegen_icosphere": (C.c_int, [C.c_int, _f32, C.c_int, _u32, _vp, _vp]),
     "ilm_scenegen_boxes": (C.c_int, [C.c_int, _f32, _f32, _f32, _u32, _vp, _vp, _i64]),
     "ilm_scenegen_vertex_normals": (C.c_int, [_vp, _i64, _vp, _i64, _vp]),
+    "ilm_instances_create": (C.c_int, [_vp, _vp, C.c_int, C.POINTER(_vp)]),
+    "ilm_instances_destroy": (C.c_int, [_vp]),
+    "ilm_instances_set": (C.c_int, [_vp, _vp, _i64]),
+    "ilm_instances_trace_closest": (C.c_int, [_vp, _vp, _i64, _vp, _vp, _vp]),
+    "ilm_instances_trace_any": (C.c_int, [_vp, _vp, _i64, _vp, _vp]),
+    "ilm_instances_info": (C.c_int, [_vp, C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64), C.POINTER(_i64),
+                                     C.POINTER(C.c_int32)]),
+    "ilm_instances_check": (C.c_int, [_vp]),
     "ilm_space_create": (C.c_int, [_vp, C.POINTER(_vp)]),
     "ilm_space_destroy": (C.c_int, [_vp]),
     "ilm_space_add": (C.c_int, [_vp, _vp, _i64, _vp, _i64, _vp, _vp, _vp, _vp, C.c_int32, C.c_int32,
@@ -206,6 +215,7 @@ SIGNATURES = {
     "ilm_batch_destroy": (C.c_int, [_vp]),
     "ilm_render_batch": (C.c_int, [_vp, C.POINTER(BatchView), C.c_int, C.POINTER(RenderParams), _vp,
                                    C.POINTER(RenderStats)]),
+    "ilm_batch_tree": (C.c_int, [_vp, C.POINTER(_vp), _vp]),
     "ilm_batch_set_graph": (C.c_int, [_vp, C.c_int]),
     "ilm_batch_configure": (C.c_int, [_vp, C.c_int, C.c_int, _i64]),
     "ilm_batch_autotune": (C.c_int, [_vp, C.POINTER(BatchView), C.c_int, C.POINTER(RenderParams),
@@ -322,6 +332,10 @@ class Context:
     def copy(self, dst: DeviceBuffer, src: DeviceBuffer, nbytes: int):
         check(load().ilm_device_copy(self.h, dst.ptr, src.ptr, nbytes, None))
 
+    def read_probe(self, src: DeviceBuffer, nbytes: int, passes: int, blocks_per_cu: int, sink: DeviceBuffer):
+        """Launches the cache-bandwidth probe (reads cu_count * blocks_per_cu * passes * nbytes bytes)."""
+        check(load().ilm_read_probe(self.h, src.ptr, nbytes, passes, blocks_per_cu, sink.ptr, None))
+
     def mark(self) -> int:
         s = C.c_int()
         check(load().ilm_timer_mark(self.h, C.byref(s)))
@@ -428,6 +442,7 @@ class Batch:
         h = C.c_void_p()
         check(load().ilm_batch_create(ctx.h, arr, len(scenes), C.byref(h)))
         self.h = h.value
+        self.nscenes = len(scenes)
 
     @staticmethod
     def _views(views):
@@ -470,6 +485,16 @@ class Batch:
         """Node format and BVH width (2, 4, 8; 2 and 8 with float nodes) of
         later frames, and paths per wavefront batch (0 = default)."""
         check(load().ilm_batch_configure(self.h, node_format, width, batch_paths))
+
+    def tree(self):
+        """(packed mesh, roots): the batch's float BVH4 as a Mesh view (not
+        owned; export() and info() work on it) and each scene's root node."""
+        h = C.c_void_p()
+        roots = np.zeros(self.nscenes, np.uint32)
+        check(load().ilm_batch_tree(self.h, C.byref(h), _ptr(roots)))
+        m = Mesh.__new__(Mesh)
+        m.h, m.ctx, m.borrowed = h.value, self.ctx, True
+        return m, roots
 
     def set_graph(self, enable: bool):
         check(load().ilm_batch_set_graph(self.h, int(enable)))
@@ -580,6 +605,57 @@ class Persistent:
     def close(self):
         if self.h:
             load().ilm_persistent_destroy(self.h)
+            self.h = None
+
+
+class Instance(C.Structure):
+    _fields_ = [("mesh", C.c_int32), ("rot", C.c_float * 9), ("scl", C.c_float * 3), ("trn", C.c_float * 3)]
+
+
+class Instances:
+    """The flat instance table: a two-level BVH over shared meshes (each mesh
+    stored once; instances are transforms under a top-level tree)."""
+
+    def __init__(self, ctx: Context, meshes):
+        self.ctx = ctx
+        arr = (C.c_void_p * len(meshes))(*[m.h for m in meshes])
+        h = C.c_void_p()
+        check(load().ilm_instances_create(ctx.h, arr, len(meshes), C.byref(h)))
+        self.h = h.value
+
+    def set(self, instances):
+        """instances: list of (mesh index, rot (9, or None), scl (3, or None), trn (3, or None))."""
+        arr = (Instance * max(1, len(instances)))()
+        for i, (mesh, rot, scl, trn) in enumerate(instances):
+            arr[i].mesh = mesh
+            r = np.eye(3, dtype=np.float32).reshape(-1) if rot is None else np.asarray(rot, np.float32).reshape(-1)
+            sc = np.ones(3, np.float32) if scl is None else np.asarray(scl, np.float32).reshape(-1)
+            t = np.zeros(3, np.float32) if trn is None else np.asarray(trn, np.float32).reshape(-1)
+            for k in range(9):
+                arr[i].rot[k] = float(r[k])
+            for k in range(3):
+                arr[i].scl[k], arr[i].trn[k] = float(sc[k]), float(t[k])
+        check(load().ilm_instances_set(self.h, arr, len(instances)))
+
+    def trace_closest(self, rays: DeviceBuffer, n: int, hits: DeviceBuffer, inst: DeviceBuffer):
+        check(load().ilm_instances_trace_closest(self.h, rays.ptr, n, hits.ptr, inst.ptr, None))
+
+    def trace_any(self, rays: DeviceBuffer, n: int, occ: DeviceBuffer):
+        check(load().ilm_instances_trace_any(self.h, rays.ptr, n, occ.ptr, None))
+
+    def info(self) -> dict:
+        v = [C.c_int64() for _ in range(4)]
+        need = C.c_int32()
+        check(load().ilm_instances_info(self.h, *[C.byref(x) for x in v], C.byref(need)))
+        return {"instances": v[0].value, "mesh_bytes": v[1].value, "table_bytes": v[2].value,
+                "tlas_bytes": v[3].value, "stack_need": need.value}
+
+    def check(self):
+        check(load().ilm_instances_check(self.h))
+
+    def close(self):
+        if self.h:
+            load().ilm_instances_destroy(self.h)
             self.h = None
 
 

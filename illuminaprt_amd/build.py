@@ -24,7 +24,7 @@ SANDBOX = os.path.join(HERE, "ilm_plugin_sandbox")
 PLUGIN_SRC = os.path.join(HERE, "plugin")
 ARCH = "gfx950"
 
-SOURCES = ["api.cpp", "bvh_build.cpp", "scenegen.cpp", "objfile.cpp", "persistent.cpp", "stats.cpp", "trace.hip", "lbvh.hip", "space.hip", "post.hip", "pt.hip", "nrc.hip", "icache.hip"]
+SOURCES = ["api.cpp", "bvh_build.cpp", "scenegen.cpp", "objfile.cpp", "persistent.cpp", "stats.cpp", "trace.hip", "lbvh.hip", "space.hip", "inst.hip", "post.hip", "pt.hip", "nrc.hip", "icache.hip"]
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 

@@ -430,22 +430,37 @@ __global__ void __launch_bounds__(kPtBlock) k_shade(PtArgs a, int bounce, int cu
                                          1u + (uint32_t)(l - vl0));
                 int2 rg = a.lrange[l];
                 float4 le = a.lights[l];
-                int kk = min((int)((float)rg.y * lb.v[2]), rg.y - 1);
-                const float4* lt = a.ltris + 3 * (int64_t)(rg.x + kk);
-                float4 p1 = lt[0], p2 = lt[1], p3 = lt[2];
-                float su = sqrtf(lb.v[0]);
-                float b1 = 1.f - su, b2 = lb.v[1] * su;  // Montecarlo::UniformSampleTriangle
-                float b3 = 1.f - b1 - b2;
-                f3 Q = mk(b1 * p1.x + b2 * p2.x + b3 * p3.x, b1 * p1.y + b2 * p2.y + b3 * p3.y,
-                          b1 * p1.z + b2 * p2.z + b3 * p3.z);
-                f3 nl = normalize(cross(mk(p3.x - p1.x, p3.y - p1.y, p3.z - p1.z),
-                                        mk(p2.x - p1.x, p2.y - p1.y, p2.z - p1.z)));
-                if (dot(Q - P, nl) > 0.f) nl = -nl;  // ITriangleMesh::SamplePoint (TriangleMesh.cpp:150-161)
+                f3 Q, nl;
+                if (le.w < 0.f) {
+                    // Quad::SamplePoint (Shape/Quad.cpp:146-153): the bilinear map
+                    // v1 + (v3 - v1) u + (v4 - v3) v + (v1 - v3 + v2 - v4) u v and the
+                    // winding normal cross(v3 - v1, v2 - v1), normalised by
+                    // DiffuseAreaLight::SamplePoint (DiffuseAreaLight.cpp:95-97) and
+                    // not turned toward the shaded point
+                    const float4* lt = a.ltris + 3 * (int64_t)rg.x;
+                    const f3 v1 = mk(lt[0].x, lt[0].y, lt[0].z), v3 = mk(lt[1].x, lt[1].y, lt[1].z);
+                    const f3 v2 = mk(lt[2].x, lt[2].y, lt[2].z), v4 = mk(lt[4].x, lt[4].y, lt[4].z);
+                    const float u = lb.v[0], v = lb.v[1];
+                    Q = v1 + (v3 - v1) * u + (v4 - v3) * v + (v1 - v3 + v2 - v4) * (u * v);
+                    nl = normalize(cross(v3 - v1, v2 - v1));
+                } else {
+                    int kk = min((int)((float)rg.y * lb.v[2]), rg.y - 1);
+                    const float4* lt = a.ltris + 3 * (int64_t)(rg.x + kk);
+                    float4 p1 = lt[0], p2 = lt[1], p3 = lt[2];
+                    float su = sqrtf(lb.v[0]);
+                    float b1 = 1.f - su, b2 = lb.v[1] * su;  // Montecarlo::UniformSampleTriangle
+                    float b3 = 1.f - b1 - b2;
+                    Q = mk(b1 * p1.x + b2 * p2.x + b3 * p3.x, b1 * p1.y + b2 * p2.y + b3 * p3.y,
+                           b1 * p1.z + b2 * p2.z + b3 * p3.z);
+                    nl = normalize(cross(mk(p3.x - p1.x, p3.y - p1.y, p3.z - p1.z),
+                                         mk(p2.x - p1.x, p2.y - p1.y, p2.z - p1.z)));
+                    if (dot(Q - P, nl) > 0.f) nl = -nl;  // ITriangleMesh::SamplePoint (TriangleMesh.cpp:150-161)
+                }
                 f3 wIn = P - Q;
                 float d2 = dot(wIn, wIn);
                 wIn = normalize(wIn);
                 float c = fmaxf(0.f, dot(wIn, nl));
-                float pdf = 1.0f / le.w;
+                float pdf = 1.0f / fabsf(le.w);
                 f3 Ls = mk(le.x, le.y, le.z) * c / (pdf * d2);
                 if (!is_black(Ls)) {
                     wIn = -wIn;
@@ -823,6 +838,19 @@ int ilm_scene_create(ilm_ctx* ctx, ilm_mesh* mesh, const float* nrm, const int32
         lrec[4 * l + 1] = lights[4 * l + 1];
         lrec[4 * l + 2] = lights[4 * l + 2];
         lrec[4 * l + 3] = (float)area;
+        if (lights[4 * l + 3] == 1.0f) {
+            // the reference's Quad (Shape/Quad.cpp:11-31): triangles (v1, v3, v2)
+            // and (v1, v4, v3); sampled by its SamplePoint map, marked by a
+            // negative area
+            if (ltri[l].size() != 2) return fail(ILM_ERR_INVALID, "a quad light needs exactly two triangles");
+            const int32_t* a0 = &mesh->idx[3 * ltri[l][0]];
+            const int32_t* a1 = &mesh->idx[3 * ltri[l][1]];
+            if (a0[0] != a1[0] || a0[1] != a1[2])
+                return fail(ILM_ERR_INVALID, "a quad light's triangles must be (v1, v3, v2) and (v1, v4, v3)");
+            lrec[4 * l + 3] = -(float)area;
+        } else if (lights[4 * l + 3] != 0.0f) {
+            return fail(ILM_ERR_INVALID, "light sampling mode must be 0 (mesh) or 1 (quad)");
+        }
     }
     if (lt.empty()) lt.assign(12, 0.f);
     ilm_scene* s = new ilm_scene();
@@ -1419,6 +1447,8 @@ int ilm_batch_create(ilm_ctx* ctx, ilm_scene* const* scenes, int n, ilm_batch** 
                        reinterpret_cast<const uint8_t*>(fnodes.data()) + (size_t)nodes * sizeof(ilm::WideNode<4>));
     pm->h_order.swap(order);
     pm->info.nodes = nodes;
+    pm->info.node_stride = (int32_t)sizeof(ilm::WideNode<4>);
+    pm->info.node_bytes = nodes * (int64_t)sizeof(ilm::WideNode<4>);
     pm->info.tri_bytes = (int64_t)(tris.size() * sizeof(float));
     pm->info.width = 4;
     pm->info.depth = depth;
@@ -1441,6 +1471,15 @@ int ilm_batch_create(ilm_ctx* ctx, ilm_scene* const* scenes, int n, ilm_batch** 
     b->merged = ms;
     batch_use_format(b, ILM_NODES_FLOAT);
     *out = b;
+    return ILM_OK;
+}
+
+int ilm_batch_tree(ilm_batch* b, ilm_mesh** mesh, uint32_t* roots) {
+    if (!b || !mesh) return fail(ILM_ERR_INVALID, "null argument");
+    if (b->node_format != ILM_NODES_FLOAT || b->width != 4)
+        return fail(ILM_ERR_STATE, "the batch's tree is exported while it uses float BVH4 nodes");
+    *mesh = b->merged->mesh;
+    if (roots) memcpy(roots, b->root4.data(), b->root4.size() * sizeof(uint32_t));
     return ILM_OK;
 }
 

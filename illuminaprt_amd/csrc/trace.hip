@@ -654,6 +654,28 @@ __global__ void k_copy(const float4* __restrict__ src, float4* __restrict__ dst,
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
 }
 
+// Cache-bandwidth probe: every workgroup streams the whole (small) buffer
+// `passes` times with 16-byte loads, starting 64 KiB after the previous
+// workgroup and wrapping around. A buffer that fits each XCD's L2 (4 MiB)
+// but not a CU's L1 (32 KiB) measures the aggregate L2-to-CU read bandwidth.
+// The loaded words are folded into a value that is stored only if it equals
+// an impossible pattern, so the loads cannot be dropped.
+__global__ void __launch_bounds__(256) k_read_probe(const uint4* __restrict__ src, int64_t n16, int passes,
+                                                    uint4* __restrict__ sink) {
+    uint4 acc = make_uint4(0u, 0u, 0u, 0u);
+    const int64_t start = ((int64_t)blockIdx.x * 4096) % n16;
+    for (int p = 0; p < passes; ++p) {
+#pragma unroll 8
+        for (int64_t i = threadIdx.x; i < n16; i += blockDim.x) {
+            int64_t j = start + i;
+            if (j >= n16) j -= n16;
+            const uint4 v = src[j];
+            acc.x ^= v.x; acc.y += v.y; acc.z ^= v.z; acc.w += v.w;
+        }
+    }
+    if (acc.x == 0x9E3779B9u && acc.y == 0x7F4A7C15u && acc.z == 0xF39CC060u && acc.w == 0x5CEDC834u) sink[0] = acc;
+}
+
 }  // namespace ilm
 
 namespace ilm {
@@ -780,6 +802,17 @@ int ilm_device_copy(ilm_ctx* ctx, void* dst, const void* src, size_t bytes, void
     int64_t blocks = std::min<int64_t>((n16 + 255) / 256, (int64_t)ctx->cu_count * 32);
     hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(256), 0, pick_stream(ctx, stream),
                        reinterpret_cast<const float4*>(src), reinterpret_cast<float4*>(dst), n16);
+    ILM_HIP(hipGetLastError());
+    return ILM_OK;
+}
+
+int ilm_read_probe(ilm_ctx* ctx, const void* src, size_t bytes, int passes, int blocks_per_cu, void* sink,
+                   void* stream) {
+    if (!ctx || !src || !sink || bytes < 16 || (bytes % 16) != 0 || passes <= 0 || blocks_per_cu <= 0)
+        return fail(ILM_ERR_INVALID, "read probe needs a buffer of 16-byte multiples, passes and blocks");
+    const int64_t blocks = (int64_t)ctx->cu_count * blocks_per_cu;
+    hipLaunchKernelGGL(k_read_probe, dim3((unsigned)blocks), dim3(256), 0, pick_stream(ctx, stream),
+                       reinterpret_cast<const uint4*>(src), (int64_t)(bytes / 16), passes, reinterpret_cast<uint4*>(sink));
     ILM_HIP(hipGetLastError());
     return ILM_OK;
 }

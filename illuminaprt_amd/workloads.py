@@ -178,6 +178,45 @@ def box_room():
             np.array(tm, np.int32), np.array(tl, np.int32), ROOM_MATS.copy(), lights)
 
 
+def box_room_reference(light_y=0.999):
+    """The box room as the path-tracing workload specifies it, lit the
+    reference's way: 5 walls (floor, ceiling, back, left, right; 10
+    triangles) and the 2 boxes (24 triangles), 34 triangles, plus one ceiling
+    quad light (2 triangles, Le = (17, 12, 4)) hung just below the ceiling
+    and sampled as the reference's Quad (light mode 1): its triangles are
+    (v1, v3, v2), (v1, v4, v3) (Shape/Quad.cpp:11-31) with the winding
+    normal cross(v3 - v1, v2 - v1) facing down. Returns pos, nrm, idx,
+    tri_mat, tri_light, mats, lights (36 triangles)."""
+    quads = []
+    quads.append((*_quad([0, 0, 0], [1, 0, 0], [1, 0, 1], [0, 0, 1], [0, 1, 0]), 0))    # floor
+    quads.append((*_quad([0, 1, 0], [1, 1, 0], [1, 1, 1], [0, 1, 1], [0, -1, 0]), 0))   # ceiling
+    quads.append((*_quad([0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1], [0, 0, -1]), 0))   # back
+    quads.append((*_quad([0, 0, 0], [0, 1, 0], [0, 1, 1], [0, 0, 1], [1, 0, 0]), 1))    # left
+    quads.append((*_quad([1, 0, 0], [1, 1, 0], [1, 1, 1], [1, 0, 1], [-1, 0, 0]), 2))   # right
+    for lo, hi in (([0.13, 0.0, 0.15], [0.43, 0.30, 0.45]), ([0.55, 0.0, 0.45], [0.85, 0.60, 0.75])):
+        for p, n in _box_faces(lo, hi):
+            quads.append((p, n, 0))
+    pos, nrm, idx, tm, tl = [], [], [], [], []
+    for p, n, m in quads:
+        b = len(pos)
+        pos += list(p)
+        nrm += [n] * 4
+        idx += [[b, b + 1, b + 2], [b, b + 2, b + 3]]
+        tm += [m, m]
+        tl += [-1, -1]
+    y = light_y
+    v1, v2, v3, v4 = [0.4, y, 0.4], [0.4, y, 0.6], [0.6, y, 0.6], [0.6, y, 0.4]
+    b = len(pos)
+    pos += [v1, v2, v3, v4]
+    nrm += [[0, -1, 0]] * 4
+    idx += [[b, b + 2, b + 1], [b, b + 3, b + 2]]   # (v1, v3, v2), (v1, v4, v3)
+    tm += [3, 3]
+    tl += [0, 0]
+    lights = np.array([[*ROOM_LE, 1.0]], np.float32)
+    return (np.array(pos, np.float32), np.array(nrm, np.float32), np.array(idx, np.int32),
+            np.array(tm, np.int32), np.array(tl, np.int32), ROOM_MATS.copy(), lights)
+
+
 def room_camera():
     return Camera.make((0.5, 0.5, -1.4), (0.5, 0.5, 0.5), (0.0, 1.0, 0.0), 40.0, 1.0)
 
@@ -329,3 +368,46 @@ def large_cameras(n=8, aspect=16 / 9):
         cams.append(Camera.make((4.0 * np.sin(a), 0.8, -4.0 * np.cos(a)), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0),
                                 55.0, aspect))
     return cams
+
+
+# ---------------------------------------------------------------- instanced scene
+# The flat instance table's workload: 2,048 instances of three meshes (a
+# displaced level-5 icosphere of 20,480 triangles, a cluster of 200 boxes of
+# 2,400 triangles, a level-3 icosphere of 1,280 triangles) with uniformly
+# random rotations, scalings in [0.2, 0.6] and positions in a 16 x 16 x 16
+# cube; a camera outside the cube looks at its centre.
+def _rotation(rng):
+    q, r = np.linalg.qr(rng.normal(size=(3, 3)))
+    q = q * np.sign(np.diag(r))
+    if np.linalg.det(q) < 0:
+        q[:, 0] = -q[:, 0]
+    return q
+
+
+def instanced_scene(n_inst=2048, seed=13):
+    """(meshes [(pos, idx)], placements [(mesh, rot9, scl3, trn3)])."""
+    rng = np.random.default_rng(seed)
+    meshes = [icosphere(5, 0.1, 4, 3), boxes(200, 0.05, 0.3, 1.0, 5), icosphere(3, 0.0, 1, 0)]
+    placements = []
+    for _ in range(n_inst):
+        m = int(rng.integers(len(meshes)))
+        placements.append((m, _rotation(rng).astype(np.float32).reshape(-1),
+                           rng.uniform(0.2, 0.6, 3).astype(np.float32), rng.uniform(-8, 8, 3).astype(np.float32)))
+    return meshes, placements
+
+
+def flatten_placements(meshes, placements):
+    """World-space positions and indices of every instance (the space's
+    flattening: world_i = sum_j rot[3j+i] scl_j p_j + trn_i, in float64)."""
+    ps, ix, off = [], [], 0
+    for m, rot, scl, trn in placements:
+        pos, idx = meshes[m]
+        p = (pos.astype(np.float64) * scl.astype(np.float64)) @ rot.astype(np.float64).reshape(3, 3) + trn
+        ps.append(p.astype(np.float32))
+        ix.append(idx + off)
+        off += len(pos)
+    return np.concatenate(ps), np.concatenate(ix).astype(np.int32)
+
+
+def instanced_camera():
+    return Camera.make((0.0, 3.0, -24.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 45.0, 1.0)

@@ -98,6 +98,15 @@ int ilm_sync(ilm_ctx* ctx);
  * reference counterpart: it re-measures HBM bandwidth and calibrates the
  * profiler's byte counters against a transfer of known size. */
 int ilm_device_copy(ilm_ctx* ctx, void* dst, const void* src, size_t bytes, void* stream);
+/* Cache-bandwidth probe: blocks_per_cu workgroups per CU each read the
+ * whole buffer (bytes a multiple of 16) `passes` times with 16-byte loads,
+ * each workgroup starting 64 KiB after the previous one. A buffer larger
+ * than a CU's L1 and smaller than an XCD's L2 (e.g. 2 MiB) measures the
+ * aggregate L2 read bandwidth (bytes read = CUs x blocks_per_cu x passes x
+ * bytes). sink (16 bytes, device) is written only for an impossible fold of
+ * the data. No reference counterpart. */
+int ilm_read_probe(ilm_ctx* ctx, const void* src, size_t bytes, int passes, int blocks_per_cu, void* sink,
+                   void* stream);
 /* Event timing on the context stream: ilm_timer_mark records an event and
  * returns its slot; ilm_timer_elapsed gives milliseconds between two slots
  * (synchronises). Used by benchmarks to time kernels on the stream they
@@ -275,6 +284,53 @@ int ilm_space_geometry(ilm_space* space, float* pos, int32_t* idx);
 int ilm_space_resolve(ilm_space* space, const ilm_hit* hits, int64_t n, int32_t* prim, int32_t* local,
                       void* stream);
 
+/* ---------------------------------------------------------------- instances */
+
+/* The flat instance table: a two-level BVH over shared meshes, the
+ * instancing counterpart of a space. Each mesh's tree and triangles are
+ * stored once however many instances place it; an instance is a record
+ * (world-to-object transform, its mesh's root node, its index) under a
+ * top-level BVH4 over the instances' world boxes. Replaces BasicSpace's
+ * linear loop over primitives (Space/BasicSpace.cpp:58-114), each testing
+ * the ray transformed into its object space (GeometricPrimitive::Intersects,
+ * Scene/GeometricPrimitive.cpp:68-200; Ray::ApplyInverse, Geometry/Ray.cpp:
+ * 50-60): the object-space direction is not normalised, so t is the same in
+ * both spaces and hits keep the reference's t window. */
+typedef struct ilm_instances ilm_instances;
+typedef struct {
+    int32_t mesh;     /* index into the meshes of ilm_instances_create */
+    float rot[9];     /* Matrix3x3 in storage order, applied as in ilm_space_add */
+    float scl[3];     /* scaling (non-zero) */
+    float trn[3];     /* translation */
+} ilm_instance;
+/* meshes: built meshes with float BVH4 nodes (ILM_NODES_FLOAT, width 4),
+ * whose trees and triangles are copied to the table's device memory once;
+ * the meshes may be destroyed afterwards. */
+int ilm_instances_create(ilm_ctx* ctx, ilm_mesh* const* meshes, int nmeshes, ilm_instances** out);
+int ilm_instances_destroy(ilm_instances* table);
+/* Sets the instance table (or replaces it: ISpace::Update when transforms
+ * change, as the reference's frame loop calls it, Utility/Sandbox/
+ * Multithreaded.h:399) and rebuilds the top-level tree on the host; the
+ * meshes' trees are not touched. Synchronous. */
+int ilm_instances_set(ilm_instances* table, const ilm_instance* inst, int64_t n);
+/* Closest hit of n rays (dev pointers): hits[i] = (t, triangle index within
+ * the instance's mesh, u, v) and inst[i] = the instance's index in the last
+ * ilm_instances_set (-1 on a miss), as BasicSpace::Intersects reports the
+ * primitive and its surface. */
+int ilm_instances_trace_closest(ilm_instances* table, const ilm_ray* rays, int64_t n, ilm_hit* hits, int32_t* inst,
+                                void* stream);
+/* Any hit (BasicSpace::Intersects(const Ray&), BasicSpace.cpp:92-101). */
+int ilm_instances_trace_any(ilm_instances* table, const ilm_ray* rays, int64_t n, uint8_t* occluded, void* stream);
+/* Instance count and device bytes: meshes' trees and triangles (stored
+ * once), instance records (64 B each), top-level tree; the stack entries a
+ * ray may need. Any pointer may be NULL. */
+int ilm_instances_info(ilm_instances* table, int64_t* ninst, int64_t* mesh_bytes, int64_t* table_bytes,
+                       int64_t* tlas_bytes, int32_t* stack_need);
+/* Waits for the context stream and reports ILM_ERR_STATE (once) if a
+ * traversal exceeded its stack (never expected: ilm_instances_set rejects
+ * tables whose stack need exceeds the kernel's 64 entries). */
+int ilm_instances_check(ilm_instances* table);
+
 /* ---------------------------------------------------------------- ray sets */
 
 /* Camera description: PerspectiveCamera with the factory's frame
@@ -373,9 +429,17 @@ int ilm_persistent_destroy(ilm_persistent* p);
 
 /* Material: 8 floats — kd rgb, ks rgb, phong exponent, type (0 = Lambertian
  * MatteMaterial, Material/Matte.h:16-65; 1 = Lambertian + normalised Phong
- * lobe, which has no reference counterpart). Light: 4 floats — Le rgb, 0;
- * its emitter is every triangle whose tri_light equals its index
- * (DiffuseAreaLight over a triangle mesh, Light/DiffuseAreaLight.cpp:17-149). */
+ * lobe, which has no reference counterpart). Light: 4 floats — Le rgb,
+ * sampling mode; its emitter is every triangle whose tri_light equals its
+ * index (DiffuseAreaLight, Light/DiffuseAreaLight.cpp:17-149). Mode 0: the
+ * emitter is sampled as ITriangleMesh::SamplePoint does (a triangle chosen
+ * uniformly by index, a uniform point on it, the normal turned toward the
+ * shaded point; Shape/TriangleMesh.cpp:143-161). Mode 1: the emitter is the
+ * reference's Quad, exactly two triangles (v1, v3, v2) and (v1, v4, v3) in
+ * that order (Shape/Quad.cpp:11-31), sampled by Quad::SamplePoint's map
+ * v1 + (v3-v1) u + (v4-v3) v + (v1-v3+v2-v4) u v with the one-sided winding
+ * normal cross(v3-v1, v2-v1) (Quad.cpp:146-153); the map does not cover the
+ * quad unless it is degenerate, which the reference's renders share. */
 int ilm_scene_create(ilm_ctx* ctx, ilm_mesh* mesh, const float* nrm,
                      const int32_t* tri_mat, const int32_t* tri_light,
                      const float* mats, int nmat, const float* lights, int nlight,
@@ -494,6 +558,13 @@ typedef struct {
  * default value); its camera, width, height and spp are not used. */
 int ilm_render_batch(ilm_batch* batch, const ilm_batch_view* views, int nviews, const ilm_render_params* params,
                      double* accum, ilm_render_stats* stats);
+/* The batch's packed tree, for validation: the packed mesh (owned by the
+ * batch; ilm_bvh_get_info / ilm_bvh_export read its float BVH4 nodes and its
+ * triangle slots, whose indices count the scenes' triangles one scene after
+ * another) and the root node of each scene (roots: nscenes uint32, may be
+ * NULL). ILM_ERR_STATE while the batch uses another node format or width.
+ * No reference counterpart. */
+int ilm_batch_tree(ilm_batch* batch, ilm_mesh** mesh, uint32_t* roots);
 /* As ilm_scene_set_graph, for the batch's frames. */
 int ilm_batch_set_graph(ilm_batch* batch, int enable);
 /* Fixes the tree of later frames -- node format (ILM_NODES_*) and BVH width

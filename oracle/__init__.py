@@ -40,6 +40,7 @@ class RenderParams(C.Structure):
         ("max_depth", C.c_int32), ("rr_depth", C.c_int32),
         ("eps", C.c_float), ("seed", C.c_uint32), ("rng_mode", C.c_int32),
         ("x0", C.c_int32), ("y0", C.c_int32), ("x1", C.c_int32), ("y1", C.c_int32),
+        ("pix0", C.c_int64),
     ]
 
 
@@ -60,9 +61,12 @@ def lib():
     L.orc_philox4x64.argtypes = [_u64p, _u64p, _u64p]
     L.orc_cosine_hemisphere.argtypes = [C.c_float, C.c_float, _f32p]
     L.orc_uniform_triangle.argtypes = [C.c_float, C.c_float, fp3, fp3]
+    L.orc_quad_sample.argtypes = [_f32p, C.c_float, C.c_float, _f32p, _f32p]
     L.orc_camera_ray.argtypes = [_f32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, _f32p, _f32p]
     L.orc_bsdf_eval.argtypes = [_f32p, C.c_int64, _f32p, _f32p, _f32p, _f32p]
     L.orc_bsdf_sample.argtypes = [_f32p, C.c_int64, _f32p, _f32p, _f32p, _f32p, _f32p]
+    L.orc_ic_render.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, _f64p, _i64p, C.c_int]
+    L.orc_ic_render.restype = C.c_int
     L.orc_mesh_create.argtypes = [_f32p, C.c_int64, _i32p, C.c_int64]
     L.orc_mesh_create.restype = C.c_void_p
     L.orc_mesh_destroy.argtypes = [C.c_void_p]
@@ -136,6 +140,14 @@ def uniform_triangle(u, v):
     b1, b2 = C.c_float(), C.c_float()
     lib().orc_uniform_triangle(u, v, C.byref(b1), C.byref(b2))
     return b1.value, b2.value
+
+
+def quad_sample(v1, v2, v3, v4, u, v):
+    """The reference's Quad::SamplePoint: (point, normalised winding normal)."""
+    q = np.ascontiguousarray(np.concatenate([v1, v2, v3, v4]), np.float32)
+    p, n = np.zeros(3, np.float32), np.zeros(3, np.float32)
+    lib().orc_quad_sample(q, u, v, p, n)
+    return p, n
 
 
 def camera_ray(cam, w, h, px, py, u, v):
@@ -273,6 +285,13 @@ class Space:
             self._h = None
 
 
+class IcParams(C.Structure):
+    _fields_ = [("cache_depth", C.c_int32), ("error_threshold", C.c_float), ("ambient_resolution", C.c_float),
+                ("ambient_multiplier", C.c_float), ("azimuth_strata", C.c_int32), ("altitude_strata", C.c_int32),
+                ("ray_depth", C.c_int32), ("eps", C.c_float), ("indirect_factor", C.c_float),
+                ("miss_radiance", C.c_float * 3)]
+
+
 class Scene:
     """Shaded scene for the PathIntegrator restatement."""
 
@@ -288,12 +307,15 @@ class Scene:
                                          mats, len(mats), lights, len(lights))
 
     def render(self, cam, width, height, spp, max_depth=5, rr_depth=4, eps=1e-4, seed=1234,
-               rng_mode=0, spp_offset=0, window=None, nthreads=0, accum=None):
+               rng_mode=0, spp_offset=0, window=None, nthreads=0, accum=None, pix0=0):
+        """pix0: offset of the pixel counter of the counter-based streams (the
+        first pixel of this image in a batch of images)."""
         p = RenderParams()
         for i, v in enumerate(cam):
             p.cam[i] = float(v)
         p.width, p.height, p.spp, p.spp_offset = width, height, spp, spp_offset
         p.max_depth, p.rr_depth, p.eps, p.seed, p.rng_mode = max_depth, rr_depth, eps, seed, rng_mode
+        p.pix0 = pix0
         if window is not None:
             p.x0, p.y0, p.x1, p.y1 = window
         if accum is None:
@@ -302,6 +324,27 @@ class Scene:
         used = lib().orc_render(self._h, C.byref(p), accum.reshape(-1), counts, nthreads)
         self.last_threads = used
         self.last_rays = counts
+        return accum
+
+    def ic_render(self, cam, width, height, spp, seed=1234, window=None, nthreads=0, cache_depth=16,
+                  error_threshold=1.0, ambient_resolution=1e-3, ambient_multiplier=1e3, azimuth_strata=12,
+                  altitude_strata=4, ray_depth=6, eps=1e-4, indirect_factor=1.0 / np.pi,
+                  miss_radiance=(20.0, 50.0, 100.0), spp_offset=0):
+        """ICIntegrator restatement (ICIntegrator factory defaults): radiance
+        sums per pixel (height, width, 3); self.last_counts = camera rays,
+        gather path rays, shadow rays, records created."""
+        p = RenderParams()
+        for i, v in enumerate(cam):
+            p.cam[i] = float(v)
+        p.width, p.height, p.spp, p.spp_offset, p.seed = width, height, spp, spp_offset, seed
+        if window is not None:
+            p.x0, p.y0, p.x1, p.y1 = window
+        q = IcParams(cache_depth, error_threshold, ambient_resolution, ambient_multiplier, azimuth_strata,
+                     altitude_strata, ray_depth, eps, indirect_factor, (C.c_float * 3)(*miss_radiance))
+        accum = np.zeros((height, width, 3), np.float64)
+        counts = np.zeros(4, np.int64)
+        self.last_threads = lib().orc_ic_render(self._h, C.byref(p), C.byref(q), accum.reshape(-1), counts, nthreads)
+        self.last_counts = counts
         return accum
 
     def __del__(self):

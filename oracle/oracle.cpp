@@ -23,7 +23,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <random>
+#include <shared_mutex>
 #include <thread>
 #include <vector>
 
@@ -681,7 +683,9 @@ struct orc_scene {
     std::vector<int> tri_mat, tri_light;
     struct Mat { V3 kd, ks; float exponent; int type; };
     std::vector<Mat> mats;
-    struct Light { V3 Le; std::vector<int> tris; float area = 0; };
+    // quad: sampled as the reference's Quad (Shape/Quad.cpp:146-153) from its
+    // two triangles (v1, v3, v2) and (v1, v4, v3); otherwise as ITriangleMesh
+    struct Light { V3 Le; std::vector<int> tris; float area = 0; bool quad = false; };
     std::vector<Light> lights;
 };
 
@@ -799,18 +803,35 @@ inline Surface make_surface(const orc_scene& s, const Ray& r, const Hit& h) {
 // DiffuseAreaLight::SampleRadiance (DiffuseAreaLight.cpp:52-69) through
 // SamplePoint (:72-100) and ITriangleMesh::SamplePoint (TriangleMesh.cpp:143-161),
 // IndexedTriangle::SamplePoint (IndexedTriangle.cpp:400-414).
+// Quad::SamplePoint (Shape/Quad.cpp:146-153) over the tessellated triangles
+// T0 = (v1, v3, v2), T1 = (v1, v4, v3): T0.V0 + (T0.V1 - T0.V0) u +
+// (T1.V1 - T0.V1) v + (T0.V0 - T0.V1 + T0.V2 - T1.V1) u v, normal
+// cross(T0.V1 - T0.V0, T0.V2 - T0.V0), normalised by DiffuseAreaLight::
+// SamplePoint (DiffuseAreaLight.cpp:95-97) and not turned toward the viewer.
+inline V3 quad_sample(V3 v1, V3 v2, V3 v3, V3 v4, float u, float v, V3& n) {
+    n = normalize(cross(v3 - v1, v2 - v1));
+    return v1 + (v3 - v1) * u + (v4 - v3) * v + (v1 - v3 + v2 - v4) * (u * v);
+}
+
 inline V3 light_sample(const orc_scene& s, int l, V3 P, float u, float v, float pick, V3& wIn, Ray& seg) {
     const orc_scene::Light& L = s.lights[l];
-    int n = (int)L.tris.size();
-    int k = (int)rmin((size_t)(n * pick), (size_t)(n - 1));
-    const Tri& tr = s.mesh->tris[L.tris[k]];
-    float b1, b2;
-    uniform_triangle(u, v, b1, b2);
-    const V3& p1 = s.mesh->pos[tr.v[0]]; const V3& p2 = s.mesh->pos[tr.v[1]]; const V3& p3 = s.mesh->pos[tr.v[2]];
-    V3 Q = b1 * p1 + b2 * p2 + (1.f - b1 - b2) * p3;
-    V3 nl = normalize(cross(p3 - p1, p2 - p1));
-    if (dot(Q - P, nl) > 0) nl = -nl;
-    nl = normalize(nl);
+    V3 Q, nl;
+    if (L.quad) {
+        const Tri& t0 = s.mesh->tris[L.tris[0]];
+        const Tri& t1 = s.mesh->tris[L.tris[1]];
+        Q = quad_sample(s.mesh->pos[t0.v[0]], s.mesh->pos[t0.v[2]], s.mesh->pos[t0.v[1]], s.mesh->pos[t1.v[1]], u, v, nl);
+    } else {
+        int n = (int)L.tris.size();
+        int k = (int)rmin((size_t)(n * pick), (size_t)(n - 1));
+        const Tri& tr = s.mesh->tris[L.tris[k]];
+        float b1, b2;
+        uniform_triangle(u, v, b1, b2);
+        const V3& p1 = s.mesh->pos[tr.v[0]]; const V3& p2 = s.mesh->pos[tr.v[1]]; const V3& p3 = s.mesh->pos[tr.v[2]];
+        Q = b1 * p1 + b2 * p2 + (1.f - b1 - b2) * p3;
+        nl = normalize(cross(p3 - p1, p2 - p1));
+        if (dot(Q - P, nl) > 0) nl = -nl;
+        nl = normalize(nl);
+    }
     float pdf = 1.0f / L.area;
     seg = Ray(P, Q - P, 1e-4f, 1.0f - 1e-4f);  // Visibility.cpp:50-53, Ray::Epsilon
     wIn = P - Q;
@@ -932,6 +953,13 @@ void orc_cosine_hemisphere(float u, float v, float out[3]) {
     out[0] = w.x; out[1] = w.y; out[2] = w.z;
 }
 void orc_uniform_triangle(float u, float v, float* b1, float* b2) { uniform_triangle(u, v, *b1, *b2); }
+void orc_quad_sample(const float quad[12], float u, float v, float point[3], float normal[3]) {
+    V3 n;
+    V3 p = quad_sample(V3(quad[0], quad[1], quad[2]), V3(quad[3], quad[4], quad[5]), V3(quad[6], quad[7], quad[8]),
+                       V3(quad[9], quad[10], quad[11]), u, v, n);
+    point[0] = p.x; point[1] = p.y; point[2] = p.z;
+    normal[0] = n.x; normal[1] = n.y; normal[2] = n.z;
+}
 
 void orc_camera_ray(const float cam[11], int width, int height, int px, int py, float u, float v,
                     float o[3], float d[3]) {
@@ -1187,7 +1215,10 @@ orc_scene* orc_scene_create(const float* pos, const float* nrm, int64_t nv, cons
         s->mats.push_back(orc_scene::Mat{V3(q[0], q[1], q[2]), V3(q[3], q[4], q[5]), q[6], (int)q[7]});
     }
     s->lights.resize(nlight);
-    for (int l = 0; l < nlight; ++l) s->lights[l].Le = V3(lights[4 * l], lights[4 * l + 1], lights[4 * l + 2]);
+    for (int l = 0; l < nlight; ++l) {
+        s->lights[l].Le = V3(lights[4 * l], lights[4 * l + 1], lights[4 * l + 2]);
+        s->lights[l].quad = lights[4 * l + 3] == 1.0f;
+    }
     for (int64_t i = 0; i < nt; ++i) {
         int l = tri_light[i];
         if (l >= 0 && l < nlight) {
@@ -1229,7 +1260,7 @@ int orc_render(const orc_scene* s, const orc_render_params* p, double* accum, in
                         Sampler smp{};
                         smp.mode = p->rng_mode;
                         smp.seed = p->seed;
-                        smp.pixel = (uint64_t)y * W + x;
+                        smp.pixel = (uint64_t)p->pix0 + (uint64_t)y * W + x;
                         smp.sample = (uint32_t)(p->spp_offset + si);
                         smp.mt = &mt;
                         smp.light_mt = &lmt;
@@ -1251,6 +1282,282 @@ int orc_render(const orc_scene* s, const orc_render_params* p, double* accum, in
     return nt;
 }
 
+
+// ---------------------------------------------------------------- irradiance cache integrator
+// ICIntegrator (Integrator/ICIntegrator.cpp) on the host: the CPU comparator
+// of the neural radiance cache. Radiance (:348-424) shades a camera hit with
+// SampleAllLights plus GetIrradiance * albedo * indirect_factor; GetIrradiance
+// (:438-484) averages the records FindRecords (:142-183) accepts with W_Ward
+// (:185-196), or computes a record (ComputeRecord, :486-543: AltitudeStrata x
+// AzimuthStrata stratified cosine gather rays from P + w eps, each followed by
+// PathLi, :545-642) and inserts it (Insert, :107-140). The octree's root cell
+// and RMin/RMax follow Initialise (:306-330). Random numbers come from the
+// counter-based streams (a record's gather path k of a camera sample draws
+// from bounce counter 64 + k * 32 + path length), not the shared sampler.
+namespace {
+
+struct IcRec { V3 P, N, E; float Ri, RiClamp; };
+struct IcNode { V3 lo, hi; int child = -1; std::vector<int> recs; };
+
+struct IcCache {
+    std::vector<IcNode> nodes;
+    std::vector<IcRec> recs;
+    int depth = 16;
+    float inv_threshold = 1.f;
+    std::shared_mutex mu;
+
+    static bool contains(const IcNode& n, V3 p) {
+        return !(n.lo.x > p.x || n.lo.y > p.y || n.lo.z > p.z || n.hi.x < p.x || n.hi.y < p.y || n.hi.z < p.z);
+    }
+    // IrradianceCache::SetBounds (:54-86): child i takes the upper half in x
+    // when bit 0 is set, in z for bit 1, in y for bit 2
+    static void child_bounds(const IcNode& n, int i, V3& lo, V3& hi) {
+        V3 c = n.lo + (n.hi - n.lo) * 0.5f;
+        lo = V3((i & 1) ? c.x : n.lo.x, (i & 4) ? c.y : n.lo.y, (i & 2) ? c.z : n.lo.z);
+        hi = V3((i & 1) ? n.hi.x : c.x, (i & 4) ? n.hi.y : c.y, (i & 2) ? n.hi.z : c.z);
+    }
+    static bool sphere_box(V3 lo, V3 hi, V3 c, float r) {  // SphereBoxOverlap (:88-101)
+        float d = 0;
+        const float cc[3] = {c.x, c.y, c.z}, ll[3] = {lo.x, lo.y, lo.z}, hh[3] = {hi.x, hi.y, hi.z};
+        for (int i = 0; i < 3; ++i) {
+            if (cc[i] < ll[i]) d += (cc[i] - ll[i]) * (cc[i] - ll[i]);
+            else if (cc[i] > hh[i]) d += (cc[i] - hh[i]) * (cc[i] - hh[i]);
+        }
+        return d <= r * r;
+    }
+    void insert(int n, int r, int d) {
+        const IcRec& rec = recs[r];
+        V3 ext = (nodes[n].hi - nodes[n].lo) * 0.5f;   // AxisAlignedBoundingBox::GetExtent
+        float half = rmax(std::fabs(ext.x), rmax(std::fabs(ext.y), std::fabs(ext.z)));
+        if (d <= 0 || rec.RiClamp > half) {
+            nodes[n].recs.push_back(r);
+            return;
+        }
+        if (nodes[n].child < 0) {
+            int first = (int)nodes.size();
+            nodes[n].child = first;
+            for (int i = 0; i < 8; ++i) {
+                IcNode c;
+                child_bounds(nodes[n], i, c.lo, c.hi);
+                nodes.push_back(c);
+            }
+        }
+        const int first = nodes[n].child;
+        for (int i = 0; i < 8; ++i)
+            if (sphere_box(nodes[first + i].lo, nodes[first + i].hi, rec.P, rec.RiClamp)) insert(first + i, r, d - 1);
+    }
+    float weight(V3 p, V3 nrm, const IcRec& r) const {  // W_Ward
+        float dist = length(p - r.P) / r.RiClamp, c = dot(nrm, r.N);
+        if (c <= 0) return -1.f;
+        return 1.f / (dist + 1.f - c) - inv_threshold;
+    }
+    bool lookup(V3 p, V3 nrm, V3& E) const {
+        int n = 0;
+        if (!contains(nodes[0], p)) return false;
+        double num[3] = {0, 0, 0}, den = 0;
+        for (;;) {
+            for (int r : nodes[n].recs) {
+                float w = weight(p, nrm, recs[r]);
+                if (w > 0.f) {
+                    num[0] += w * recs[r].E.x; num[1] += w * recs[r].E.y; num[2] += w * recs[r].E.z;
+                    den += w;
+                }
+            }
+            const int c = nodes[n].child;
+            if (c < 0) break;
+            int k = 0;
+            while (k < 8 && !contains(nodes[c + k], p)) ++k;
+            if (k == 8) break;
+            n = c + k;
+        }
+        if (den <= 0) return false;
+        E = V3((float)(num[0] / den), (float)(num[1] / den), (float)(num[2] / den));
+        return true;
+    }
+};
+
+struct IcStats { int64_t camera = 0, gather = 0, shadow = 0, created = 0; };
+
+// SampleAllLights at a surface, excluding the light that was hit (Integrator.cpp:114-185)
+V3 direct_light(const orc_scene& s, const Surface& sf, const orc_scene::Mat& mat, V3 wOut, uint32_t seed,
+                uint64_t pixel, uint32_t sample, uint32_t bounce, int64_t& shadow) {
+    V3 Ls;
+    Basis gb = basis_from_w(sf.Ng);
+    for (int l = 0; l < (int)s.lights.size(); ++l) {
+        if (l == sf.light) continue;
+        Block8 b = philox_block(seed, pixel, sample, bounce, 1u + (uint32_t)l);
+        V3 wIn;
+        Ray seg;
+        V3 Le = light_sample(s, l, sf.P, b.v[0], b.v[1], b.v[2], wIn, seg);
+        if (is_black(Le)) continue;
+        ++shadow;
+        int op;
+        if (mesh_any(*s.mesh, seg, 1, op)) continue;
+        wIn = -wIn;
+        V3 f = bsdf_f(mat, to_local(gb, wOut), to_local(gb, wIn));
+        Ls = Ls + mul(Le * rmax(0.f, dot(wIn, sf.Ns)), f);
+    }
+    return Ls;
+}
+
+// ICIntegrator::PathLi (:545-642): NEE at every vertex, no emission (no
+// specular bounces), the next ray from P with tmin eps, roulette after the
+// third vertex with q = min(0.5, 0.33 (r + g + b)). first_t = the first hit's
+// distance (1 when the gather ray finds no material, as PathLi leaves it).
+V3 path_li(const orc_scene& s, Ray ray, int ray_depth, float eps, uint32_t seed, uint64_t pixel, uint32_t sample,
+           uint32_t ctr0, float& first_t, IcStats& st) {
+    V3 L, beta(1.f, 1.f, 1.f);
+    first_t = 1.f;
+    for (int len = 0;; ++len) {
+        Hit h;
+        ++st.gather;
+        if (!mesh_closest(*s.mesh, ray, 1, h.prim, h.t, h.b, h.g)) break;
+        Surface sf = make_surface(s, ray, h);
+        if (sf.mat < 0) break;
+        if (len == 0) first_t = h.t;
+        const orc_scene::Mat& mat = s.mats[sf.mat];
+        V3 wOut = -ray.d;
+        L = L + mul(beta, direct_light(s, sf, mat, wOut, seed, pixel, sample, ctr0 + (uint32_t)len, st.shadow));
+        if (len + 1 == ray_depth) break;
+        Block8 b = philox_block(seed, pixel, sample, ctr0 + (uint32_t)len, 0);
+        Basis sb = basis_from_w(sf.Ns);
+        V3 woL = to_local(sb, wOut), wiL;
+        float pdf = 0;
+        V3 f = bsdf_sample(mat, woL, wiL, b.v[2], b.v[3], b.v[4], pdf);
+        if (is_black(f) || pdf == 0.0f) break;
+        V3 wIn = to_world(sb, wiL);
+        ray = Ray(sf.P, wIn, eps, kFltMax);
+        beta = mul(beta, f * std::fabs(dot(wIn, sf.Ns)) / pdf);
+        if (len > 2) {
+            float q = rmin(0.5f, 0.33f * (beta.x + beta.y + beta.z));
+            if (b.v[5] > q) break;
+            beta = beta / q;
+        }
+    }
+    return L;
+}
+
+}  // namespace
+
+extern "C" int orc_ic_render(const orc_scene* s, const orc_render_params* p, const orc_ic_params* icp,
+                             double* accum, int64_t counts[4], int nthreads) {
+    const int W = p->width, H = p->height;
+    int x0 = p->x0, y0 = p->y0, x1 = p->x1 ? p->x1 : W, y1 = p->y1 ? p->y1 : H;
+    Camera cam = make_camera(p->cam);
+    const orc_mesh& m = *s->mesh;
+    // Initialise (:306-330): root cell from the scene's bounds
+    V3 lo(kFltMax, kFltMax, kFltMax), hi(-kFltMax, -kFltMax, -kFltMax);
+    for (const V3& v : m.pos) {
+        lo = V3(rmin(lo.x, v.x), rmin(lo.y, v.y), rmin(lo.z, v.z));
+        hi = V3(rmax(hi.x, v.x), rmax(hi.y, v.y), rmax(hi.z, v.z));
+    }
+    V3 size = hi - lo, centre = lo + size * 0.5f;
+    float edge = rmax(std::fabs(size.x), rmax(std::fabs(size.y), std::fabs(size.z)));
+    V3 a = (centre - V3(edge, edge, edge)) * (1.0f + FLT_EPSILON), b = (centre + V3(edge, edge, edge)) * (1.0f + FLT_EPSILON);
+    IcCache cache;
+    IcNode root;
+    root.lo = V3(rmin(a.x, b.x), rmin(a.y, b.y), rmin(a.z, b.z));
+    root.hi = V3(rmax(a.x, b.x), rmax(a.y, b.y), rmax(a.z, b.z));
+    cache.nodes.push_back(root);
+    cache.depth = icp->cache_depth;
+    cache.inv_threshold = 1.f / icp->error_threshold;
+    const float rmin_ = edge * icp->ambient_resolution, rmax_ = edge * icp->ambient_multiplier;
+    const int M = icp->altitude_strata, N = icp->azimuth_strata, mn = M * N;
+    const int tile = 32;
+    int tx = (x1 - x0 + tile - 1) / tile, ty = (y1 - y0 + tile - 1) / tile;
+    int nt = resolve_threads(nthreads);
+    std::atomic<int64_t> c_cam(0), c_gather(0), c_shadow(0), c_created(0);
+    parallel_chunks((int64_t)tx * ty, 1, nt, [&](int64_t t0, int64_t t1) {
+        IcStats st;
+        for (int64_t ti = t0; ti < t1; ++ti) {
+            int bx = x0 + (int)(ti % tx) * tile, by = y0 + (int)(ti / tx) * tile;
+            for (int y = by; y < std::min(by + tile, y1); ++y) {
+                for (int x = bx; x < std::min(bx + tile, x1); ++x) {
+                    const uint64_t pixel = (uint64_t)p->pix0 + (uint64_t)y * W + x;
+                    double acc[3] = {0, 0, 0};
+                    for (int si = 0; si < p->spp; ++si) {
+                        const uint32_t sample = (uint32_t)(p->spp_offset + si);
+                        Block8 cb = philox_block(p->seed, pixel, sample, 0, 0);
+                        Ray r = camera_ray(cam, W, H, x, y, cb.v[0], cb.v[1]);
+                        ++st.camera;
+                        Hit h;
+                        V3 L;
+                        if (!mesh_closest(m, r, 1, h.prim, h.t, h.b, h.g)) {
+                            L = V3(icp->miss_radiance[0], icp->miss_radiance[1], icp->miss_radiance[2]);
+                        } else {
+                            Surface sf = make_surface(*s, r, h);
+                            if (sf.mat >= 0) {
+                                const orc_scene::Mat& mat = s->mats[sf.mat];
+                                V3 wOut = -normalize(r.d);
+                                if (sf.light >= 0) {
+                                    if (dot(sf.Ns, wOut) > 0) L = s->lights[sf.light].Le;
+                                } else {
+                                    L = direct_light(*s, sf, mat, wOut, p->seed, pixel, sample, 0, st.shadow);
+                                    V3 rho = mat.type == 1 ? mat.kd + mat.ks : mat.kd;
+                                    V3 E;
+                                    bool found;
+                                    {
+                                        std::shared_lock<std::shared_mutex> g(cache.mu);
+                                        found = cache.lookup(sf.P, sf.Ns, E);
+                                    }
+                                    if (!found) {
+                                        // ComputeRecord (:486-543)
+                                        V3 sum;
+                                        float inv_len = 0;
+                                        Basis sb = basis_from_w(sf.Ns);
+                                        for (int ai = 0; ai < M; ++ai) {
+                                            for (int zi = 0; zi < N; ++zi) {
+                                                const int k = ai * N + zi;
+                                                Block8 gb = philox_block(p->seed, pixel, sample, 63, 1u + (uint32_t)k);
+                                                // Montecarlo::CosineSampleHemisphere with strata (Montecarlo.h:20-29)
+                                                float su = (gb.v[0] + (float)ai) / (float)M;
+                                                float sv = (gb.v[1] + (float)zi) / (float)N;
+                                                float ph = kPiTwo * sv, rr = std::sqrt(su);
+                                                V3 vh(rr * std::cos(ph), rr * std::sin(ph), std::sqrt(rmax(0.f, 1.f - su)));
+                                                V3 w = to_world(sb, vh);
+                                                float ft;
+                                                sum = sum + path_li(*s, Ray(sf.P + w * icp->eps, w, icp->eps, kFltMax),
+                                                                    icp->ray_depth, icp->eps, p->seed, pixel, sample,
+                                                                    64u + 32u * (uint32_t)k, ft, st);
+                                                inv_len += 1.f / ft;
+                                            }
+                                        }
+                                        IcRec rec;
+                                        rec.P = sf.P;
+                                        rec.N = sf.Ns;
+                                        rec.E = sum / (float)mn;
+                                        rec.Ri = (float)mn / inv_len;
+                                        rec.RiClamp = rmax(rmin_, rmin(rmax_, rec.Ri));
+                                        E = rec.E;
+                                        ++st.created;
+                                        std::unique_lock<std::shared_mutex> g(cache.mu);
+                                        cache.recs.push_back(rec);
+                                        cache.insert(0, (int)cache.recs.size() - 1, cache.depth);
+                                    }
+                                    L = L + mul(E, rho) * icp->indirect_factor;
+                                }
+                            }
+                        }
+                        acc[0] += L.x; acc[1] += L.y; acc[2] += L.z;
+                    }
+                    double* o = accum + 3 * ((int64_t)y * W + x);
+                    o[0] += acc[0]; o[1] += acc[1]; o[2] += acc[2];
+                }
+            }
+        }
+        c_cam += st.camera;
+        c_gather += st.gather;
+        c_shadow += st.shadow;
+        c_created += st.created;
+    });
+    if (counts) {
+        counts[0] = c_cam.load();
+        counts[1] = c_gather.load();
+        counts[2] = c_shadow.load();
+        counts[3] = c_created.load();
+    }
+    return nt;
+}
 
 // Local-frame BSDF of a material record (kd rgb, ks rgb, exponent, type):
 // f, and the pdf BSDF::SampleF reports for a direction it sampled

@@ -49,6 +49,9 @@ void orc_philox4x64(const uint64_t ctr[4], const uint64_t key[2], uint64_t out[4
 /* Warps, Core/Maths/Montecarlo.h:30-37 (cosine hemisphere), :60-64 (triangle). */
 void orc_cosine_hemisphere(float u, float v, float out[3]);
 void orc_uniform_triangle(float u, float v, float* b1, float* b2);
+/* Quad::SamplePoint (Shape/Quad.cpp:146-153) for the quad v1..v4 (quad =
+ * 12 floats): point and the normalised winding normal cross(v3-v1, v2-v1). */
+void orc_quad_sample(const float quad[12], float u, float v, float point[3], float normal[3]);
 
 /* Camera ray, Core/Camera/PerspectiveCamera.cpp:44-51 with the frame of
  * Camera.cpp:30-44/:72-78, Basis.cpp:28-44 and the half-pixel shift of
@@ -124,8 +127,11 @@ typedef struct orc_scene orc_scene;
 /* Material record, 8 floats: kd rgb, ks rgb, phong exponent, type
  * (0 = Lambertian (Material/LambertianBxDF.cpp), 1 = Lambertian + Phong lobe;
  * the Phong lobe has no counterpart in the reference). */
-/* Light record, 4 floats: Le rgb, unused. The triangles whose tri_light
- * equals the light's index form its emitter mesh. */
+/* Light record, 4 floats: Le rgb, sampling mode. The triangles whose
+ * tri_light equals the light's index form its emitter: mode 0 samples it as
+ * ITriangleMesh (a triangle by index, the normal turned toward the shaded
+ * point; Shape/TriangleMesh.cpp:143-161), mode 1 as the reference's Quad
+ * from its two triangles (v1, v3, v2), (v1, v4, v3) (orc_quad_sample). */
 orc_scene* orc_scene_create(const float* pos, const float* nrm, int64_t nv,
                             const int32_t* idx, const int32_t* tri_mat,
                             const int32_t* tri_light, int64_t nt,
@@ -144,6 +150,8 @@ typedef struct {
     uint32_t seed;
     int32_t rng_mode;   /* 0: Philox counter-based, 1: MT19937 per tile */
     int32_t x0, y0, x1, y1; /* pixel window (x1/y1 exclusive; 0 = full) */
+    int64_t pix0;       /* added to the pixel index of the counter-based streams
+                           (a view's first pixel in a batch of images) */
 } orc_render_params;
 
 /* RenderThread-style pool (Utility/Sandbox/Multithreaded.h:28-264) over
@@ -155,6 +163,27 @@ typedef struct {
  * {extension+camera rays, shadow rays}. Returns the threads used. */
 int orc_render(const orc_scene* s, const orc_render_params* p, double* accum,
                int64_t ray_counts[2], int nthreads);
+
+/* ICIntegrator on the host (Integrator/ICIntegrator.cpp:306-642): camera
+ * samples as orc_render draws them; a hit is shaded with SampleAllLights plus
+ * the cache's irradiance x albedo x indirect_factor (the reference: 1/pi,
+ * :394); records are computed and inserted on a cache miss, shared by all
+ * threads (a reader-writer lock; the reference inserts without one); misses
+ * return miss_radiance. counts = {camera rays, gather path rays, shadow
+ * rays, records created}. Returns the threads used. */
+typedef struct {
+    int32_t cache_depth;       /* CacheDepth */
+    float error_threshold;     /* ErrorThreshold */
+    float ambient_resolution;  /* RMin = longest scene edge x this */
+    float ambient_multiplier;  /* RMax = longest scene edge x this */
+    int32_t azimuth_strata, altitude_strata;
+    int32_t ray_depth;         /* RayDepth of the gather paths */
+    float eps;                 /* Epsilon */
+    float indirect_factor;
+    float miss_radiance[3];
+} orc_ic_params;
+int orc_ic_render(const orc_scene* s, const orc_render_params* p, const orc_ic_params* icp, double* accum,
+                  int64_t counts[4], int nthreads);
 
 /* ---- materials -------------------------------------------------------- */
 

@@ -453,3 +453,61 @@ def test_direct_lighting_matches_lambert_formula(orc):
             got = img[y, x, 0] / spp
             # pixel footprint: the sub-pixel jitter moves P by < 1 pixel; allow 3%
             assert abs(got - want) < 0.03 * want, (x, y, got, want)
+
+
+def test_quad_sample_map(orc):
+    """Quad::SamplePoint (Shape/Quad.cpp:146-153) on the unit square v1..v4 =
+    (0,0), (1,0), (1,1), (0,1): the corners of the parameter square go to v1,
+    v3, v1 + v4 - v3 and v1 + v2 - v3, so the map's image is not the quad
+    (its centre goes to v1, not to the quad's centre); the normal is the
+    normalised cross(v3 - v1, v2 - v1)."""
+    v = [np.array(q, np.float32) for q in ([0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0])]
+    want = {(0, 0): v[0], (1, 0): v[2], (0, 1): v[0] + v[3] - v[2], (1, 1): v[0] + v[1] - v[2]}
+    for (a, b), q in want.items():
+        p, n = orc.quad_sample(*v, a, b)
+        assert np.allclose(p, q, atol=1e-7) and np.allclose(n, [0, 0, -1])
+    p, _ = orc.quad_sample(*v, 0.5, 0.5)
+    assert np.allclose(p, v[0], atol=1e-7)
+
+
+def test_quad_light_lights_the_image_of_its_map(orc):
+    """A quad light (mode 1) whose map is affine (v2 = v3 + v4 - v1) samples
+    the parallelogram v1, v3, v4, v1 + v4 - v3 uniformly with density
+    1 / area (the map's Jacobian equals the two triangles' area), so with
+    depth 1 a floor pixel is rho/pi * E(P), E from Lambert's formula over that
+    parallelogram -- here the square [-0.3, 0.3]^2 at y = 1 -- not over the
+    emitter's own triangles; sampling the same emitter as a mesh light (mode
+    0) gives Lambert's formula over its two (equal-area) triangles instead."""
+    rho, le = 0.6, 3.0
+    fp, fi = meshgen.quad([-4, 0, -4], [-4, 0, 4], [4, 0, 4], [4, 0, -4])     # floor, normal +y
+    v1, v3, v4 = np.array([-0.3, 1, -0.3]), np.array([0.3, 1, -0.3]), np.array([0.3, 1, 0.3])
+    v2 = v3 + v4 - v1
+    lp = np.array([v1, v2, v3, v4], np.float32)
+    li = np.array([[0, 2, 1], [0, 3, 2]], np.int32)                           # (v1, v3, v2), (v1, v4, v3)
+    assert np.cross(v3 - v1, v2 - v1)[1] < 0                                  # winding normal faces down
+    pos, idx = meshgen.merge([(fp, fi), (lp, li)])
+    nrm = np.array([[0, 1, 0]] * 4 + [[0, -1, 0]] * 4, np.float32)
+    mats = np.array([[rho] * 3 + [0, 0, 0, 1, 0], [0, 0, 0, 0, 0, 0, 1, 0]], np.float32)
+    cam = [0.0, 0.8, 0.0, 0.0, 0.0, 0.0, 0, 0, 1, 60.0, 1.0]
+    W, spp = 16, 4000
+    square = np.array([[-0.3, 1, -0.3], [0.3, 1, -0.3], [0.3, 1, 0.3], [-0.3, 1, 0.3]])
+    imgs, apart = {}, 0.0
+    for mode in (1, 0):
+        s = orc.Scene(pos, nrm, idx, np.array([0, 0, 1, 1], np.int32), np.array([-1, -1, 0, 0], np.int32), mats,
+                      np.array([[le, le, le, mode]], np.float32))
+        imgs[mode] = s.render(cam, W, W, spp, max_depth=1, seed=3, window=(4, 4, 12, 12))
+    for y in range(4, 12, 3):
+        for x in range(4, 12, 3):
+            o, d = orc.camera_ray(cam, W, W, x, y, 0.5, 0.5)
+            P = o + (-o[1] / d[1]) * d
+            up = np.array([0, 1, 0])
+            want = rho / math.pi * _lambert_polygon_irradiance(P, up, square, le)
+            got = imgs[1][y, x, 0] / spp
+            assert abs(got - want) < 0.03 * want, (x, y, got, want)
+            # mesh-light sampling of the same (equal-area) triangles: Lambert over each triangle
+            want0 = rho / math.pi * (_lambert_polygon_irradiance(P, up, np.array([v1, v3, v2]), le) +
+                                     _lambert_polygon_irradiance(P, up, np.array([v1, v4, v3]), le))
+            got0 = imgs[0][y, x, 0] / spp
+            assert abs(got0 - want0) < 0.03 * want0, (x, y, got0, want0)
+            apart = max(apart, abs(want0 - want) / want)
+    assert apart > 0.1   # the two closed forms are told apart
