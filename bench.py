@@ -7,13 +7,22 @@ illuminaprt_amd.workloads. A step is one closest-hit launch over the
 sets are timed the same way and reported beside it.
 
 Prints one JSON line. `value` = incoherent Mrays/s. `roofline` prices the
-traversal kernel against HBM bandwidth with algorithmic bytes per ray
-= 32 (ray) + 16 (hit) + N_node * S_node + N_tri * 48 (S_node = 112 or 64 bytes by
-node format), where N_node and N_tri are
-the mean node visits and triangle tests of the same traversal over the same
-rays (counted by the kernel's statistics variant). `cpu_baseline` times the
-oracle (the C++ restatement of the reference BVHMesh build + recursive
-traversal) on a bounded sample of the same incoherent rays on all host cores.
+traversal kernel at two levels of the memory hierarchy:
+- HBM (`roofline.achieved` / `frac`): the bytes a launch must move through
+  HBM -- 32 B read per ray, 16 B written per hit (1 B per occlusion flag)
+  and the tree and triangle arrays read once -- per launch duration, against
+  the 8 TB/s peak. `traffic` is the HBM traffic per launch measured by the
+  FETCH_SIZE / WRITE_SIZE counters (profiles/traffic_trace_incoherent.json,
+  used only when it was measured on the current traversal source);
+- L2 (`roofline.l2`): bytes per ray = 32 + 16 + N_node * S_node + N_tri * 48
+  (S_node = 112 bytes, float BVH4), with N_node and N_tri the mean node
+  visits and triangle tests of a deterministic ordered traversal of the same
+  flattened tree over the same rays on the host (ilm_bvh_visit_counts), per
+  launch duration, against the L2 read bandwidth this run measures with the
+  cache-bandwidth probe (ilm_read_probe over a 2 MiB buffer).
+`cpu_baseline` times the oracle (the C++ restatement of the reference BVHMesh
+build + recursive traversal) on a bounded sample of the same incoherent rays
+on all host cores.
 """
 import argparse
 import json
@@ -55,12 +64,89 @@ def time_launches(ctx, launch, steps):
     return wall, per
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_trace_incoherent.json")
+TRACE_SOURCES = [os.path.join(ROOT, "illuminaprt_amd", "csrc", f) for f in ("trace.hip", "geom.hpp")]
+
+
+def trace_source_sha16():
+    """Digest of the traversal kernel's sources: a traffic measurement counts
+    for this run only if it was taken on the same sources."""
+    import hashlib
+    h = hashlib.sha256()
+    for f in TRACE_SOURCES:
+        with open(f, "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
 def traffic_from_profiles():
-    p = os.path.join(ROOT, "profiles", "traffic_trace_incoherent.json")
-    if not os.path.exists(p):
-        return None
-    with open(p) as f:
-        return json.load(f).get("hbm_bytes_per_launch")
+    """(HBM bytes per incoherent launch, note): from the committed counter
+    summary when its source digest matches the current traversal sources."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None, "no counter summary"
+    with open(TRAFFIC_FILE) as f:
+        d = json.load(f)
+    rel = os.path.relpath(TRAFFIC_FILE, ROOT)
+    if d.get("trace_source_sha16") != trace_source_sha16():
+        return None, f"{rel} was measured on other traversal sources ({d.get('trace_source_sha16')})"
+    return d.get("hbm_bytes_per_launch"), f"{rel} (FETCH_SIZE and WRITE_SIZE, traversal sources {d['trace_source_sha16']})"
+
+
+def pmc_summary(dirs, copy_mib, out_path=TRAFFIC_FILE):
+    """Reads four rocprofv3 --pmc runs (CSV output directories: trace FETCH_SIZE,
+    trace WRITE_SIZE, copy FETCH_SIZE, copy WRITE_SIZE), scales each trace
+    counter by what it reads for the plain copy of copy_mib MiB each way,
+    writes the traffic summary JSON and prints it."""
+    import csv
+    import glob
+
+    def per_dispatch(d, counter):
+        vals = {}
+        for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            with open(f) as fh:
+                for row in csv.DictReader(fh):
+                    if row.get("Counter_Name") != counter:
+                        continue
+                    key = (int(row["Dispatch_Id"]), row["Kernel_Name"])
+                    vals[key] = vals.get(key, 0.0) + float(row["Counter_Value"])
+        return [(k[0], k[1], v) for k, v in sorted(vals.items())]
+
+    tf, tw, cf, cw = dirs
+    kib = copy_mib * 1024.0
+    copy_f = [v for _, k, v in per_dispatch(cf, "FETCH_SIZE") if "k_copy" in k]
+    copy_w = [v for _, k, v in per_dispatch(cw, "WRITE_SIZE") if "k_copy" in k]
+    fs, ws = kib / float(np.mean(copy_f)), kib / float(np.mean(copy_w))
+    name = "k_trace4<false, false, 0, false>"
+    tr_f = [v for _, k, v in per_dispatch(tf, "FETCH_SIZE") if name in k]
+    tr_w = [v for _, k, v in per_dispatch(tw, "WRITE_SIZE") if name in k]
+    # the last six dispatches trace the incoherent set (1 warm-up + 5 timed)
+    f_kib, w_kib = float(np.mean(tr_f[-6:])), float(np.mean(tr_w[-6:]))
+    out = {"kernel": "ilm::" + name, "workload": "incoherent rays, 1,334,720 triangles, 4,194,304 rays",
+           "trace_source_sha16": trace_source_sha16(),
+           "fetch_kib_per_dispatch": round(f_kib, 1), "write_kib_per_dispatch": round(w_kib, 1),
+           "copy_fetch_kib": round(float(np.mean(copy_f)), 1), "copy_write_kib": round(float(np.mean(copy_w)), 1),
+           "fetch_scale": round(fs, 4), "write_scale": round(ws, 4),
+           "hbm_bytes_per_launch": int(round((f_kib * fs + w_kib * ws) * 1024.0))}
+    with open(out_path, "w") as f:
+        json.dump(out, f, indent=1)
+        f.write("\n")
+    print(json.dumps(out))
+
+
+def l2_peak_gbs(ctx, steps=5):
+    """Aggregate L2 read bandwidth measured by the cache-bandwidth probe: a
+    2 MiB buffer (inside each XCD's 4 MiB L2, far beyond a CU's 32 KiB L1)
+    read 8 times by 8 workgroups per CU."""
+    nbytes, passes, per_cu = 2 << 20, 8, 8
+    src, sink = ctx.buffer(nbytes).zero(), ctx.buffer(16)
+    launch = lambda: ctx.read_probe(src, nbytes, passes, per_cu, sink)
+    for _ in range(2):
+        launch()
+    _, per = time_launches(ctx, launch, steps)
+    src.free()
+    sink.free()
+    ms = float(np.mean(per))
+    return ctx.cu_count * per_cu * passes * nbytes / (ms * 1e-3) / 1e9, ms
 
 
 def cpu_baseline(pos, idx, rays_host, budget_s=15.0):
@@ -161,6 +247,33 @@ def frames_leg(ctx, ilm, workloads, args):
     return out
 
 
+def ic_frames_leg(ctx, ilm, workloads, args):
+    """Six irradiance-cache frames of the radiance-cache workload (1920 x 1080,
+    1 spp each, the cache kept across frames, error threshold --ic-threshold,
+    indirect factor 1, misses 0): a short run for kernel-level profiles of the
+    irradiance cache's placement passes, lookups and gather paths."""
+    W, H, depth = args.cache_width, args.cache_height, 10
+    pos, nrm, idx, tm, tl, mats, lights = workloads.cache_scene()
+    mesh = ilm.Mesh(ctx, pos, idx)
+    mesh.build()
+    sc = ilm.Scene(ctx, mesh, nrm, tm, tl, mats, lights)
+    cam = workloads.cache_camera(W / H)
+    acc = ctx.buffer(W * H * 24).zero()
+    ic = ilm.IrradianceCache(sc, ray_depth=depth - 1, error_threshold=args.ic_threshold, indirect_factor=1.0,
+                             miss_radiance=(0.0, 0.0, 0.0))
+    frames = [ic.render(cam, W, H, 1, acc, seed=30001 + f, spp_offset=f) for f in range(6)]
+    ic.close()
+    acc.free()
+    sc.close()
+    mesh.close()
+    out = {"workload": f"radiance-cache scene {len(idx)} tris, {W}x{H}, 1 spp, irradiance cache "
+                       f"(threshold {args.ic_threshold})",
+           "frame_ms": [round(f["ms"], 2) for f in frames], "records_total": [int(f["records_total"]) for f in frames],
+           "gather_rays": [int(f["gather_rays"]) for f in frames]}
+    log(f"ic frames: {out}")
+    return out
+
+
 def _trace_rate(ctx, mesh, buf, n, steps, warmup, hits):
     for _ in range(warmup):
         mesh.trace_closest(buf, n, hits)
@@ -219,6 +332,57 @@ def large_scene_leg(ctx, ilm, workloads, args):
     return out
 
 
+def instances_leg(ctx, ilm, workloads, args):
+    """The flat instance table: 2,048 instances of three meshes under a
+    top-level tree, against the same instances flattened into one mesh
+    (device LBVH). 2048 x 2048 camera rays and their cosine bounces (made on
+    the flattened mesh), closest hits by both; the table's device bytes
+    against the flattened mesh's, and the time of ilm_instances_set (the
+    top-level rebuild of an update)."""
+    meshes, placements = workloads.instanced_scene()
+    gm = []
+    for pos, idx in meshes:
+        m = ilm.Mesh(ctx, pos, idx)
+        m.build()
+        gm.append(m)
+    table = ilm.Instances(ctx, gm)
+    t0 = time.perf_counter()
+    table.set(placements)
+    set_ms = (time.perf_counter() - t0) * 1e3
+    info = table.info()
+    fpos, fidx = workloads.flatten_placements(meshes, placements)
+    flat = ilm.Mesh(ctx, fpos, fidx)
+    finfo = flat.build(builder=ilm.ILM_BUILDER_LBVH)
+    W = H = 2048
+    n = W * H
+    rs = workloads.intersection_rays(ctx, flat, W, H, cam=workloads.instanced_camera(),
+                                     light=np.array([-2.0, 12.0, -2.0, 4.0, 0, 0, 0, 0, 4.0], np.float32))
+    hits, inst = ctx.buffer(n * HIT_BYTES), ctx.buffer(n * 4)
+    steps = max(3, args.steps // 4)
+    out = {"workload": f"{len(placements)} instances of {len(meshes)} meshes ({len(fidx)} triangles flattened), "
+                       f"{W}x{H} camera rays and their cosine bounces",
+           "instances": info["instances"], "table_bytes": info["table_bytes"], "mesh_bytes": info["mesh_bytes"],
+           "tlas_bytes": info["tlas_bytes"], "flattened_bytes": finfo["node_bytes"] + finfo["tri_bytes"],
+           "set_ms": round(set_ms, 2)}
+    for name, buf in (("primary", rs.primary), ("incoherent", rs.incoherent)):
+        for _ in range(args.warmup):
+            table.trace_closest(buf, n, hits, inst)
+        _, per = time_launches(ctx, lambda: table.trace_closest(buf, n, hits, inst), steps)
+        out[f"{name}_mrays_s"] = round(n / float(np.mean(per)) / 1e3, 1)
+        rate, _ = _trace_rate(ctx, flat, buf, n, steps, args.warmup, hits)
+        out[f"{name}_flattened_mrays_s"] = rate
+    table.check()
+    for b in (hits, inst):
+        b.free()
+    rs.free()
+    flat.close()
+    table.close()
+    for m in gm:
+        m.close()
+    log(f"instances: {out}")
+    return out
+
+
 def large_render_leg(ctx, ilm, workloads, args):
     """11,085,808 triangles (two displaced level-9 icospheres, 50,000 boxes and
     a 48-triangle room with three panel lights; 30% glossy), eight 3840 x 2160
@@ -252,7 +416,7 @@ def large_render_leg(ctx, ilm, workloads, args):
     acc.zero()
     plain = [b.render(views, acc, max_depth=depth, seed=301 + i) for i in range(args.large_frames)]
     plain_means = view_means() / args.large_frames
-    net = ilm.Nrc(ctx, *workloads.LARGE_BOUNDS, lr=args.cache_lr, seed=5)
+    net = ilm.Nrc(ctx, *workloads.LARGE_BOUNDS, lr=args.large_lr, seed=5)
     acc.zero()
     skip, cache = args.large_skip, []
     for i in range(skip + args.large_frames):
@@ -334,7 +498,10 @@ def batch_leg(ctx, ilm, workloads, args):
     return out
 
 
-MLP_FLOP_PER_QUERY = 2 * (96 * 64 + 4 * 64 * 64 + 64 * 16)  # padded network as executed
+# the network's algorithmic FLOP per query: 83 inputs -> 5 x 64 (ReLU) -> 3
+MLP_FLOP_PER_QUERY = 2 * (83 * 64 + 4 * 64 * 64 + 64 * 3)
+# as executed: inputs padded to 96, outputs to 16
+MLP_FLOP_EXECUTED = 2 * (96 * 64 + 4 * 64 * 64 + 64 * 16)
 MFMA_BF16_PEAK_TFLOPS = 2500.0
 
 
@@ -414,32 +581,46 @@ def cache_convergence(ctx, ilm, workloads, args):
                  "mrays_s": round(plain_rays / plain_ms / 1e3, 1), "trace": trace}
     log(f"cache plain: {plain_out}")
 
-    # Neural cache: 1-spp frames with one training step each; the image is the
-    # running mean of frames from `skip` on (earlier frames see a cold cache).
-    net = ilm.Nrc(ctx, *workloads.CACHE_BOUNDS, lr=args.cache_lr, seed=5)
-    buf.zero()
-    nrc_ms, frames, acc_frames, ntrace = 0.0, 0, 0, []
-    nrc_hit = None
-    skip = args.cache_skip
-    while frames < args.cache_max_frames:
-        st = sc.render_nrc(net, cam, W, H, 1, buf if frames >= skip else None, seed=20001 + frames,
-                           prefix_depth=prefix, suffix_depth=depth - prefix, train_fraction=train_fraction(args, W * H),
-                           train_batch=65536, train_steps=args.cache_train_steps)
-        nrc_ms += st["ms"]
-        frames += 1
-        if frames > skip:
-            acc_frames += 1
-            if acc_frames in (1, 2, 4) or acc_frames % 4 == 0:
-                e, raw = _corrected(buf.download(np.float64, n3).reshape(H, W, 3) / acc_frames, ref, noise2)
-                ntrace.append((frames, round(nrc_ms, 2), round(e, 5)))
-                if e <= target:
-                    nrc_hit = (frames, nrc_ms)
-                    break
-    nrc_out = {"ms_per_frame": round(nrc_ms / frames, 3), "frames_to_target": nrc_hit[0] if nrc_hit else None,
-               "ms_to_target": round(nrc_hit[1], 1) if nrc_hit else None, "skip_frames": skip,
-               "train_steps_per_frame": args.cache_train_steps,
-               "train_fraction": round(train_fraction(args, W * H), 5), "trace": ntrace}
+    # Neural cache: 1-spp frames, each followed by its Adam steps; the image is
+    # the running mean of frames from `skip` on (earlier frames see a cold
+    # cache). The configured run trains at Adam lr 1e-3 on 65,536-record
+    # batches from 1/16 of the paths; the variant at a larger learning rate on
+    # one batch filled from fewer paths.
+    def nrc_run(lr, frac, steps, skip, seed):
+        net = ilm.Nrc(ctx, *workloads.CACHE_BOUNDS, lr=lr, seed=5)
+        buf.zero()
+        nrc_ms, frames, acc_frames, ntrace = 0.0, 0, 0, []
+        nrc_hit = None
+        records = 0
+        while frames < args.cache_max_frames:
+            st = sc.render_nrc(net, cam, W, H, 1, buf if frames >= skip else None, seed=seed + frames,
+                               prefix_depth=prefix, suffix_depth=depth - prefix, train_fraction=frac,
+                               train_batch=65536, train_steps=steps)
+            nrc_ms += st["ms"]
+            records = st["train_records"]
+            frames += 1
+            if frames > skip:
+                acc_frames += 1
+                if acc_frames in (1, 2, 4) or acc_frames % 4 == 0:
+                    e, raw = _corrected(buf.download(np.float64, n3).reshape(H, W, 3) / acc_frames, ref, noise2)
+                    ntrace.append((frames, round(nrc_ms, 2), round(e, 5)))
+                    if e <= target:
+                        nrc_hit = (frames, nrc_ms)
+                        break
+        out = {"adam_lr": lr, "train_fraction": round(frac, 6), "train_records_per_frame": int(records),
+               "train_batch": 65536, "train_steps_per_frame": steps, "skip_frames": skip,
+               "ms_per_frame": round(nrc_ms / frames, 3), "frames_to_target": nrc_hit[0] if nrc_hit else None,
+               "ms_to_target": round(nrc_hit[1], 1) if nrc_hit else None, "trace": ntrace}
+        return out, net
+
+    nrc_out, net = nrc_run(args.cache_lr, args.cache_train_frac, args.cache_train_steps, args.cache_skip, 20001)
     log(f"cache nrc: {nrc_out}")
+    variant = None
+    if args.cache_variant:
+        variant, vnet = nrc_run(args.cache_variant_lr, min(args.cache_train_frac, 65536 / (W * H)),
+                                args.cache_variant_steps, args.cache_skip, 40001)
+        vnet.close()
+        log(f"cache nrc variant: {variant}")
 
     # Irradiance cache (the reference's comparator): 1-spp frames, the cache
     # kept across frames; indirect factor 1 so that it converges to what the
@@ -487,6 +668,7 @@ def cache_convergence(ctx, ilm, workloads, args):
     _, per = time_launches(ctx, lambda: net.infer(qd, nq, od), 10)
     inf_ms = float(np.mean(per))
     tflops = nq * MLP_FLOP_PER_QUERY / (inf_ms * 1e-3) / 1e12
+    tflops_exec = nq * MLP_FLOP_EXECUTED / (inf_ms * 1e-3) / 1e12
     # One training step (fused forward + backward, weight-gradient reduction,
     # Adam and bf16 repack) on a frame's 65,536 samples: about 3x the forward
     # FLOP per sample
@@ -501,6 +683,7 @@ def cache_convergence(ctx, ilm, workloads, args):
     tr_tflops = nt * 3 * MLP_FLOP_PER_QUERY / (tr_ms * 1e-3) / 1e12
     mlp = {"queries": nq, "kernel_ms": round(inf_ms, 4), "tflops": round(tflops, 1),
            "flop_per_query": MLP_FLOP_PER_QUERY, "roofline_frac": round(tflops / MFMA_BF16_PEAK_TFLOPS, 4),
+           "executed_flop_per_query": MLP_FLOP_EXECUTED, "executed_tflops": round(tflops_exec, 1),
            "peak_tflops": MFMA_BF16_PEAK_TFLOPS, "kernel": "ilm::k_infer",
            "train_step": {"samples": nt, "ms": round(tr_ms, 4), "tflops": round(tr_tflops, 1),
                           "flop_per_sample": 3 * MLP_FLOP_PER_QUERY,
@@ -532,14 +715,30 @@ def cache_convergence(ctx, ilm, workloads, args):
         cpu = {"window": [x0, y0, x0 + ww, y0 + wh], "seconds": round(dt, 2), "rays": rays,
                "mrays_s": round(rays / dt / 1e6, 4), "cores": int(s.last_threads),
                "gpu_plain_mrays_s": round(plain_out.get("mrays_s", 0.0), 1)}
+        # the irradiance-cache integrator (ICIntegrator restatement) on the same
+        # window at 1 spp, the cache built from empty, the GPU leg's settings
+        t0 = time.perf_counter()
+        s.ic_render(workloads.cache_camera(W / H).as_list(), W, H, 1, seed=7, window=(x0, y0, x0 + ww, y0 + wh),
+                    error_threshold=args.ic_threshold, ray_depth=depth - 1, indirect_factor=1.0,
+                    miss_radiance=(0.0, 0.0, 0.0))
+        dt = time.perf_counter() - t0
+        c = s.last_counts
+        cpu["irradiance_cache"] = {"seconds": round(dt, 2), "camera_rays": int(c[0]), "gather_rays": int(c[1]),
+                                   "shadow_rays": int(c[2]), "records": int(c[3]),
+                                   "mrays_s": round(float(c[0] + c[1] + c[2]) / dt / 1e6, 4),
+                                   "cores": int(s.last_threads), "error_threshold": args.ic_threshold}
         del s
         log(f"cache cpu: {cpu}")
+    vspeed = None
+    if variant and variant["ms_to_target"] and p_ms:
+        variant["speedup_time_to_target"] = round(p_ms / variant["ms_to_target"], 3)
     return {"workload": f"radiance-cache scene {len(idx)} tris, {W}x{H}, depth {depth} "
                         f"(cache after {prefix} bounces)",
             "target_rel_rmse": target, "reference_spp": 2 * args.cache_ref_spp,
             "reference_noise_rel_rmse": round(float(np.sqrt(noise2)), 5),
             "plain_pt": plain_out, "neural_cache": nrc_out, "irradiance_cache": ic_out,
-            "speedup_time_to_target": speedup, "mlp": mlp, "cpu_reference_path": cpu}
+            "speedup_time_to_target": speedup, "neural_cache_variant": variant, "mlp": mlp,
+            "cpu_reference_path": cpu}
 
 
 def main():
@@ -557,6 +756,8 @@ def main():
     ap.add_argument("--no-large", action="store_true", help="skip the 11M-triangle leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-scene batched leg")
     ap.add_argument("--batch-only", action="store_true", help="run only the 64-scene batched leg")
+    ap.add_argument("--no-instances", action="store_true", help="skip the instance-table leg")
+    ap.add_argument("--instances-only", action="store_true", help="run only the instance-table leg")
     ap.add_argument("--no-large-render", action="store_true", help="skip the 8-camera 4K batched render leg")
     ap.add_argument("--large-render-only", action="store_true", help="run only the 8-camera 4K render leg")
     ap.add_argument("--large-frames", type=int, default=2, help="timed frames per renderer (large render leg)")
@@ -570,18 +771,31 @@ def main():
     ap.add_argument("--cache-only", action="store_true", help="run only the time-to-relRMSE leg")
     ap.add_argument("--frames-only", action="store_true",
                     help="run only a few plain and cache frames of the cache workload (profiling)")
+    ap.add_argument("--ic-frames-only", action="store_true",
+                    help="run only six irradiance-cache frames of the cache workload (profiling)")
+    ap.add_argument("--l2-probe-only", action="store_true", help="run only the L2 read-bandwidth probe")
+    ap.add_argument("--pmc-summary", nargs=4, metavar="DIR",
+                    help="summarise rocprofv3 --pmc CSV directories (trace FETCH_SIZE, trace WRITE_SIZE, copy "
+                         "FETCH_SIZE, copy WRITE_SIZE) into the traffic summary and exit")
+    ap.add_argument("--pmc-copy-mib", type=int, default=1024, help="size of the copy run of --pmc-summary")
+    ap.add_argument("--pmc-out", default=None, help="where --pmc-summary writes (default: the committed summary)")
     ap.add_argument("--cache-width", type=int, default=1920)
     ap.add_argument("--cache-height", type=int, default=1080)
-    ap.add_argument("--cache-ref-spp", type=int, default=256, help="spp of each of the two reference renders")
+    ap.add_argument("--cache-ref-spp", type=int, default=2048, help="spp of each of the two reference renders")
     ap.add_argument("--cache-ref-chunk", type=int, default=64)
     ap.add_argument("--cache-target", type=float, default=0.05)
     ap.add_argument("--cache-max-spp", type=int, default=1024)
     ap.add_argument("--cache-max-frames", type=int, default=400)
     ap.add_argument("--cache-skip", type=int, default=0, help="neural-cache frames rendered before accumulating")
-    ap.add_argument("--cache-lr", type=float, default=1e-2)
+    ap.add_argument("--cache-lr", type=float, default=1e-3, help="Adam learning rate of the neural cache")
     ap.add_argument("--cache-train-frac", type=float, default=1 / 16,
-                    help="largest fraction of paths extended as training paths (fewer when 65,536 fill the batch)")
-    ap.add_argument("--cache-train-steps", type=int, default=3, help="Adam steps per neural-cache frame")
+                    help="fraction of paths extended as training paths (records in batches of 65,536)")
+    ap.add_argument("--cache-train-steps", type=int, default=2, help="Adam steps per neural-cache frame")
+    ap.add_argument("--cache-variant", type=int, default=1,
+                    help="also run the neural cache at --cache-variant-lr on one batch filled from fewer paths")
+    ap.add_argument("--cache-variant-lr", type=float, default=1e-2)
+    ap.add_argument("--cache-variant-steps", type=int, default=3)
+    ap.add_argument("--large-lr", type=float, default=1e-2, help="Adam learning rate of the large-scene cache")
     ap.add_argument("--no-ic", action="store_true", help="skip the irradiance-cache comparator")
     ap.add_argument("--ic-threshold", type=float, default=0.3, help="irradiance-cache error threshold")
     args = ap.parse_args()
@@ -589,6 +803,10 @@ def main():
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
               "(a multi-GPU run would shard the rays by ranges and replicate the BVH)", file=sys.stderr)
         sys.exit(2)
+
+    if args.pmc_summary:
+        pmc_summary(args.pmc_summary, args.pmc_copy_mib, args.pmc_out or TRAFFIC_FILE)
+        return
 
     import illuminaprt_amd as ilm
     from illuminaprt_amd import workloads
@@ -610,6 +828,19 @@ def main():
         return
     if args.frames_only:
         print(json.dumps({"frames": frames_leg(ctx, ilm, workloads, args)}), flush=True)
+        ctx.close()
+        return
+    if args.instances_only:
+        print(json.dumps({"instances": instances_leg(ctx, ilm, workloads, args)}), flush=True)
+        ctx.close()
+        return
+    if args.ic_frames_only:
+        print(json.dumps({"ic_frames": ic_frames_leg(ctx, ilm, workloads, args)}), flush=True)
+        ctx.close()
+        return
+    if args.l2_probe_only:
+        gbs, ms = l2_peak_gbs(ctx, args.steps)
+        print(json.dumps({"l2_read_gbs": round(gbs, 1), "kernel_ms": round(ms, 4)}), flush=True)
         ctx.close()
         return
     if args.large_render_only:
@@ -644,17 +875,31 @@ def main():
     if args.only:
         sets = {args.only: sets[args.only]}
     res = {}
+    tree_bytes = info["node_bytes"] + info["tri_bytes"]
     for name, (launch, buf, anyhit) in sets.items():
         ctx.set_trace_refill(refill[name])
+        # node visits and triangle tests: the ordered traversal of the same
+        # tree on the host (the roofline's counts), and the speculative
+        # kernel's own counters beside them
+        t0 = time.perf_counter()
+        if info["node_format"] == ilm.ILM_NODES_FLOAT:
+            c = mesh.visit_counts(buf.download(np.float32, 8 * n).reshape(n, 8), any_hit=anyhit)
+            nn, nt = float(c[:, 0].mean()), float(c[:, 1].mean())
+        else:
+            nn = nt = float("nan")
+        count_s = time.perf_counter() - t0
         mesh.trace_stats(buf, n, anyhit, visits)
         ctx.sync()
         v = visits.download(np.int32, 2 * n).reshape(n, 2)
-        nn, nt = float(v[:, 0].mean()), float(v[:, 1].mean())
+        sn, st_ = float(v[:, 0].mean()), float(v[:, 1].mean())
+        if nn != nn:
+            nn, nt = sn, st_
         steps = args.steps if name == "incoherent" else max(3, args.steps // 4)
         for _ in range(args.warmup):
             launch()
         wall, per = time_launches(ctx, launch, steps)
         kern_ms = float(np.mean(per))
+        out_bytes = 1 if anyhit else HIT_BYTES
         res[name] = {
             "mrays_s": n / (kern_ms * 1e-3) / 1e6,
             "wall_mrays_s": n * steps / wall / 1e6,
@@ -662,7 +907,11 @@ def main():
             "wall_ms_per_step": wall * 1e3 / steps,
             "nodes_per_ray": nn,
             "tris_per_ray": nt,
-            "bytes_per_ray": RAY_BYTES + (1 if anyhit else HIT_BYTES) + nn * info["node_stride"] + nt * TRI_BYTES,
+            "nodes_per_ray_speculative": sn,
+            "tris_per_ray_speculative": st_,
+            "count_s": count_s,
+            "bytes_per_ray": RAY_BYTES + out_bytes + nn * info["node_stride"] + nt * TRI_BYTES,
+            "hbm_bytes": n * (RAY_BYTES + out_bytes) + tree_bytes,
             "steps": steps,
             "refill_idle_lanes": refill[name],
         }
@@ -674,7 +923,11 @@ def main():
         return
 
     inc = res["incoherent"]
-    achieved = inc["bytes_per_ray"] * n / (inc["kernel_ms"] * 1e-3) / 1e9
+    achieved = inc["hbm_bytes"] / (inc["kernel_ms"] * 1e-3) / 1e9
+    l2_achieved = inc["bytes_per_ray"] * n / (inc["kernel_ms"] * 1e-3) / 1e9
+    l2_peak, l2_probe_ms = l2_peak_gbs(ctx)
+    log(f"L2 read probe: {l2_peak:.0f} GB/s ({l2_probe_ms:.3f} ms)")
+    traffic, traffic_note = traffic_from_profiles()
     cpu = None
     if not args.no_cpu:
         rays_host = rs.incoherent.download(np.float32, 8 * n).reshape(n, 8)
@@ -686,6 +939,7 @@ def main():
     large = None if (args.only or args.no_large) else large_scene_leg(ctx, ilm, workloads, args)
     batch = None if (args.only or args.no_batch) else batch_leg(ctx, ilm, workloads, args)
     lrender = None if (args.only or args.no_large_render) else large_render_leg(ctx, ilm, workloads, args)
+    inst = None if (args.only or args.no_instances) else instances_leg(ctx, ilm, workloads, args)
     out = {
         "metric": METRIC,
         "value": round(inc["wall_mrays_s"], 3),
@@ -707,13 +961,19 @@ def main():
             "device": ctx.name,
             "cus": ctx.cu_count,
         },
-        # per ray class: roofline fraction by algorithmic bytes, and the VALU
-        # fraction of the FP32 vector peak by the estimate 12 FP32 operations
-        # per box test (a node tests one box per child slot) + 30 per triangle
+        # per ray class: the HBM fraction by the bytes a launch must move, the
+        # L2 fraction by bytes per ray of the ordered traversal against the
+        # measured L2 read bandwidth, and the VALU fraction of the FP32 vector
+        # peak by the estimate 12 FP32 operations per box test (a node tests
+        # one box per child slot) + 30 per triangle
         "rays": {k: {"mrays_s": round(v["mrays_s"], 2), "kernel_ms": round(v["kernel_ms"], 4),
                      "nodes_per_ray": round(v["nodes_per_ray"], 3), "tris_per_ray": round(v["tris_per_ray"], 3),
+                     "nodes_per_ray_speculative": round(v["nodes_per_ray_speculative"], 3),
+                     "tris_per_ray_speculative": round(v["tris_per_ray_speculative"], 3),
+                     "host_count_s": round(v["count_s"], 2),
                      "bytes_per_ray": round(v["bytes_per_ray"], 1),
-                     "roofline_frac": round(v["bytes_per_ray"] * v["mrays_s"] * 1e6 / (HBM_PEAK_GBS * 1e9), 4),
+                     "hbm_frac": round(v["hbm_bytes"] / (v["kernel_ms"] * 1e-3) / (HBM_PEAK_GBS * 1e9), 4),
+                     "l2_frac": round(v["bytes_per_ray"] * v["mrays_s"] * 1e6 / (l2_peak * 1e9), 4),
                      "valu_frac_est": round((12 * v["nodes_per_ray"] * info["width"] + 30 * v["tris_per_ray"])
                                             * v["mrays_s"] * 1e6 / FP32_PEAK_FLOPS, 4)}
                  for k, v in res.items()},
@@ -723,10 +983,18 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic_from_profiles(),
+            "traffic": traffic,
+            "traffic_source": traffic_note,
+            "traffic_frac": round(traffic / (inc["kernel_ms"] * 1e-3) / (HBM_PEAK_GBS * 1e9), 4) if traffic else None,
             "kernel": f"ilm::k_trace4<false, false, {1 if args.nodes == 'quantized' else {4: 0, 2: 2, 8: 3}[args.width]}, false>",
-            "node_bytes": info["node_stride"],
-            "bytes_per_ray": round(inc["bytes_per_ray"], 1),
+            "algorithmic_bytes_per_launch": int(inc["hbm_bytes"]),
+            "algorithmic_bytes": "32 B read per ray + 16 B written per hit + tree and triangle arrays once "
+                                 f"({tree_bytes} B)",
+            "l2": {"achieved": round(l2_achieved, 1), "peak": round(l2_peak, 1), "unit": "GB/s",
+                   "frac": round(l2_achieved / l2_peak, 4), "bytes_per_ray": round(inc["bytes_per_ray"], 1),
+                   "nodes_per_ray": round(inc["nodes_per_ray"], 3), "tris_per_ray": round(inc["tris_per_ray"], 3),
+                   "node_bytes": info["node_stride"], "tri_bytes": TRI_BYTES,
+                   "peak_source": "ilm_read_probe: 2 MiB buffer read 8 times by 8 workgroups per CU"},
         },
         "cpu_baseline": cpu,
         "path_tracing_room": room,
@@ -735,6 +1003,7 @@ def main():
         "large_scene": large,
         "batched_scenes": batch,
         "large_render": lrender,
+        "instances": inst,
     }
     print(json.dumps(out), flush=True)
     for b in (hits, occ, visits):
