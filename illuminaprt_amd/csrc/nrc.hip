@@ -354,7 +354,13 @@ __global__ void __launch_bounds__(256) k_train_fb(TrainArgs a) {
 }
 
 // ------------------------------------------------------------------ weight gradients
-// 92 output tiles of 16x16: layer 0 (4 x 6), layers 1..4 (4 x 4 each), layer 5 (1 x 4).
+// 92 output tiles of 16x16 in 21 strips of tiles sharing their output rows:
+// layer 0 (4 strips of 6 tiles), layers 1..4 (4 strips of 4 tiles each),
+// layer 5 (1 strip of 4 tiles). A wave computes one strip over its
+// workgroup's chunk of samples, so the strip's G rows are read once for all
+// of its tiles.
+constexpr int kStrips = 21;
+constexpr int kStripWaves = 24;  // six workgroups of four waves per chunk
 struct DwArgs {
     const __bf16* H;
     const __bf16* G;
@@ -364,46 +370,58 @@ struct DwArgs {
     float* partial;    // [blocks][kParams]
 };
 
-__device__ __forceinline__ void tile_of(int tile, int& layer, int& to, int& ti) {
-    if (tile < 24) { layer = 0; to = tile / 6; ti = tile % 6; return; }
-    if (tile < 88) { int q = tile - 24; layer = 1 + q / 16; q %= 16; to = q / 4; ti = q % 4; return; }
-    layer = 5; to = 0; ti = tile - 88;
+__device__ __forceinline__ void strip_of(int strip, int& layer, int& to, int& ntiles) {
+    if (strip < 4) { layer = 0; to = strip; ntiles = 6; return; }
+    if (strip < 20) { layer = 1 + (strip - 4) / 4; to = (strip - 4) % 4; ntiles = 4; return; }
+    layer = 5; to = 0; ntiles = 4;
 }
 
 __global__ void __launch_bounds__(256) k_dw(DwArgs a) {
     const int lane = (int)__lane_id(), g = lane >> 4, m = lane & 15;
     const int wave = threadIdx.x >> 6;
+    const int strip = (int)blockIdx.y * 4 + wave;
+    if (strip >= kStrips) return;
     const int64_t n = a.n, ld = a.ld;
     const int64_t s0 = (int64_t)blockIdx.x * a.chunk, s1 = min(n, s0 + a.chunk);
     float* part = a.partial + (int64_t)blockIdx.x * kParams;
-    // the 92 output tiles are shared by the gridDim.y workgroups of a chunk
-    for (int tile = (int)blockIdx.y * 4 + wave; tile < 92; tile += 4 * (int)gridDim.y) {
-        int layer, to, ti;
-        tile_of(tile, layer, to, ti);
-        // A rows: G_{layer+1} output features; B rows: H_layer input features
-        const __bf16* gA = a.G + (int64_t)layer * kHid * ld + (int64_t)(16 * to + m) * ld;
-        const __bf16* hB = a.H + (layer == 0 ? 0 : (int64_t)kIn * ld + (int64_t)(layer - 1) * kHid * ld) + (int64_t)(16 * ti + m) * ld;
-        f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
-        for (int64_t s = s0; s < s1; s += 32) {
-            bf16x8 A, B;
-            int64_t k0 = s + 8 * g;
-            if (k0 + 8 <= s1) {
-                A = *reinterpret_cast<const bf16x8*>(gA + k0);
-                B = *reinterpret_cast<const bf16x8*>(hB + k0);
-            } else {
-                for (int e = 0; e < 8; ++e) {
-                    bool ok = k0 + e < s1;
-                    A[e] = ok ? gA[k0 + e] : (__bf16)0.f;
-                    B[e] = ok ? hB[k0 + e] : (__bf16)0.f;
-                }
-            }
-            acc = mfma32(A, B, acc);
+    int layer, to, ntiles;
+    strip_of(strip, layer, to, ntiles);
+    // A rows: G_{layer+1} output features; B rows: H_layer input features
+    const __bf16* gA = a.G + (int64_t)layer * kHid * ld + (int64_t)(16 * to + m) * ld;
+    const __bf16* hB = a.H + (layer == 0 ? 0 : (int64_t)kIn * ld + (int64_t)(layer - 1) * kHid * ld) + (int64_t)m * ld;
+    f32x4 acc[6];
+#pragma unroll
+    for (int t = 0; t < 6; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int64_t s = s0; s < s1; s += 32) {
+        const int64_t k0 = s + 8 * g;
+        const bool full = k0 + 8 <= s1;
+        bf16x8 A;
+        if (full) {
+            A = *reinterpret_cast<const bf16x8*>(gA + k0);
+        } else {
+            for (int e = 0; e < 8; ++e) A[e] = k0 + e < s1 ? gA[k0 + e] : (__bf16)0.f;
         }
-        const int in_dim = layer == 0 ? kIn : kHid;
-        const int base = layer == 0 ? kW0 : (layer <= 4 ? kWh + (layer - 1) * kHid * kHid : kW5);
+#pragma unroll
+        for (int t = 0; t < 6; ++t) {
+            if (t >= ntiles) break;
+            const __bf16* hp = hB + (int64_t)(16 * t) * ld;
+            bf16x8 B;
+            if (full) {
+                B = *reinterpret_cast<const bf16x8*>(hp + k0);
+            } else {
+                for (int e = 0; e < 8; ++e) B[e] = k0 + e < s1 ? hp[k0 + e] : (__bf16)0.f;
+            }
+            acc[t] = mfma32(A, B, acc[t]);
+        }
+    }
+    const int in_dim = layer == 0 ? kIn : kHid;
+    const int base = layer == 0 ? kW0 : (layer <= 4 ? kWh + (layer - 1) * kHid * kHid : kW5);
+#pragma unroll
+    for (int t = 0; t < 6; ++t) {
+        if (t >= ntiles) break;
         for (int r = 0; r < 4; ++r) {
-            int o = 16 * to + 4 * g + r, i = 16 * ti + m;
-            part[base + o * in_dim + i] = acc[r];
+            const int o = 16 * to + 4 * g + r, i = 16 * t + m;
+            part[base + o * in_dim + i] = acc[t][r];
         }
     }
 }
@@ -422,13 +440,22 @@ struct AdamArgs {
     int do_update;
 };
 
-__global__ void k_adam(AdamArgs a) {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= kParams) return;
+// 64 parameters per workgroup: its four waves sum every fourth sample
+// chunk's partial gradients (a fixed order, so the sum is deterministic),
+// the first wave adds the four sums and updates.
+__global__ void __launch_bounds__(256) k_adam(AdamArgs a) {
+    __shared__ float quarter[4][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int p = blockIdx.x * 64 + lane;
+    float part = 0.f;
+    if (a.do_update && p < kParams)
+        for (int b = wave; b < a.nblocks; b += 4) part += a.partial[(int64_t)b * kParams + p];
+    quarter[wave][lane] = part;
+    __syncthreads();
+    if (wave != 0 || p >= kParams) return;
     float wv = a.w[p];
     if (a.do_update) {
-        float gsum = 0.f;
-        for (int b = 0; b < a.nblocks; ++b) gsum += a.partial[(int64_t)b * kParams + p];
+        const float gsum = (quarter[0][lane] + quarter[1][lane]) + (quarter[2][lane] + quarter[3][lane]);
         a.grad[p] = gsum;
         float m1 = a.b1 * a.mom1[p] + (1.f - a.b1) * gsum;
         float m2 = a.b2 * a.mom2[p] + (1.f - a.b2) * gsum * gsum;
@@ -530,7 +557,7 @@ int nrc_repack(ilm_nrc* r, hipStream_t st, int update, float c1, float c2, int n
     a.c1 = c1;
     a.c2 = c2;
     a.do_update = update;
-    hipLaunchKernelGGL(k_adam, dim3((kParams + 255) / 256), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_adam, dim3((kParams + 63) / 64), dim3(256), 0, st, a);
     return hipGetLastError() == hipSuccess ? ILM_OK : ilm::fail(ILM_ERR_HIP, "adam launch");
 }
 
@@ -708,8 +735,8 @@ int ilm_nrc_train_step(ilm_nrc* r, const float* queries, const float* targets, i
     d.ld = (n + 31) / 32 * 32;
     d.chunk = chunk;
     d.partial = r->partial;
-    // four workgroups per chunk of samples, each on a quarter of the tiles
-    hipLaunchKernelGGL(k_dw, dim3((unsigned)dw_blocks, 4), dim3(256), 0, st, d);
+    // six workgroups per chunk of samples, one wave per strip of output tiles
+    hipLaunchKernelGGL(k_dw, dim3((unsigned)dw_blocks, kStripWaves / 4), dim3(256), 0, st, d);
     ILM_HIP(hipGetLastError());
     r->step += 1;
     float c1 = 1.f - powf(r->b1, (float)r->step), c2 = 1.f - powf(r->b2, (float)r->step);
