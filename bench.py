@@ -765,6 +765,8 @@ def main():
     ap.add_argument("--large-train-steps", type=int, default=8, help="Adam steps per cache frame (large render)")
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--batch-spp", type=int, default=2)
+    ap.add_argument("--refill", type=int, default=0,
+                    help="idle lanes that trigger a traversal wave's refill, for every ray set (0: 24, 56 for camera rays)")
     ap.add_argument("--nodes", choices=["float", "quantized"], default="float",
                     help="BVH4 node format of the traced trees")
     ap.add_argument("--width", type=int, choices=[2, 4, 8], default=4, help="BVH width (float nodes)")
@@ -790,7 +792,7 @@ def main():
     ap.add_argument("--cache-lr", type=float, default=1e-3, help="Adam learning rate of the neural cache")
     ap.add_argument("--cache-train-frac", type=float, default=1 / 16,
                     help="fraction of paths extended as training paths (records in batches of 65,536)")
-    ap.add_argument("--cache-train-steps", type=int, default=2, help="Adam steps per neural-cache frame")
+    ap.add_argument("--cache-train-steps", type=int, default=12, help="Adam steps per neural-cache frame")
     ap.add_argument("--cache-variant", type=int, default=1,
                     help="also run the neural cache at --cache-variant-lr on one batch filled from fewer paths")
     ap.add_argument("--cache-variant-lr", type=float, default=1e-2)
@@ -867,6 +869,8 @@ def main():
     # incoherent and shadow sets, 56 for coherent camera rays, as the path
     # integrator traces them
     refill = {"incoherent": 24, "primary": 56, "shadow": 24}
+    if args.refill:
+        refill = {k: args.refill for k in refill}
     sets = {
         "incoherent": (lambda: mesh.trace_closest(rs.incoherent, n, hits), rs.incoherent, False),
         "primary": (lambda: mesh.trace_closest(rs.primary, n, hits), rs.primary, False),
