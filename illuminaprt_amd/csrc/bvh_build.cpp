@@ -456,7 +456,8 @@ struct Collapser {
     size_t node_bytes;
 
     Collapser(const Bvh2& bb, int w, std::vector<uint8_t>& o) : b(bb), W(w), out(o) {
-        node_bytes = ((size_t)W * 28 + 15) / 16 * 16;
+        // the stride the kernels index by: the node struct's own size
+        node_bytes = W == 2 ? sizeof(WideNode<2>) : W == 4 ? sizeof(WideNode<4>) : sizeof(WideNode<8>);
     }
 
     static float area(const float* lo, const float* hi) {
