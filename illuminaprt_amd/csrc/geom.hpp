@@ -166,7 +166,7 @@ struct RayState {
     f3 o, d;
     f3 inv;   // 1/d with zero components replaced by +-1e-18
     f3 oinv;  // o * inv
-    int kx, ky, kz;
+    int kp;   // axis permutation of the watertight test: kx | ky << 2 | kz << 4 (one register)
     float Sx, Sy, Sz;
     float tlo, thi;  // accepted t window (thi shrinks with closest hits)
 };
@@ -188,7 +188,7 @@ ILM_HD void ray_setup(RayState& r, f3 o, f3 d, float tmax) {
     int ky = kx + 1 == 3 ? 0 : kx + 1;
     float dkz = comp(d, kz);
     if (dkz < 0.0f) { int t = kx; kx = ky; ky = t; }
-    r.kx = kx; r.ky = ky; r.kz = kz;
+    r.kp = kx | ky << 2 | kz << 4;
     r.Sx = comp(d, kx) / dkz;
     r.Sy = comp(d, ky) / dkz;
     r.Sz = 1.0f / dkz;
@@ -234,6 +234,44 @@ ILM_HD void sort_children(float (&d)[W], uint32_t (&c)[W]) {
     }
 }
 
+// Child keys of the ordered traversal: a child's entry distance as bits
+// (non-negative floats order as unsigned integers) with its slot in the low
+// bits, as slot * 4 (the byte offset of its child word); the distance loses
+// its lowest mantissa bits (2^-19 relative), which only reorders children
+// whose entry distances nearly tie, and equal keys cannot occur. Sorting the
+// keys as integers orders the children nearest first, ties by slot; a missed
+// child (distance +inf) has a key of at least kMissKey and sorts last.
+constexpr uint32_t kMissKey = 0x7F800000u;
+template <int W> constexpr uint32_t kSlotMask = W == 8 ? 31u : W == 4 ? 15u : 7u;
+template <int W>
+ILM_HD uint32_t child_key(float d, int slot) {
+    return (float_bits(d) & ~kSlotMask<W>) | (uint32_t)(slot << 2);
+}
+template <int W>
+ILM_HD int key_slot(uint32_t key) { return (int)((key >> 2) & (uint32_t)(W - 1)); }
+
+ILM_HD void kswap(uint32_t& a, uint32_t& b) {
+    const uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+    a = lo;
+    b = hi;
+}
+
+// Sorts child keys ascending with the exchange networks of sort_children
+// (branch-free: a minimum and a maximum per exchange).
+template <int W>
+ILM_HD void sort_keys(uint32_t (&k)[W]) {
+    if (W == 2) {
+        kswap(k[0], k[1]);
+    } else if (W == 4) {
+        kswap(k[0], k[1]); kswap(k[2], k[3]); kswap(k[0], k[2]); kswap(k[1], k[3]); kswap(k[1], k[2]);
+    } else {
+        constexpr int net[19][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {0, 2}, {1, 3}, {4, 6}, {5, 7}, {1, 2}, {5, 6},
+                                    {0, 4}, {3, 7}, {1, 5}, {2, 6}, {1, 4}, {3, 6}, {2, 4}, {3, 5}, {3, 4}};
+#pragma unroll
+        for (int e = 0; e < 19; ++e) kswap(k[net[e][0]], k[net[e][1]]);
+    }
+}
+
 // ------------------------------------------------------------------ triangle test
 // Watertight ray/triangle test (Woop, Benthin, Wald 2013). The edge functions
 // are evaluated without fused multiply-add so that the two triangles sharing
@@ -256,10 +294,11 @@ ILM_HD float sub_rn(float a, float b) {
 
 ILM_HD bool tri_watertight(const RayState& r, f3 v0, f3 v1, f3 v2, float& t_out, float& u_out, float& v_out) {
     f3 A = v0 - r.o, B = v1 - r.o, C = v2 - r.o;
-    float Akz = comp(A, r.kz), Bkz = comp(B, r.kz), Ckz = comp(C, r.kz);
-    float Ax = fmaf(-r.Sx, Akz, comp(A, r.kx)), Ay = fmaf(-r.Sy, Akz, comp(A, r.ky));
-    float Bx = fmaf(-r.Sx, Bkz, comp(B, r.kx)), By = fmaf(-r.Sy, Bkz, comp(B, r.ky));
-    float Cx = fmaf(-r.Sx, Ckz, comp(C, r.kx)), Cy = fmaf(-r.Sy, Ckz, comp(C, r.ky));
+    const int kx = r.kp & 3, ky = (r.kp >> 2) & 3, kz = r.kp >> 4;
+    float Akz = comp(A, kz), Bkz = comp(B, kz), Ckz = comp(C, kz);
+    float Ax = fmaf(-r.Sx, Akz, comp(A, kx)), Ay = fmaf(-r.Sy, Akz, comp(A, ky));
+    float Bx = fmaf(-r.Sx, Bkz, comp(B, kx)), By = fmaf(-r.Sy, Bkz, comp(B, ky));
+    float Cx = fmaf(-r.Sx, Ckz, comp(C, kx)), Cy = fmaf(-r.Sy, Ckz, comp(C, ky));
     float U = sub_rn(mul_rn(Cx, By), mul_rn(Cy, Bx));
     float V = sub_rn(mul_rn(Ax, Cy), mul_rn(Ay, Cx));
     float Wf = sub_rn(mul_rn(Bx, Ay), mul_rn(By, Ax));

@@ -42,19 +42,16 @@ void count_rays(const WideNode<W>* nodes, const TriSoA& tris, const ilm_ray* ray
             if (!is_leaf(cur)) {
                 ++nn;
                 const WideNode<W>& n = nodes[cur];
-                float d[W];
-                uint32_t c[W];
-                for (int k = 0; k < W; ++k) {
-                    d[k] = n.child[k] == kEmptySlot ? INFINITY : slab<W>(n, k, r);
-                    c[k] = n.child[k];
-                }
-                // nearest first, by the kernels' own exchange network (so
-                // that ties in entry distance order the children alike)
-                sort_children<W>(d, c);
+                uint32_t key[W];
+                for (int k = 0; k < W; ++k)
+                    key[k] = child_key<W>(n.child[k] == kEmptySlot ? INFINITY : slab<W>(n, k, r), k);
+                // nearest first, by the kernels' own keys and exchange network
+                // (so that near-ties in entry distance order the children alike)
+                sort_keys<W>(key);
                 for (int k = W - 1; k >= 1; --k)
-                    if (d[k] < INFINITY) stack.push_back(c[k]);
-                if (d[0] < INFINITY) {
-                    cur = c[0];
+                    if (key[k] < kMissKey) stack.push_back(n.child[key_slot<W>(key[k])]);
+                if (key[0] < kMissKey) {
+                    cur = n.child[key_slot<W>(key[0])];
                     continue;
                 }
             } else {

@@ -13,15 +13,21 @@
 // waves of the grid take part, so short launches stay cheap. A wave refills once at
 // least refill_min of its lanes are idle (24 by default; 56 for coherent
 // camera rays, whose lanes finish together). Per lane: a 16-entry
-// traversal stack in LDS (lane-strided, conflict-free) that spills its
-// oldest entries to a global area only when deeper than 16. Nodes are wide
-// (BVH2/4/8 with float boxes, 64/112/224 bytes, or 64-byte quantised BVH4),
-// loaded as 16-byte vectors; children are sorted by entry distance with a
-// comparator network and the nearest is visited first. Traversal is
-// speculative (Aila and Laine, "Understanding the Efficiency of Ray Traversal
-// on GPUs", HPG 2009): a lane that reaches a leaf parks it and keeps
-// descending until most lanes of the wave hold a leaf, so that the wave
-// tests triangles together instead of idling lanes at each leaf.
+// traversal stack ring in LDS (lane-strided, conflict-free) that spills its
+// oldest entries to a global area only when a node's pushes could overrun
+// it. Nodes are wide (BVH2/4/8 with float boxes, 64/112/224 bytes, or
+// 64-byte quantised BVH4), loaded as 16-byte vectors. The vector ALU and the
+// vector-memory path (address unit, L1) both charge a wave instruction in
+// full however few of its lanes are active, so the kernel keeps instructions
+// per ray low and lanes busy. A node step avoids branches: the children's
+// keys (entry distance bits with the slot in the low bits) are sorted with a
+// min/max comparator network, the child words are read back from LDS by slot,
+// and every push is written at the top and kept only when its child was hit;
+// the nearest child is visited first. Traversal is speculative (Aila and
+// Laine, "Understanding the Efficiency of Ray Traversal on GPUs", HPG 2009):
+// a lane that reaches a leaf parks it and keeps descending until half of the
+// lanes in the node loop hold a leaf; the leaf loop then tests one triangle
+// per lane per iteration, each lane moving on to its next leaf on its own.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -139,6 +145,33 @@ __device__ __forceinline__ void node_hits(const float4* __restrict__ nodes, uint
     }
 }
 
+// Child keys of a node (child_key: entry distance bits with the slot in the
+// low bits), and its child words to the lane's W staging words in LDS, from
+// which the traversal reads them back by slot once the keys are sorted.
+template <int K>
+__device__ __forceinline__ void node_keys(const float4* __restrict__ nodes, uint32_t cur, const RayState& r,
+                                          uint32_t (&key)[Kind<K>::W], uint32_t* __restrict__ stage) {
+    constexpr int W = Kind<K>::W;
+    float d[W];
+    uint32_t c[W];
+    node_hits<K>(nodes, cur, r, d, c);
+#pragma unroll
+    for (int k = 0; k < W; ++k) key[k] = child_key<W>(d[k], k);
+    if (W == 2) {
+        *reinterpret_cast<uint2*>(stage) = make_uint2(c[0], c[1]);
+    } else {
+#pragma unroll
+        for (int q = 0; q < W / 4; ++q)
+            reinterpret_cast<uint4*>(stage)[q] = make_uint4(c[4 * q], c[4 * q + 1], c[4 * q + 2], c[4 * q + 3]);
+    }
+}
+
+// Entries a lane keeps in its LDS ring before a node's W - 1 pushes: with at
+// most this many live entries the pushes (each written at the top, kept when
+// the child was hit) never overwrite a live entry of the 16-entry ring.
+template <int W> constexpr int kKeep = kLds - (W - 1);
+static_assert((kLds & (kLds - 1)) == 0, "the LDS ring is indexed by position & (kLds - 1)");
+
 // Waves per SIMD the traversal kernels are compiled for: seven (72 VGPRs)
 // measured 4% faster than the six that the unconstrained register count
 // allows; BVH8 nodes need ~120 VGPRs and keep the four those allow.
@@ -147,8 +180,13 @@ template <int K> constexpr int kTraceWaves = K == kFloat8 ? 4 : 7;
 template <bool ANY, bool STATS, int KIND, bool DUAL>
 __global__ void __launch_bounds__(kBlock, kTraceWaves<KIND>) k_trace4(TraceArgs a) {
     constexpr int W = Kind<KIND>::W;
-    __shared__ uint32_t lds[kLds * kBlock];
+    // per lane: a kLds-entry stack ring (lane-strided, conflict-free) and W
+    // staging words for the current node's child words
+    __shared__ __align__(16) uint32_t lds[(kLds + W) * kBlock];
     const int tid = threadIdx.x;
+    uint32_t* const ring = lds + tid;
+    uint32_t* const stage = lds + kLds * kBlock + tid * W;
+    const uint32_t stage_off = (uint32_t)((kLds * kBlock + tid * W) * sizeof(uint32_t));
     const int64_t gtid = (int64_t)blockIdx.x * kBlock + tid;
     const int64_t n1 = a.n_dev ? (int64_t)min((unsigned long long)a.n, *a.n_dev) : a.n;
     const int64_t n = DUAL ? n1 + (a.n2_dev ? (int64_t)min((unsigned long long)a.n2, *a.n2_dev) : a.n2) : n1;
@@ -165,52 +203,43 @@ __global__ void __launch_bounds__(kBlock, kTraceWaves<KIND>) k_trace4(TraceArgs 
     // the segment the wave fetches from (it starts on its XCD's) and the
     // number of segments it has found spent; uniform across the wave
     unsigned int seg = blockIdx.x % kTraceSegs, spent = 0;
-    int64_t ray_id = 0;
+    uint32_t ray_id = 0;  // launches hold fewer than 2^32 rays (checked on the host)
     bool any_ray = ANY;  // this lane's ray wants any hit (dual launches: the second queue's rays)
     RayState r;
     uint32_t cur = kNoNode;
     uint32_t stash = kNoNode;  // a leaf parked during speculative descent
-    int sp = 0, lcnt = 0;
+    uint32_t tk = 0;           // next triangle of the parked leaf
+    // stack: entries at positions [0, sp); those at [base, sp) are in the
+    // LDS ring (position p in slot p & (kLds - 1)), the older ones in the
+    // global spill area (position p at spill[p * grid_threads + gtid])
+    int sp = 0, base = 0;
     float best_t = 0.f, best_u = 0.f, best_v = 0.f;
     int32_t best_prim = -1;
     int32_t n_nodes = 0, n_tris = 0;
 
-    auto push = [&](uint32_t x, float) {
-        if (lcnt == kLds) {
-            int pos = sp - kLds;
-            if (pos < a.spill_depth) {
-                a.spill[(int64_t)pos * a.grid_threads + gtid] = lds[(sp % kLds) * kBlock + tid];
-            } else {
-                atomicOr(&a.counters[2], 1ull);  // stack bound exceeded (never expected)
-                return;
-            }
-        } else {
-            ++lcnt;
-        }
-        lds[(sp % kLds) * kBlock + tid] = x;
-        ++sp;
-    };
     auto pop = [&]() -> uint32_t {
+        if (sp > base) {
+            --sp;
+            return ring[(sp & (kLds - 1)) * kBlock];
+        }
         if (sp == 0) return kNoNode;
         --sp;
-        if (lcnt > 0) {
-            --lcnt;
-            return lds[(sp % kLds) * kBlock + tid];
-        }
-        return a.spill[(int64_t)sp * a.grid_threads + gtid];
+        base = sp;
+        // a position beyond the spill area was dropped (flagged when it was)
+        return sp < a.spill_depth ? a.spill[(int64_t)sp * a.grid_threads + gtid] : kNoNode;
     };
     auto finish = [&]() {
         if (ANY) {
             a.occ[ray_id] = best_prim >= 0 ? 1 : 0;
         } else if (DUAL && any_ray) {
-            a.occ2[ray_id - n1] = best_prim >= 0 ? 1 : 0;
+            a.occ2[ray_id - (uint32_t)n1] = best_prim >= 0 ? 1 : 0;
         } else {
             float4 h = make_float4(best_t, __int_as_float(best_prim), best_u, best_v);
             reinterpret_cast<float4*>(a.hits)[ray_id] = h;
         }
         if (STATS) {
-            a.stats[2 * ray_id] = n_nodes;
-            a.stats[2 * ray_id + 1] = n_tris;
+            a.stats[2 * (int64_t)ray_id] = n_nodes;
+            a.stats[2 * (int64_t)ray_id + 1] = n_tris;
         }
         active = false;
     };
@@ -243,15 +272,16 @@ __global__ void __launch_bounds__(kBlock, kTraceWaves<KIND>) k_trace4(TraceArgs 
             if (!active) {
                 const int64_t my = (int64_t)mine;
                 if (mine < (unsigned long long)n) {
-                    ray_id = my;
+                    ray_id = (uint32_t)my;
                     any_ray = ANY || (DUAL && my >= n1);
                     const float4* rp = reinterpret_cast<const float4*>(DUAL && my >= n1 ? a.rays2 + (my - n1) : a.rays + my);
                     float4 o4 = rp[0], d4 = rp[1];
                     ray_setup(r, mk(o4.x, o4.y, o4.z), mk(d4.x, d4.y, d4.z), d4.w);
                     cur = a.rooted ? __float_as_uint(o4.w) : 0u;
                     sp = 0;
-                    lcnt = 0;
+                    base = 0;
                     stash = kNoNode;
+                    tk = 0;
                     best_prim = -1;
                     best_t = best_u = best_v = 0.f;
                     n_nodes = n_tris = 0;
@@ -262,48 +292,71 @@ __global__ void __launch_bounds__(kBlock, kTraceWaves<KIND>) k_trace4(TraceArgs 
         if (!__any(active)) break;
 
         // inner nodes; a lane that meets a leaf parks it and keeps descending
-        // until three quarters of the lanes in the loop hold a parked leaf
-        // (waiting for all of them measured 2% slower on incoherent rays)
+        // until half of the lanes in the loop hold a parked leaf (three
+        // quarters measured 5% slower on incoherent rays, all of them slower
+        // still: the node loop's lanes idle longer than the leaf loop saves)
         while (active && cur != kNoNode && !is_leaf(cur)) {
             if (STATS) ++n_nodes;
-            float d[W];
-            uint32_t c[W];
-            node_hits<KIND>(a.nodes, cur, r, d, c);
-            sort_children<W>(d, c);
+            uint32_t key[W];
+            node_keys<KIND>(a.nodes, cur, r, key, stage);
+            sort_keys<W>(key);
+            // make room for W - 1 pushes: the oldest ring entries go to the
+            // spill area (rare: stacks deeper than kKeep)
+            while (sp - base > kKeep<W>) {
+                if (base < a.spill_depth)
+                    a.spill[(int64_t)base * a.grid_threads + gtid] = ring[(base & (kLds - 1)) * kBlock];
+                else
+                    atomicOr(&a.counters[2], 1ull);  // stack bound exceeded (never expected)
+                ++base;
+            }
+            uint32_t cw[W];
 #pragma unroll
-            for (int k = W - 1; k >= 1; --k)
-                if (d[k] < INFINITY) push(c[k], d[k]);
-            cur = d[0] < INFINITY ? c[0] : pop();
+            for (int k = 0; k < W; ++k)  // stage byte offset | slot * 4 (the offset's low bits are clear)
+                cw[k] = *reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(lds) +
+                                                           (stage_off | (key[k] & ((W - 1) << 2))));
+            // farthest first; a missed child (sorted last) is written at the
+            // top and left there for the next write to overwrite
+#pragma unroll
+            for (int k = W - 1; k >= 1; --k) {
+                ring[(sp & (kLds - 1)) * kBlock] = cw[k];
+                sp += key[k] < kMissKey ? 1 : 0;
+            }
+            cur = key[0] < kMissKey ? cw[0] : pop();
             if (cur != kNoNode && is_leaf(cur) && stash == kNoNode) {
                 stash = cur;
                 cur = pop();
             }
-            if (__popcll(__ballot(stash != kNoNode)) * 4 >= __popcll(__ballot(true)) * 3) break;
+            if (__popcll(__ballot(stash != kNoNode)) * 2 >= __popcll(__ballot(true))) break;
         }
-        // parked leaves, then any leaves that follow on the stack
+        // parked leaves, one triangle per lane per iteration: a lane moves on
+        // to the next leaf on its stack as soon as its current one is done, so
+        // lanes with short leaves do not wait for the wave's longest one
         while (active && stash != kNoNode) {
-            uint32_t first = leaf_first(stash), cnt = leaf_count(stash);
-            bool done = false;
-            for (uint32_t k = 0; k < cnt; ++k) {
-                const float4* tp = a.tris + 3 * (int64_t)(first + k);
-                float4 A = tp[0], B = tp[1], C = tp[2];
-                if (STATS) ++n_tris;
-                float t, u, v;
-                if (tri_watertight(r, mk(A.x, A.y, A.z), mk(B.x, B.y, B.z), mk(C.x, C.y, C.z), t, u, v)) {
-                    best_t = t;
-                    best_u = u;
-                    best_v = v;
-                    best_prim = __float_as_int(A.w);
-                    r.thi = tmax_window(t);
-                    if (ANY || (DUAL && any_ray)) { done = true; break; }
+            const float4* tp = a.tris + 3 * (int64_t)(leaf_first(stash) + tk);
+            float4 A = tp[0], B = tp[1], C = tp[2];
+            if (STATS) ++n_tris;
+            float t, u, v;
+            if (tri_watertight(r, mk(A.x, A.y, A.z), mk(B.x, B.y, B.z), mk(C.x, C.y, C.z), t, u, v)) {
+                best_t = t;
+                best_u = u;
+                best_v = v;
+                best_prim = __float_as_int(A.w);
+                r.thi = tmax_window(t);
+                if (ANY || (DUAL && any_ray)) {
+                    stash = kNoNode;
+                    tk = 0;
+                    finish();
+                    break;
                 }
             }
-            if (done) { stash = kNoNode; finish(); break; }
-            if (cur != kNoNode && is_leaf(cur)) {
-                stash = cur;
-                cur = pop();
-            } else {
-                stash = kNoNode;
+            if (++tk >= leaf_count(stash)) {
+                tk = 0;
+                if (cur != kNoNode && is_leaf(cur)) {
+                    stash = cur;
+                    cur = pop();
+                } else {
+                    stash = kNoNode;
+                }
             }
         }
         if (active && cur == kNoNode && stash == kNoNode) finish();
@@ -378,6 +431,7 @@ static int launch_trace(ilm_mesh* m, const ilm_ray* rays, int64_t n, ilm_hit* hi
         return fail(ILM_ERR_INVALID, "bad second queue");
     const int64_t total = n + (DUAL ? q2->capacity : 0);
     if (total == 0) return ILM_OK;
+    if (total > (int64_t)UINT32_MAX) return fail(ILM_ERR_INVALID, "a launch traces fewer than 2^32 rays");
     if (!ANY && !hits) return fail(ILM_ERR_INVALID, "null hits");
     if (ANY && !occ) return fail(ILM_ERR_INVALID, "null occlusion output");
     if (STATS && !stats) return fail(ILM_ERR_INVALID, "null stats");
@@ -410,11 +464,13 @@ static int launch_trace(ilm_mesh* m, const ilm_ray* rays, int64_t n, ilm_hit* hi
     a.rooted = rooted ? 1 : 0;
     a.refill_min = refill > 0 ? refill : ctx->trace_refill;
     if (fixed_scratch) {
-        // the spill area holds the entries beyond the kLds in LDS: a stack of
-        // at most stack_need entries spills at most stack_need - kLds of them
+        // the spill area holds the positions below the ring's: a stack of at
+        // most stack_need entries keeps at least kKeep<W> of them in LDS, so
+        // it spills at most its stack_need - kKeep<W> oldest
         void* base = ctx->device < kMaxDevices ? g_scratch[ctx->device] : nullptr;
         if (!base) return fail(ILM_ERR_STATE, "fixed scratch not set up on this device");
-        a.spill_depth = std::max(1, m->info.stack_need + 1 - kLds);
+        const int keep = m->width == 2 ? kKeep<2> : m->width == 8 ? kKeep<8> : kKeep<4>;
+        a.spill_depth = std::max(1, m->info.stack_need + 1 - keep);
         const int64_t per_block = (int64_t)a.spill_depth * kBlock * (int64_t)sizeof(uint32_t);
         const int64_t fit = (int64_t)(kScratchBytes - kScratchCounterBytes) / per_block;
         if (fit < std::min<int64_t>(blocks, ctx->cu_count))
