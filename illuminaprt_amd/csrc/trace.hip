@@ -25,9 +25,10 @@
 // and every push is written at the top and kept only when its child was hit;
 // the nearest child is visited first. Traversal is speculative (Aila and
 // Laine, "Understanding the Efficiency of Ray Traversal on GPUs", HPG 2009):
-// a lane that reaches a leaf parks it and keeps descending until half of the
+// a lane that reaches a leaf parks it and keeps descending until 3/8 of the
 // lanes in the node loop hold a leaf; the leaf loop then tests one triangle
-// per lane per iteration, each lane moving on to its next leaf on its own.
+// per lane per iteration, each lane moving on to its next leaf on its own,
+// and hands the wave back to the node loop once few lanes hold triangles.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -170,6 +171,12 @@ __device__ __forceinline__ void node_keys(const float4* __restrict__ nodes, uint
 // most this many live entries the pushes (each written at the top, kept when
 // the child was hit) never overwrite a live entry of the 16-entry ring.
 template <int W> constexpr int kKeep = kLds - (W - 1);
+
+// Speculation thresholds: the node loop ends once kSpecNum / kSpecDen of its
+// lanes hold a parked leaf; the leaf loop ends once fewer than kLeafNum /
+// kLeafDen of the wave's active lanes hold triangles (and a lane can descend).
+constexpr int kSpecNum = 3, kSpecDen = 8;
+constexpr int kLeafNum = 1, kLeafDen = 4;
 static_assert((kLds & (kLds - 1)) == 0, "the LDS ring is indexed by position & (kLds - 1)");
 
 // Waves per SIMD the traversal kernels are compiled for: seven (72 VGPRs)
@@ -292,9 +299,9 @@ __global__ void __launch_bounds__(kBlock, kTraceWaves<KIND>) k_trace4(TraceArgs 
         if (!__any(active)) break;
 
         // inner nodes; a lane that meets a leaf parks it and keeps descending
-        // until half of the lanes in the loop hold a parked leaf (three
-        // quarters measured 5% slower on incoherent rays, all of them slower
-        // still: the node loop's lanes idle longer than the leaf loop saves)
+        // until kSpecNum / kSpecDen of the lanes in the loop hold a parked leaf
+        // (3/8 measured best with the leaf loop below: three quarters was 9%
+        // slower on incoherent rays, waiting for all of them slower still)
         while (active && cur != kNoNode && !is_leaf(cur)) {
             if (STATS) ++n_nodes;
             uint32_t key[W];
@@ -326,36 +333,47 @@ __global__ void __launch_bounds__(kBlock, kTraceWaves<KIND>) k_trace4(TraceArgs 
                 stash = cur;
                 cur = pop();
             }
-            if (__popcll(__ballot(stash != kNoNode)) * 2 >= __popcll(__ballot(true))) break;
+            if (__popcll(__ballot(stash != kNoNode)) * kSpecDen >= __popcll(__ballot(true)) * kSpecNum) break;
         }
         // parked leaves, one triangle per lane per iteration: a lane moves on
         // to the next leaf on its stack as soon as its current one is done, so
-        // lanes with short leaves do not wait for the wave's longest one
-        while (active && stash != kNoNode) {
-            const float4* tp = a.tris + 3 * (int64_t)(leaf_first(stash) + tk);
-            float4 A = tp[0], B = tp[1], C = tp[2];
-            if (STATS) ++n_tris;
-            float t, u, v;
-            if (tri_watertight(r, mk(A.x, A.y, A.z), mk(B.x, B.y, B.z), mk(C.x, C.y, C.z), t, u, v)) {
-                best_t = t;
-                best_u = u;
-                best_v = v;
-                best_prim = __float_as_int(A.w);
-                r.thi = tmax_window(t);
-                if (ANY || (DUAL && any_ray)) {
-                    stash = kNoNode;
-                    tk = 0;
-                    finish();
-                    break;
+        // lanes with short leaves do not wait for the wave's longest one. The
+        // wave goes back to descending once fewer than kLeafNum / kLeafDen of
+        // its active lanes hold triangles while some lane has an inner node
+        // to visit (a lane keeps its parked leaf and its place in it).
+        for (;;) {
+            const bool has = active && stash != kNoNode;
+            const unsigned long long hm = __ballot(has);
+            if (hm == 0ull) break;
+            const unsigned long long dm = __ballot(active && cur != kNoNode && !is_leaf(cur));
+            if (dm != 0ull && __popcll(hm) * kLeafDen < __popcll(__ballot(active)) * kLeafNum) break;
+            if (has) {
+                const float4* tp = a.tris + 3 * (int64_t)(leaf_first(stash) + tk);
+                float4 A = tp[0], B = tp[1], C = tp[2];
+                if (STATS) ++n_tris;
+                float t, u, v;
+                bool stop = false;
+                if (tri_watertight(r, mk(A.x, A.y, A.z), mk(B.x, B.y, B.z), mk(C.x, C.y, C.z), t, u, v)) {
+                    best_t = t;
+                    best_u = u;
+                    best_v = v;
+                    best_prim = __float_as_int(A.w);
+                    r.thi = tmax_window(t);
+                    if (ANY || (DUAL && any_ray)) {
+                        stash = kNoNode;
+                        tk = 0;
+                        finish();
+                        stop = true;
+                    }
                 }
-            }
-            if (++tk >= leaf_count(stash)) {
-                tk = 0;
-                if (cur != kNoNode && is_leaf(cur)) {
-                    stash = cur;
-                    cur = pop();
-                } else {
-                    stash = kNoNode;
+                if (!stop && ++tk >= leaf_count(stash)) {
+                    tk = 0;
+                    if (cur != kNoNode && is_leaf(cur)) {
+                        stash = cur;
+                        cur = pop();
+                    } else {
+                        stash = kNoNode;
+                    }
                 }
             }
         }
