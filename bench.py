@@ -15,7 +15,8 @@ traversal kernel at two levels of the memory hierarchy:
   FETCH_SIZE / WRITE_SIZE counters (profiles/traffic_trace_incoherent.json,
   used only when it was measured on the current traversal source);
 - L2 (`roofline.l2`): bytes per ray = 32 + 16 + N_node * S_node + N_tri * 48
-  (S_node = 112 bytes, float BVH4), with N_node and N_tri the mean node
+  (S_node = 64 bytes for the quantised BVH4 traced by default, 112 with
+  --nodes float), with N_node and N_tri the mean node
   visits and triangle tests of a deterministic ordered traversal of the same
   flattened tree over the same rays on the host (ilm_bvh_visit_counts), per
   launch duration, against the L2 read bandwidth this run measures with the
@@ -79,9 +80,20 @@ def trace_source_sha16():
     return h.hexdigest()[:16]
 
 
-def traffic_from_profiles():
+def node_format(ilm, args):
+    """The BVH4 node format every single-scene leg builds (--nodes)."""
+    return ilm.ILM_NODES_QUANTIZED if args.nodes == "quantized" else ilm.ILM_NODES_FLOAT
+
+
+def trace_kernel_name(nodes):
+    """The closest-hit traversal kernel that traces float or quantised BVH4 nodes."""
+    return f"k_trace4<false, false, {1 if nodes == 'quantized' else 0}, false>"
+
+
+def traffic_from_profiles(nodes):
     """(HBM bytes per incoherent launch, note): from the committed counter
-    summary when its source digest matches the current traversal sources."""
+    summary when it was measured on the current traversal sources and on the
+    kernel of the node format traced."""
     if not os.path.exists(TRAFFIC_FILE):
         return None, "no counter summary"
     with open(TRAFFIC_FILE) as f:
@@ -89,10 +101,12 @@ def traffic_from_profiles():
     rel = os.path.relpath(TRAFFIC_FILE, ROOT)
     if d.get("trace_source_sha16") != trace_source_sha16():
         return None, f"{rel} was measured on other traversal sources ({d.get('trace_source_sha16')})"
+    if d.get("kernel") != "ilm::" + trace_kernel_name(nodes):
+        return None, f"{rel} was measured on {d.get('kernel')}, not on this node format's kernel"
     return d.get("hbm_bytes_per_launch"), f"{rel} (FETCH_SIZE and WRITE_SIZE, traversal sources {d['trace_source_sha16']})"
 
 
-def pmc_summary(dirs, copy_mib, out_path=TRAFFIC_FILE):
+def pmc_summary(dirs, copy_mib, out_path=TRAFFIC_FILE, nodes="quantized"):
     """Reads four rocprofv3 --pmc runs (CSV output directories: trace FETCH_SIZE,
     trace WRITE_SIZE, copy FETCH_SIZE, copy WRITE_SIZE), scales each trace
     counter by what it reads for the plain copy of copy_mib MiB each way,
@@ -116,7 +130,7 @@ def pmc_summary(dirs, copy_mib, out_path=TRAFFIC_FILE):
     copy_f = [v for _, k, v in per_dispatch(cf, "FETCH_SIZE") if "k_copy" in k]
     copy_w = [v for _, k, v in per_dispatch(cw, "WRITE_SIZE") if "k_copy" in k]
     fs, ws = kib / float(np.mean(copy_f)), kib / float(np.mean(copy_w))
-    name = "k_trace4<false, false, 0, false>"
+    name = trace_kernel_name(nodes)
     tr_f = [v for _, k, v in per_dispatch(tf, "FETCH_SIZE") if name in k]
     tr_w = [v for _, k, v in per_dispatch(tw, "WRITE_SIZE") if name in k]
     # the last six dispatches trace the incoherent set (1 warm-up + 5 timed)
@@ -181,7 +195,7 @@ def path_tracing_room(ctx, ilm, workloads, args):
     sa = workloads.box_room()
     pos, nrm, idx, tm, tl, mats, lights = sa
     mesh = ilm.Mesh(ctx, pos, idx)
-    mesh.build()
+    mesh.build(node_format=node_format(ilm, args))
     sc = ilm.Scene(ctx, mesh, nrm, tm, tl, mats, lights)
     cam = workloads.room_camera()
     acc = ctx.buffer(256 * 256 * 3 * 8).zero()
@@ -217,7 +231,7 @@ def frames_leg(ctx, ilm, workloads, args):
     W, H, depth = args.cache_width, args.cache_height, 10
     pos, nrm, idx, tm, tl, mats, lights = workloads.cache_scene()
     mesh = ilm.Mesh(ctx, pos, idx)
-    mesh.build()
+    mesh.build(node_format=node_format(ilm, args))
     sc = ilm.Scene(ctx, mesh, nrm, tm, tl, mats, lights)
     cam = workloads.cache_camera(W / H)
     acc = ctx.buffer(W * H * 24).zero()
@@ -255,7 +269,7 @@ def ic_frames_leg(ctx, ilm, workloads, args):
     W, H, depth = args.cache_width, args.cache_height, 10
     pos, nrm, idx, tm, tl, mats, lights = workloads.cache_scene()
     mesh = ilm.Mesh(ctx, pos, idx)
-    mesh.build()
+    mesh.build(node_format=node_format(ilm, args))
     sc = ilm.Scene(ctx, mesh, nrm, tm, tl, mats, lights)
     cam = workloads.cache_camera(W / H)
     acc = ctx.buffer(W * H * 24).zero()
@@ -530,7 +544,7 @@ def cache_convergence(ctx, ilm, workloads, args):
     depth, prefix = 10, 2
     pos, nrm, idx, tm, tl, mats, lights = workloads.cache_scene()
     mesh = ilm.Mesh(ctx, pos, idx)
-    mesh.build()
+    mesh.build(node_format=node_format(ilm, args))
     sc = ilm.Scene(ctx, mesh, nrm, tm, tl, mats, lights)
     cam = workloads.cache_camera(W / H)
     n3 = W * H * 3
@@ -767,7 +781,7 @@ def main():
     ap.add_argument("--batch-spp", type=int, default=2)
     ap.add_argument("--refill", type=int, default=0,
                     help="idle lanes that trigger a traversal wave's refill, for every ray set (0: 24, 56 for camera rays)")
-    ap.add_argument("--nodes", choices=["float", "quantized"], default="float",
+    ap.add_argument("--nodes", choices=["float", "quantized"], default="quantized",
                     help="BVH4 node format of the traced trees")
     ap.add_argument("--width", type=int, choices=[2, 4, 8], default=4, help="BVH width (float nodes)")
     ap.add_argument("--cache-only", action="store_true", help="run only the time-to-relRMSE leg")
@@ -807,7 +821,7 @@ def main():
         sys.exit(2)
 
     if args.pmc_summary:
-        pmc_summary(args.pmc_summary, args.pmc_copy_mib, args.pmc_out or TRAFFIC_FILE)
+        pmc_summary(args.pmc_summary, args.pmc_copy_mib, args.pmc_out or TRAFFIC_FILE, args.nodes)
         return
 
     import illuminaprt_amd as ilm
@@ -886,18 +900,13 @@ def main():
         # tree on the host (the roofline's counts), and the speculative
         # kernel's own counters beside them
         t0 = time.perf_counter()
-        if info["node_format"] == ilm.ILM_NODES_FLOAT:
-            c = mesh.visit_counts(buf.download(np.float32, 8 * n).reshape(n, 8), any_hit=anyhit)
-            nn, nt = float(c[:, 0].mean()), float(c[:, 1].mean())
-        else:
-            nn = nt = float("nan")
+        c = mesh.visit_counts(buf.download(np.float32, 8 * n).reshape(n, 8), any_hit=anyhit)
+        nn, nt = float(c[:, 0].mean()), float(c[:, 1].mean())
         count_s = time.perf_counter() - t0
         mesh.trace_stats(buf, n, anyhit, visits)
         ctx.sync()
         v = visits.download(np.int32, 2 * n).reshape(n, 2)
         sn, st_ = float(v[:, 0].mean()), float(v[:, 1].mean())
-        if nn != nn:
-            nn, nt = sn, st_
         steps = args.steps if name == "incoherent" else max(3, args.steps // 4)
         for _ in range(args.warmup):
             launch()
@@ -931,7 +940,7 @@ def main():
     l2_achieved = inc["bytes_per_ray"] * n / (inc["kernel_ms"] * 1e-3) / 1e9
     l2_peak, l2_probe_ms = l2_peak_gbs(ctx)
     log(f"L2 read probe: {l2_peak:.0f} GB/s ({l2_probe_ms:.3f} ms)")
-    traffic, traffic_note = traffic_from_profiles()
+    traffic, traffic_note = traffic_from_profiles(args.nodes)
     cpu = None
     if not args.no_cpu:
         rays_host = rs.incoherent.download(np.float32, 8 * n).reshape(n, 8)

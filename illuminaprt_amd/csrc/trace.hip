@@ -132,14 +132,20 @@ __device__ __forceinline__ void node_hits(const float4* __restrict__ nodes, uint
         const float Sy = __uint_as_float(((ex >> 8) & 0xFFu) << 23) * r.inv.y, By = fmaf(o.y, r.inv.y, -r.oinv.y);
         const float Sz = __uint_as_float(((ex >> 16) & 0xFFu) << 23) * r.inv.z, Bz = fmaf(o.z, r.inv.z, -r.oinv.z);
         const uint32_t cw[4] = {ch.x, ch.y, ch.z, ch.w};
+        // near and far byte planes by the direction signs (S has the sign of
+        // 1/d, and t = q * S + B is monotone in q): the same distances as the
+        // min/max form, in a select per axis instead of two per child
+        const bool fx = r.inv.x < 0.f, fy = r.inv.y < 0.f, fz = r.inv.z < 0.f;
+        const uint32_t nqx = fx ? qa.y : qa.x, fqx = fx ? qa.x : qa.y;
+        const uint32_t nqy = fy ? qa.w : qa.z, fqy = fy ? qa.z : qa.w;
+        const uint32_t nqz = fz ? qb.y : qb.x, fqz = fz ? qb.x : qb.y;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const int sh = 8 * k;
-            float tx0 = fmaf((float)((qa.x >> sh) & 0xFFu), Sx, Bx), tx1 = fmaf((float)((qa.y >> sh) & 0xFFu), Sx, Bx);
-            float ty0 = fmaf((float)((qa.z >> sh) & 0xFFu), Sy, By), ty1 = fmaf((float)((qa.w >> sh) & 0xFFu), Sy, By);
-            float tz0 = fmaf((float)((qb.x >> sh) & 0xFFu), Sz, Bz), tz1 = fmaf((float)((qb.y >> sh) & 0xFFu), Sz, Bz);
-            float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), r.tlo));
-            float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), r.thi));
+            const float tn = fmaxf(fmaxf(fmaf((float)((nqx >> sh) & 0xFFu), Sx, Bx), fmaf((float)((nqy >> sh) & 0xFFu), Sy, By)),
+                                   fmaxf(fmaf((float)((nqz >> sh) & 0xFFu), Sz, Bz), r.tlo));
+            const float tf = fminf(fminf(fmaf((float)((fqx >> sh) & 0xFFu), Sx, Bx), fmaf((float)((fqy >> sh) & 0xFFu), Sy, By)),
+                                   fminf(fmaf((float)((fqz >> sh) & 0xFFu), Sz, Bz), r.thi));
             d[k] = (tn <= tf * 1.00000048f && cw[k] != kEmptySlot) ? tn : INFINITY;
             c[k] = cw[k];
         }

@@ -218,9 +218,10 @@ int ilm_trace_closest_noalloc(ilm_mesh* mesh, const ilm_ray* rays, int64_t n, il
 
 /* Host-side traversal statistics (no GPU needed, no hits produced): per ray
  * (visits, host, 2 int32 each) the node visits and triangle tests of a
- * deterministic ordered traversal of the mesh's flattened float tree, the
+ * deterministic ordered traversal of the mesh's flattened tree (float or
+ * quantised nodes, each box decided by the kernels' own formula), the
  * traversal kernels' without speculation (nearest child first by the
- * kernels' exchange network, every child box tested, a closest hit shrinks
+ * kernels' child keys and exchange network, a closest hit shrinks
  * the window, an any hit ends the ray). rays are host memory. threads 0 =
  * all host cores. Prices a launch by algorithmic bytes per ray
  * (32 + 16 + visits * node bytes + tests * 48); the measured counterpart of

@@ -21,10 +21,11 @@ pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "room_ref_256_64k.npz")
 
 
-def _gpu_render(ctx, ilm, scene_arrays, cam, W, H, spp, seed=1234, max_depth=5, rr_depth=4, batch_paths=0):
+def _gpu_render(ctx, ilm, scene_arrays, cam, W, H, spp, seed=1234, max_depth=5, rr_depth=4, batch_paths=0,
+                node_format=0):
     pos, nrm, idx, tm, tl, mats, lights = scene_arrays
     m = ilm.Mesh(ctx, pos, idx)
-    m.build()
+    m.build(node_format=node_format)
     sc = ilm.Scene(ctx, m, nrm, tm, tl, mats, lights)
     acc = ctx.buffer(W * H * 3 * 8).zero()
     st = sc.render(cam, W, H, spp, acc, max_depth=max_depth, rr_depth=rr_depth, seed=seed,
@@ -87,6 +88,19 @@ def test_room_same_samples_matches_oracle(gpu_ctx, ilm, orc):
         assert abs(g_ext - c_ext) <= 0.01 * c_ext
         assert 0.5 * c_shd <= st["shadow_rays"] <= c_shd
     assert st["camera_rays"] == W * H * 16 and st["extension_rays"] > 0 and st["shadow_rays"] > 0
+
+
+def test_room_same_samples_quantized_nodes(gpu_ctx, ilm, orc):
+    """The integrator over a quantised BVH4 (the benchmarks' node format)
+    follows the oracle's paths like the float tree does."""
+    from illuminaprt_amd import workloads as w
+    sa = w.box_room()
+    cam = w.room_camera()
+    W = H = 128
+    g, st = _gpu_render(gpu_ctx, ilm, sa, cam, W, H, 16, max_depth=5, node_format=ilm.ILM_NODES_QUANTIZED)
+    c = orc.Scene(*sa).render(cam.as_list(), W, H, 16, max_depth=5, rr_depth=4, seed=1234)
+    _same_sample_agreement(g, c, 16, "room quantised nodes depth 5")
+    assert st["camera_rays"] == W * H * 16
 
 
 def test_reference_room_same_samples_matches_oracle(gpu_ctx, ilm, orc):
