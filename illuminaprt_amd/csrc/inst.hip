@@ -10,7 +10,10 @@
 // without normalising, so a hit's t is the same in both spaces, as in the
 // reference's object-space test. Updating the transforms (Space::Update, as
 // the reference's frame loop does: Utility/Sandbox/Multithreaded.h:399)
-// rebuilds only the top-level tree and the instance records.
+// rebuilds only the top-level tree and the instance records. The traversal
+// is the flat kernel's (k_trace4 in instance mode, trace.hip): persistent
+// waves with XCD-aware ray fetch, the LDS stack ring, the branch-free node
+// step and speculation inside the meshes' trees.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -37,160 +40,10 @@ struct ilm_instances {
     float4* d_inst = nullptr;
     int64_t tlas_bytes = 0, table_bytes = 0, ninst = 0, tlas_cap = 0, inst_cap = 0;
     int tlas_need = 0;
-    uint32_t* d_flag = nullptr;   // stack bound exceeded
 };
 
 namespace ilm {
 namespace {
-
-constexpr int kInstBlock = 128;
-constexpr int kInstStack = 64;     // entries per lane in LDS (32 KiB per workgroup)
-constexpr uint32_t kReturn = 0x7FFFFFFEu;  // stack marker: back to the world-space ray
-
-struct InstArgs {
-    const float4* __restrict__ tlas;   // WideNode<4>, 7 float4 each
-    const float4* __restrict__ blas;
-    const float4* __restrict__ tris;   // 3 float4 per triangle, object space
-    const float4* __restrict__ inst;   // 4 float4 per instance: world-to-object rows 0..2, (root, index, 0, 0)
-    const ilm_ray* __restrict__ rays;
-    int64_t n;
-    ilm_hit* __restrict__ hits;
-    int32_t* __restrict__ inst_out;
-    uint8_t* __restrict__ occ;
-    uint32_t* flag;
-};
-
-__device__ __forceinline__ void wide_hits(const float4* __restrict__ nodes, uint32_t cur, const RayState& r, float (&d)[4],
-                                          uint32_t (&c)[4]) {
-    const float4* np = nodes + (int64_t)cur * 7;
-    const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
-    const uint4 ch = reinterpret_cast<const uint4*>(np)[6];
-    const float lo_x[4] = {lx.x, lx.y, lx.z, lx.w}, hi_x[4] = {hx.x, hx.y, hx.z, hx.w};
-    const float lo_y[4] = {ly.x, ly.y, ly.z, ly.w}, hi_y[4] = {hy.x, hy.y, hy.z, hy.w};
-    const float lo_z[4] = {lz.x, lz.y, lz.z, lz.w}, hi_z[4] = {hz.x, hz.y, hz.z, hz.w};
-    const uint32_t cw[4] = {ch.x, ch.y, ch.z, ch.w};
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        float tx0 = fmaf(lo_x[k], r.inv.x, -r.oinv.x), tx1 = fmaf(hi_x[k], r.inv.x, -r.oinv.x);
-        float ty0 = fmaf(lo_y[k], r.inv.y, -r.oinv.y), ty1 = fmaf(hi_y[k], r.inv.y, -r.oinv.y);
-        float tz0 = fmaf(lo_z[k], r.inv.z, -r.oinv.z), tz1 = fmaf(hi_z[k], r.inv.z, -r.oinv.z);
-        float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), r.tlo));
-        float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), r.thi));
-        d[k] = (tn <= tf * 1.00000048f && cw[k] != kEmptySlot) ? tn : INFINITY;
-        c[k] = cw[k];
-    }
-}
-
-// One ray per lane, front-to-back over the top-level tree; at an instance
-// leaf the lane pushes the return marker and descends the instance's mesh
-// tree with the object-space ray; popping the marker restores the world ray
-// (its t window shrunk by the hits found meanwhile).
-template <bool ANY>
-__global__ void __launch_bounds__(kInstBlock) k_trace_inst(InstArgs a) {
-    __shared__ uint32_t lds[kInstStack * kInstBlock];
-    const int tid = threadIdx.x;
-    const int64_t i = (int64_t)blockIdx.x * kInstBlock + tid;
-    if (i >= a.n) return;
-    const float4 o4 = reinterpret_cast<const float4*>(a.rays + i)[0], d4 = reinterpret_cast<const float4*>(a.rays + i)[1];
-    const f3 wo = mk(o4.x, o4.y, o4.z), wd = mk(d4.x, d4.y, d4.z);
-    RayState r;
-    ray_setup(r, wo, wd, d4.w);
-    const float wthi0 = r.thi;
-    int sp = 0;
-    bool in_blas = false;
-    int32_t inst = -1, best_inst = -1, best_prim = -1;
-    float best_t = 0.f, best_u = 0.f, best_v = 0.f;
-    auto push = [&](uint32_t x) {
-        if (sp < kInstStack) lds[sp * kInstBlock + tid] = x;
-        else atomicOr(a.flag, 1u);
-        ++sp;
-    };
-    auto pop = [&]() -> uint32_t {
-        if (sp == 0) return kNoNode;
-        --sp;
-        return sp < kInstStack ? lds[sp * kInstBlock + tid] : kNoNode;
-    };
-    uint32_t cur = 0;
-    bool done = false;
-    while (!done) {
-        if (cur == kNoNode) break;
-        if (cur == kReturn) {
-            // back to the world-space ray, keeping the window the hits shrank
-            const float thi = r.thi;
-            ray_setup(r, wo, wd, d4.w);
-            r.thi = fminf(thi, wthi0);
-            in_blas = false;
-            cur = pop();
-            continue;
-        }
-        if (!in_blas) {
-            if (is_leaf(cur)) {
-                const uint32_t cnt = leaf_count(cur), first = leaf_first(cur);
-                if (cnt == 0) { cur = pop(); continue; }
-                if (cnt > 1) push(make_leaf(first + 1, cnt - 1));
-                const float4* ip = a.inst + 4 * (int64_t)first;
-                const float4 m0 = ip[0], m1 = ip[1], m2 = ip[2];
-                const int4 meta = reinterpret_cast<const int4*>(ip)[3];
-                const f3 oo = mk(fmaf(m0.x, wo.x, fmaf(m0.y, wo.y, fmaf(m0.z, wo.z, m0.w))),
-                                 fmaf(m1.x, wo.x, fmaf(m1.y, wo.y, fmaf(m1.z, wo.z, m1.w))),
-                                 fmaf(m2.x, wo.x, fmaf(m2.y, wo.y, fmaf(m2.z, wo.z, m2.w))));
-                const f3 od = mk(fmaf(m0.x, wd.x, fmaf(m0.y, wd.y, m0.z * wd.z)),
-                                 fmaf(m1.x, wd.x, fmaf(m1.y, wd.y, m1.z * wd.z)),
-                                 fmaf(m2.x, wd.x, fmaf(m2.y, wd.y, m2.z * wd.z)));
-                const float thi = r.thi;
-                ray_setup(r, oo, od, d4.w);
-                r.thi = thi;
-                push(kReturn);
-                in_blas = true;
-                inst = meta.y;
-                cur = (uint32_t)meta.x;
-                continue;
-            }
-            float d[4];
-            uint32_t c[4];
-            wide_hits(a.tlas, cur, r, d, c);
-            sort_children<4>(d, c);
-#pragma unroll
-            for (int k = 3; k >= 1; --k)
-                if (d[k] < INFINITY) push(c[k]);
-            cur = d[0] < INFINITY ? c[0] : pop();
-            continue;
-        }
-        if (is_leaf(cur)) {
-            const uint32_t first = leaf_first(cur), cnt = leaf_count(cur);
-            for (uint32_t k = 0; k < cnt; ++k) {
-                const float4* tp = a.tris + 3 * (int64_t)(first + k);
-                const float4 A = tp[0], B = tp[1], C = tp[2];
-                float t, u, v;
-                if (tri_watertight(r, mk(A.x, A.y, A.z), mk(B.x, B.y, B.z), mk(C.x, C.y, C.z), t, u, v)) {
-                    best_t = t;
-                    best_u = u;
-                    best_v = v;
-                    best_prim = __float_as_int(A.w);
-                    best_inst = inst;
-                    r.thi = tmax_window(t);
-                    if (ANY) { done = true; break; }
-                }
-            }
-            if (!done) cur = pop();
-            continue;
-        }
-        float d[4];
-        uint32_t c[4];
-        wide_hits(a.blas, cur, r, d, c);
-        sort_children<4>(d, c);
-#pragma unroll
-        for (int k = 3; k >= 1; --k)
-            if (d[k] < INFINITY) push(c[k]);
-        cur = d[0] < INFINITY ? c[0] : pop();
-    }
-    if (ANY) {
-        a.occ[i] = best_prim >= 0 ? 1 : 0;
-    } else {
-        reinterpret_cast<float4*>(a.hits)[i] = make_float4(best_t, __int_as_float(best_prim), best_u, best_v);
-        a.inst_out[i] = best_prim >= 0 ? best_inst : -1;
-    }
-}
 
 void rebase(uint32_t* child, uint32_t node_base, uint32_t slot_base) {
     for (int k = 0; k < 4; ++k) {
@@ -220,7 +73,7 @@ extern "C" {
 
 int ilm_instances_destroy(ilm_instances* t) {
     if (!t) return ILM_OK;
-    for (void* p : {t->d_blas, (void*)t->d_tris, t->d_tlas, (void*)t->d_inst, (void*)t->d_flag})
+    for (void* p : {t->d_blas, (void*)t->d_tris, t->d_tlas, (void*)t->d_inst})
         if (p) ilm_free(t->ctx, p);
     delete t;
     return ILM_OK;
@@ -275,10 +128,8 @@ int ilm_instances_create(ilm_ctx* ctx, ilm_mesh* const* meshes, int nmeshes, ilm
     t->tri_bytes = (int64_t)(tris.size() * sizeof(float));
     int rc = ilm_alloc(ctx, (size_t)t->blas_bytes, &t->d_blas);
     if (!rc) rc = ilm_alloc(ctx, (size_t)t->tri_bytes, (void**)&t->d_tris);
-    if (!rc) rc = ilm_alloc(ctx, sizeof(uint32_t), (void**)&t->d_flag);
     if (!rc) rc = ilm_upload(ctx, t->d_blas, blas.data(), (size_t)t->blas_bytes);
     if (!rc) rc = ilm_upload(ctx, t->d_tris, tris.data(), (size_t)t->tri_bytes);
-    if (!rc) rc = ilm_memset(ctx, t->d_flag, 0, sizeof(uint32_t));
     if (rc) {
         ilm_instances_destroy(t);
         return rc;
@@ -354,8 +205,6 @@ int ilm_instances_set(ilm_instances* t, const ilm_instance* inst, int64_t n) {
         }
         tlas.assign(reinterpret_cast<const uint8_t*>(&e), reinterpret_cast<const uint8_t*>(&e) + sizeof e);
     }
-    if (tneed + 1 + t->blas_need + 3 > kInstStack)
-        return fail(ILM_ERR_INVALID, "two-level stack need above 64 entries");
     // instance records in the top-level tree's leaf order
     std::vector<float4> rec((size_t)std::max<size_t>(order.size(), 1) * 4);
     for (size_t s = 0; s < order.size(); ++s) {
@@ -403,26 +252,11 @@ template <bool ANY>
 int launch_inst(ilm_instances* t, const ilm_ray* rays, int64_t n, ilm_hit* hits, int32_t* inst, uint8_t* occ,
                 void* stream) {
     if (!t) return fail(ILM_ERR_INVALID, "null instance table");
-    if (n < 0 || (n > 0 && !rays)) return fail(ILM_ERR_INVALID, "bad ray buffer");
     if (!t->d_tlas) return fail(ILM_ERR_STATE, "instance table not set");
-    if (n == 0) return ILM_OK;
-    if (ANY ? !occ : (!hits || !inst)) return fail(ILM_ERR_INVALID, "null output");
-    ILM_HIP(hipSetDevice(t->ctx->device));
-    InstArgs a;
-    a.tlas = reinterpret_cast<const float4*>(t->d_tlas);
-    a.blas = reinterpret_cast<const float4*>(t->d_blas);
-    a.tris = reinterpret_cast<const float4*>(t->d_tris);
-    a.inst = t->d_inst;
-    a.rays = rays;
-    a.n = n;
-    a.hits = hits;
-    a.inst_out = inst;
-    a.occ = occ;
-    a.flag = t->d_flag;
-    hipLaunchKernelGGL((k_trace_inst<ANY>), dim3((unsigned)((n + kInstBlock - 1) / kInstBlock)), dim3(kInstBlock), 0,
-                       pick_stream(t->ctx, stream), a);
-    ILM_HIP(hipGetLastError());
-    return ILM_OK;
+    // a lane's stack holds the top-level entries, the remainder of a
+    // top-level leaf, kReturn and the mesh tree's entries
+    return launch_trace_inst(t->ctx, t->d_tlas, t->d_blas, t->d_tris, t->d_inst, t->tlas_need + 2 + t->blas_need,
+                             rays, n, hits, inst, occ, ANY, stream);
 }
 
 }  // namespace
@@ -451,16 +285,8 @@ int ilm_instances_info(ilm_instances* t, int64_t* ninst, int64_t* blas_bytes, in
 
 int ilm_instances_check(ilm_instances* t) {
     if (!t) return fail(ILM_ERR_INVALID, "null instance table");
-    uint32_t flag = 0;
-    int rc = ilm_download(t->ctx, &flag, t->d_flag, sizeof flag);
-    if (rc) return rc;
-    if (flag) {
-        const uint32_t zero = 0;
-        rc = ilm_upload(t->ctx, t->d_flag, &zero, sizeof zero);
-        if (rc) return rc;
-        return fail(ILM_ERR_STATE, "a two-level traversal exceeded its stack; results are incomplete");
-    }
-    return ILM_OK;
+    // the traversal flags a stack bound exceeded in its context's counters
+    return ilm_sync(t->ctx);
 }
 
 }  // extern "C"

@@ -79,6 +79,14 @@ int ensure_spill(ilm_ctx* ctx, size_t words);
 // and zeroes its fetch counters (once per device; called at context creation).
 int noalloc_setup(int device, hipStream_t stream);
 hipStream_t pick_stream(ilm_ctx* ctx, void* stream);
+// Two-level traversal of an instance table: tlas = top-level float BVH4 over
+// the instances' world boxes, blas = the meshes' float BVH4 trees in one node
+// array, tris = their triangles (3 float4 each, object space), inst = the
+// instance records (4 float4 each: world-to-object rows 0..2, then (mesh
+// root, instance index, mesh, 0)). Closest hits (hits, inst_out) or any hits (occ).
+int launch_trace_inst(ilm_ctx* ctx, const void* tlas, const void* blas, const float* tris, const float4* inst,
+                      int stack_need, const ilm_ray* rays, int64_t n, ilm_hit* hits, int32_t* inst_out, uint8_t* occ,
+                      bool any, void* stream);
 // PerspectiveCamera frame: out = centre, corner, across, up (12 floats).
 void camera_frame(const ilm_camera& c, float out[12]);
 // Closest hit (occ == null) or any hit over a ray queue whose length lives in

@@ -66,6 +66,13 @@ struct TraceArgs {
     int64_t n2;                        // its count, or capacity when n2_dev is set
     const unsigned long long* n2_dev;
     uint8_t* __restrict__ occ2;
+    // instance tables (two-level traversal): the top-level tree over the
+    // instances' world boxes (nodes then holds the meshes' trees), the
+    // instance records in its leaf order (world-to-object rows 0..2, then
+    // (mesh root node, instance index, mesh, 0)) and the hit instances out
+    const float4* __restrict__ tlas;
+    const float4* __restrict__ inst;
+    int32_t* __restrict__ inst_out;
 };
 
 __device__ __forceinline__ float slab1(float lox, float hix, float loy, float hiy, float loz, float hiz,
@@ -190,8 +197,19 @@ static_assert((kLds & (kLds - 1)) == 0, "the LDS ring is indexed by position & (
 // allows; BVH8 nodes need ~120 VGPRs and keep the four those allow.
 template <int K> constexpr int kTraceWaves = K == kFloat8 ? 4 : 7;
 
-template <bool ANY, bool STATS, int KIND, bool DUAL>
+// Stack marker of the two-level traversal: back to the world-space ray.
+constexpr uint32_t kReturn = 0x7FFFFFFEu;
+
+// INST: two-level traversal of an instance table. A lane walks the top-level
+// tree in world space; at an instance leaf it pushes kReturn and continues in
+// the instance's mesh tree with the ray transformed into object space (the
+// direction not normalised, so t is shared); popping kReturn reloads the
+// world ray. Leaves are parked only inside a mesh tree, and a lane holding
+// one stops at kReturn until the leaf's triangles are tested with the
+// object-space ray.
+template <bool ANY, bool STATS, int KIND, bool DUAL, bool INST = false>
 __global__ void __launch_bounds__(kBlock, kTraceWaves<KIND>) k_trace4(TraceArgs a) {
+    static_assert(!INST || (KIND == kFloat4 && !DUAL && !STATS), "instance tables hold float BVH4 trees");
     constexpr int W = Kind<KIND>::W;
     // per lane: a kLds-entry stack ring (lane-strided, conflict-free) and W
     // staging words for the current node's child words
@@ -229,6 +247,8 @@ __global__ void __launch_bounds__(kBlock, kTraceWaves<KIND>) k_trace4(TraceArgs 
     float best_t = 0.f, best_u = 0.f, best_v = 0.f;
     int32_t best_prim = -1;
     int32_t n_nodes = 0, n_tris = 0;
+    bool in_blas = false;              // INST: the lane is inside an instance's mesh tree
+    int32_t inst_id = -1, best_inst = -1;
 
     auto pop = [&]() -> uint32_t {
         if (sp > base) {
@@ -249,12 +269,35 @@ __global__ void __launch_bounds__(kBlock, kTraceWaves<KIND>) k_trace4(TraceArgs 
         } else {
             float4 h = make_float4(best_t, __int_as_float(best_prim), best_u, best_v);
             reinterpret_cast<float4*>(a.hits)[ray_id] = h;
+            if (INST) a.inst_out[ray_id] = best_prim >= 0 ? best_inst : -1;
         }
         if (STATS) {
             a.stats[2 * (int64_t)ray_id] = n_nodes;
             a.stats[2 * (int64_t)ray_id + 1] = n_tris;
         }
         active = false;
+    };
+    // one push (INST's instance entries), spilling the oldest ring entry when full
+    auto push1 = [&](uint32_t x) {
+        while (sp - base >= kLds) {
+            if (base < a.spill_depth)
+                a.spill[(int64_t)base * a.grid_threads + gtid] = ring[(base & (kLds - 1)) * kBlock];
+            else
+                atomicOr(&a.counters[2], 1ull);
+            ++base;
+        }
+        ring[(sp & (kLds - 1)) * kBlock] = x;
+        ++sp;
+    };
+    // the lane has a traversal step to take: an inner node (INST: also any
+    // top-level entry, and kReturn once no leaf is parked)
+    auto can_step = [&]() -> bool {
+        if (!active || cur == kNoNode) return false;
+        if (INST) {
+            if (!in_blas) return true;
+            if (cur == kReturn) return stash == kNoNode;
+        }
+        return !is_leaf(cur);
     };
 
     for (;;) {
@@ -298,6 +341,8 @@ __global__ void __launch_bounds__(kBlock, kTraceWaves<KIND>) k_trace4(TraceArgs 
                     best_prim = -1;
                     best_t = best_u = best_v = 0.f;
                     n_nodes = n_tris = 0;
+                    in_blas = false;
+                    inst_id = best_inst = -1;
                     active = true;
                 }
             }
@@ -308,10 +353,49 @@ __global__ void __launch_bounds__(kBlock, kTraceWaves<KIND>) k_trace4(TraceArgs 
         // until kSpecNum / kSpecDen of the lanes in the loop hold a parked leaf
         // (3/8 measured best with the leaf loop below: three quarters was 9%
         // slower on incoherent rays, waiting for all of them slower still)
-        while (active && cur != kNoNode && !is_leaf(cur)) {
+        while (can_step()) {
+            if (INST && in_blas && cur == kReturn) {
+                // back to the world-space ray, keeping the window its hits shrank
+                const float4* rp = reinterpret_cast<const float4*>(a.rays + ray_id);
+                const float4 o4 = rp[0], d4 = rp[1];
+                const float thi = r.thi;
+                ray_setup(r, mk(o4.x, o4.y, o4.z), mk(d4.x, d4.y, d4.z), d4.w);
+                r.thi = fminf(thi, r.thi);
+                in_blas = false;
+                cur = pop();
+                continue;
+            }
+            if (INST && is_leaf(cur)) {
+                // a top-level leaf: enter its first instance (the rest wait on the stack)
+                const uint32_t first = leaf_first(cur), cnt = leaf_count(cur);
+                if (cnt == 0) {
+                    cur = pop();
+                    continue;
+                }
+                if (cnt > 1) push1(make_leaf(first + 1, cnt - 1));
+                const float4* rp = reinterpret_cast<const float4*>(a.rays + ray_id);
+                const float4 o4 = rp[0], d4 = rp[1];
+                const float4* ip = a.inst + 4 * (int64_t)first;
+                const float4 m0 = ip[0], m1 = ip[1], m2 = ip[2];
+                const int4 meta = reinterpret_cast<const int4*>(ip)[3];
+                const f3 oo = mk(fmaf(m0.x, o4.x, fmaf(m0.y, o4.y, fmaf(m0.z, o4.z, m0.w))),
+                                 fmaf(m1.x, o4.x, fmaf(m1.y, o4.y, fmaf(m1.z, o4.z, m1.w))),
+                                 fmaf(m2.x, o4.x, fmaf(m2.y, o4.y, fmaf(m2.z, o4.z, m2.w))));
+                const f3 od = mk(fmaf(m0.x, d4.x, fmaf(m0.y, d4.y, m0.z * d4.z)),
+                                 fmaf(m1.x, d4.x, fmaf(m1.y, d4.y, m1.z * d4.z)),
+                                 fmaf(m2.x, d4.x, fmaf(m2.y, d4.y, m2.z * d4.z)));
+                const float thi = r.thi;
+                ray_setup(r, oo, od, d4.w);
+                r.thi = thi;
+                push1(kReturn);
+                in_blas = true;
+                inst_id = meta.y;
+                cur = (uint32_t)meta.x;
+                continue;
+            }
             if (STATS) ++n_nodes;
             uint32_t key[W];
-            node_keys<KIND>(a.nodes, cur, r, key, stage);
+            node_keys<KIND>(INST && !in_blas ? a.tlas : a.nodes, cur, r, key, stage);
             sort_keys<W>(key);
             // make room for W - 1 pushes: the oldest ring entries go to the
             // spill area (rare: stacks deeper than kKeep)
@@ -335,7 +419,7 @@ __global__ void __launch_bounds__(kBlock, kTraceWaves<KIND>) k_trace4(TraceArgs 
                 sp += key[k] < kMissKey ? 1 : 0;
             }
             cur = key[0] < kMissKey ? cw[0] : pop();
-            if (cur != kNoNode && is_leaf(cur) && stash == kNoNode) {
+            if ((!INST || in_blas) && cur != kNoNode && is_leaf(cur) && stash == kNoNode) {
                 stash = cur;
                 cur = pop();
             }
@@ -351,7 +435,7 @@ __global__ void __launch_bounds__(kBlock, kTraceWaves<KIND>) k_trace4(TraceArgs 
             const bool has = active && stash != kNoNode;
             const unsigned long long hm = __ballot(has);
             if (hm == 0ull) break;
-            const unsigned long long dm = __ballot(active && cur != kNoNode && !is_leaf(cur));
+            const unsigned long long dm = __ballot(can_step());
             if (dm != 0ull && __popcll(hm) * kLeafDen < __popcll(__ballot(active)) * kLeafNum) break;
             if (has) {
                 const float4* tp = a.tris + 3 * (int64_t)(leaf_first(stash) + tk);
@@ -364,6 +448,7 @@ __global__ void __launch_bounds__(kBlock, kTraceWaves<KIND>) k_trace4(TraceArgs 
                     best_u = u;
                     best_v = v;
                     best_prim = __float_as_int(A.w);
+                    best_inst = inst_id;
                     r.thi = tmax_window(t);
                     if (ANY || (DUAL && any_ray)) {
                         stash = kNoNode;
@@ -525,6 +610,51 @@ static int launch_trace(ilm_mesh* m, const ilm_ray* rays, int64_t n, ilm_hit* hi
         hipLaunchKernelGGL((k_trace4<ANY, STATS, kFloat8, DUAL>), dim3((unsigned)blocks), dim3(kBlock), 0, st, a);
     else
         hipLaunchKernelGGL((k_trace4<ANY, STATS, kFloat4, DUAL>), dim3((unsigned)blocks), dim3(kBlock), 0, st, a);
+    ILM_HIP(hipGetLastError());
+    return ILM_OK;
+}
+
+// Two-level traversal of an instance table (k_trace4 in instance mode): the
+// persistent grid, XCD-aware fetch and stack machinery of the flat kernel.
+int launch_trace_inst(ilm_ctx* ctx, const void* tlas, const void* blas, const float* tris, const float4* inst,
+                      int stack_need, const ilm_ray* rays, int64_t n, ilm_hit* hits, int32_t* inst_out, uint8_t* occ,
+                      bool any, void* stream) {
+    if (!ctx) return fail(ILM_ERR_STATE, "instance table without a device context");
+    if (n < 0 || (n > 0 && !rays)) return fail(ILM_ERR_INVALID, "bad ray buffer");
+    if (n == 0) return ILM_OK;
+    if (n > (int64_t)UINT32_MAX) return fail(ILM_ERR_INVALID, "a launch traces fewer than 2^32 rays");
+    if (any ? !occ : (!hits || !inst_out)) return fail(ILM_ERR_INVALID, "null output");
+    ILM_HIP(hipSetDevice(ctx->device));
+    const void* fn = any ? (const void*)k_trace4<true, false, kFloat4, false, true>
+                         : (const void*)k_trace4<false, false, kFloat4, false, true>;
+    int per_cu = 0;
+    ILM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kBlock, 0));
+    per_cu = std::max(per_cu, 1);
+    int64_t blocks = std::min<int64_t>((int64_t)per_cu * ctx->cu_count, (n + kBlock - 1) / kBlock);
+    blocks = std::max<int64_t>(blocks, 1);
+    TraceArgs a{};
+    a.nodes = reinterpret_cast<const float4*>(blas);
+    a.tris = reinterpret_cast<const float4*>(tris);
+    a.rays = rays;
+    a.n = n;
+    a.hits = hits;
+    a.occ = occ;
+    a.grid_threads = blocks * kBlock;
+    a.refill_min = ctx->trace_refill;
+    a.spill_depth = std::max(1, stack_need + 1);
+    int rc = ensure_spill(ctx, (size_t)a.spill_depth * (size_t)a.grid_threads);
+    if (rc) return rc;
+    a.spill = ctx->spill;
+    a.counters = ctx->counters;
+    a.total_waves = (uint32_t)(blocks * (kBlock / 64));
+    a.tlas = reinterpret_cast<const float4*>(tlas);
+    a.inst = inst;
+    a.inst_out = inst_out;
+    hipStream_t st = pick_stream(ctx, stream);
+    if (any)
+        hipLaunchKernelGGL((k_trace4<true, false, kFloat4, false, true>), dim3((unsigned)blocks), dim3(kBlock), 0, st, a);
+    else
+        hipLaunchKernelGGL((k_trace4<false, false, kFloat4, false, true>), dim3((unsigned)blocks), dim3(kBlock), 0, st, a);
     ILM_HIP(hipGetLastError());
     return ILM_OK;
 }

@@ -317,7 +317,9 @@ int ilm_instances_set(ilm_instances* table, const ilm_instance* inst, int64_t n)
 /* Closest hit of n rays (dev pointers): hits[i] = (t, triangle index within
  * the instance's mesh, u, v) and inst[i] = the instance's index in the last
  * ilm_instances_set (-1 on a miss), as BasicSpace::Intersects reports the
- * primitive and its surface. */
+ * primitive and its surface. The launch uses the context's ray-fetch
+ * counters and spill area, like ilm_trace_closest: a context's traversal
+ * launches must be ordered on one stream. */
 int ilm_instances_trace_closest(ilm_instances* table, const ilm_ray* rays, int64_t n, ilm_hit* hits, int32_t* inst,
                                 void* stream);
 /* Any hit (BasicSpace::Intersects(const Ray&), BasicSpace.cpp:92-101). */
@@ -328,8 +330,8 @@ int ilm_instances_trace_any(ilm_instances* table, const ilm_ray* rays, int64_t n
 int ilm_instances_info(ilm_instances* table, int64_t* ninst, int64_t* mesh_bytes, int64_t* table_bytes,
                        int64_t* tlas_bytes, int32_t* stack_need);
 /* Waits for the context stream and reports ILM_ERR_STATE (once) if a
- * traversal exceeded its stack (never expected: ilm_instances_set rejects
- * tables whose stack need exceeds the kernel's 64 entries). */
+ * traversal on the table's context exceeded its stack bound (never expected:
+ * the spill area is sized from the table's stack need), as ilm_sync does. */
 int ilm_instances_check(ilm_instances* table);
 
 /* ---------------------------------------------------------------- ray sets */
