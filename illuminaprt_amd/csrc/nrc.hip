@@ -254,7 +254,8 @@ struct TrainArgs {
 __device__ __forceinline__ int64_t hoff(int layer, int64_t ld) { return layer == 0 ? 0 : (int64_t)kIn * ld + (int64_t)(layer - 1) * kHid * ld; }
 __device__ __forceinline__ int64_t goff(int layer, int64_t ld) { return (int64_t)(layer - 1) * kHid * ld; }  // layer 1..6
 
-__global__ void __launch_bounds__(256) k_train_fb(TrainArgs a) {
+constexpr int kFbBlock = 256;
+__global__ void __launch_bounds__(kFbBlock) k_train_fb(TrainArgs a) {
     __shared__ __bf16 wf[kFwdElems];
     __shared__ __bf16 wb[kBwdElems];
     lds_copy(wf, a.fwd, kFwdElems);
@@ -262,9 +263,10 @@ __global__ void __launch_bounds__(256) k_train_fb(TrainArgs a) {
     __syncthreads();
     const int lane = (int)__lane_id(), g = lane >> 4, m = lane & 15;
     const int wave = threadIdx.x >> 6;
+    constexpr int kWaves = kFbBlock / 64;
     const int64_t n = a.n, ld = a.ld;
-    const int64_t step = (int64_t)gridDim.x * 4 * 16;
-    for (int64_t base = ((int64_t)blockIdx.x * 4 + wave) * 16; base < n; base += step) {
+    const int64_t step = (int64_t)gridDim.x * kWaves * 16;
+    for (int64_t base = ((int64_t)blockIdx.x * kWaves + wave) * 16; base < n; base += step) {
         int lo8 = lane * 8, lo4 = lane * 4;  // opaque offsets: weights stay in LDS (see k_infer)
         asm volatile("" : "+v"(lo8), "+v"(lo4));
         const int64_t smp = base + m;
@@ -448,8 +450,11 @@ __global__ void __launch_bounds__(256) k_adam(AdamArgs a) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int p = blockIdx.x * 64 + lane;
     float part = 0.f;
-    if (a.do_update && p < kParams)
+    if (a.do_update && p < kParams) {
+        // eight rows in flight per wave (independent loads; the sum keeps its order)
+#pragma unroll 8
         for (int b = wave; b < a.nblocks; b += 4) part += a.partial[(int64_t)b * kParams + p];
+    }
     quarter[wave][lane] = part;
     __syncthreads();
     if (wave != 0 || p >= kParams) return;
@@ -727,7 +732,7 @@ int ilm_nrc_train_step(ilm_nrc* r, const float* queries, const float* targets, i
     t.loss = loss;
     t.grad_scale = 1.f / (3.f * (float)n);
     int64_t fb_blocks = std::min<int64_t>((n + 63) / 64, (int64_t)r->ctx->cu_count * 2);
-    hipLaunchKernelGGL(k_train_fb, dim3((unsigned)fb_blocks), dim3(256), 0, st, t);
+    hipLaunchKernelGGL(k_train_fb, dim3((unsigned)fb_blocks), dim3(kFbBlock), 0, st, t);
     DwArgs d;
     d.H = r->H;
     d.G = r->G;
