@@ -254,6 +254,9 @@ struct TrainArgs {
 __device__ __forceinline__ int64_t hoff(int layer, int64_t ld) { return layer == 0 ? 0 : (int64_t)kIn * ld + (int64_t)(layer - 1) * kHid * ld; }
 __device__ __forceinline__ int64_t goff(int layer, int64_t ld) { return (int64_t)(layer - 1) * kHid * ld; }  // layer 1..6
 
+// Two 256-thread workgroups per CU (80 KB of packed weights each in LDS).
+// 1024-thread workgroups (4 waves per SIMD) measured no faster: the kernel's
+// time goes to its feature-major activation and gradient stores.
 constexpr int kFbBlock = 256;
 __global__ void __launch_bounds__(kFbBlock) k_train_fb(TrainArgs a) {
     __shared__ __bf16 wf[kFwdElems];
@@ -283,32 +286,41 @@ __global__ void __launch_bounds__(kFbBlock) k_train_fb(TrainArgs a) {
         if (valid)
             for (int c = 0; c < 3; ++c)
                 for (int e = 0; e < 8; ++e) (a.H + hoff(0, ld) + (int64_t)(32 * c + e) * ld)[off8] = x[c][e];
-        bf16x8 h[6][2];  // h[l] = output of layer l-1 (layers 1..5), as B chunks
+        // hc = the current layer's output, as B chunks. Each layer's output is
+        // stored (H_l, for the weight gradients) as soon as it exists, and
+        // the backward pass keeps only its ReLU derivative: bit 8c + e of
+        // relu_on[l] is set when feature (c, e) of H_l is positive. Holding
+        // the masks instead of the five layers' outputs saves ~40 registers.
+        bf16x8 hc[2];
+        uint32_t relu_on[6];
         f32x4 acc[4];
         for (int t = 0; t < 4; ++t) {
             acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
             for (int c = 0; c < 3; ++c)
                 acc[t] = mfma32(*reinterpret_cast<const bf16x8*>(wf + kF0 + (t * 3 + c) * 512 + lo8), x[c], acc[t]);
         }
-        for (int c = 0; c < 2; ++c) h[1][c] = relu_pack(acc[2 * c], acc[2 * c + 1]);
-        order_after(lo8, acc[3][3]);
-        for (int l = 1; l <= 4; ++l) {
-            for (int t = 0; t < 4; ++t) {
-                acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-                for (int c = 0; c < 2; ++c)
-                    acc[t] = mfma32(*reinterpret_cast<const bf16x8*>(wf + kFh + (l - 1) * 8 * 512 + (t * 2 + c) * 512 + lo8),
-                                    h[l][c], acc[t]);
+        for (int l = 1; l <= 5; ++l) {
+            if (l > 1) {
+                for (int t = 0; t < 4; ++t) {
+                    acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+                    for (int c = 0; c < 2; ++c)
+                        acc[t] = mfma32(*reinterpret_cast<const bf16x8*>(wf + kFh + (l - 2) * 8 * 512 + (t * 2 + c) * 512 + lo8),
+                                        hc[c], acc[t]);
+                }
             }
-            for (int c = 0; c < 2; ++c) h[l + 1][c] = relu_pack(acc[2 * c], acc[2 * c + 1]);
+            for (int c = 0; c < 2; ++c) hc[c] = relu_pack(acc[2 * c], acc[2 * c + 1]);
             order_after(lo8, acc[3][3]);
+            // store H_l (feature of K slot (c, g, e) is perm(c, g, e))
+            uint32_t on = 0;
+            for (int c = 0; c < 2; ++c)
+                for (int e = 0; e < 8; ++e) {
+                    if (valid) (a.H + hoff(l, ld) + (int64_t)perm(c, 0, e) * ld)[off4] = hc[c][e];
+                    on |= ((float)hc[c][e] > 0.f ? 1u : 0u) << (8 * c + e);
+                }
+            relu_on[l] = on;
         }
-        // store H1..H5 (feature of K slot (c, g, e) is perm(c, g, e))
-        if (valid)
-            for (int l = 1; l <= 5; ++l)
-                for (int c = 0; c < 2; ++c)
-                    for (int e = 0; e < 8; ++e) (a.H + hoff(l, ld) + (int64_t)perm(c, 0, e) * ld)[off4] = h[l][c][e];
         f32x4 o = f32x4{0.f, 0.f, 0.f, 0.f};
-        for (int c = 0; c < 2; ++c) o = mfma32(*reinterpret_cast<const bf16x8*>(wf + kF5 + c * 512 + lo8), h[5][c], o);
+        for (int c = 0; c < 2; ++c) o = mfma32(*reinterpret_cast<const bf16x8*>(wf + kF5 + c * 512 + lo8), hc[c], o);
         // relative L2 loss on rgb: (y - t)^2 / (sg(y)^2 + 0.01); lanes g == 0 hold outputs 0..3
         bf16x4 g6;
         float lsum = 0.f;
@@ -337,7 +349,7 @@ __global__ void __launch_bounds__(kFbBlock) k_train_fb(TrainArgs a) {
             for (int c = 0; c < 2; ++c)
                 for (int e = 0; e < 8; ++e) {
                     float v = d[2 * c + (e >> 2)][e & 3];
-                    gl[c][e] = (__bf16)(((float)h[l][c][e] > 0.f) ? v : 0.f);
+                    gl[c][e] = (__bf16)(((relu_on[l] >> (8 * c + e)) & 1u) ? v : 0.f);
                 }
             if (valid)
                 for (int c = 0; c < 2; ++c)
