@@ -25,8 +25,8 @@
 // and every push is written at the top and kept only when its child was hit;
 // the nearest child is visited first. Traversal is speculative (Aila and
 // Laine, "Understanding the Efficiency of Ray Traversal on GPUs", HPG 2009):
-// a lane that reaches a leaf parks it and keeps descending until 3/8 of the
-// lanes in the node loop hold a leaf; the leaf loop then tests one triangle
+// a lane that reaches a leaf parks it and keeps descending until a quarter of
+// the lanes in the node loop hold a leaf; the leaf loop then tests one triangle
 // per lane per iteration, each lane moving on to its next leaf on its own,
 // and hands the wave back to the node loop once few lanes hold triangles.
 #include <hip/hip_runtime.h>
@@ -188,7 +188,7 @@ template <int W> constexpr int kKeep = kLds - (W - 1);
 // Speculation thresholds: the node loop ends once kSpecNum / kSpecDen of its
 // lanes hold a parked leaf; the leaf loop ends once fewer than kLeafNum /
 // kLeafDen of the wave's active lanes hold triangles (and a lane can descend).
-constexpr int kSpecNum = 3, kSpecDen = 8;
+constexpr int kSpecNum = 1, kSpecDen = 4;
 constexpr int kLeafNum = 1, kLeafDen = 4;
 static_assert((kLds & (kLds - 1)) == 0, "the LDS ring is indexed by position & (kLds - 1)");
 
@@ -351,8 +351,9 @@ __global__ void __launch_bounds__(kBlock, kTraceWaves<KIND>) k_trace4(TraceArgs 
 
         // inner nodes; a lane that meets a leaf parks it and keeps descending
         // until kSpecNum / kSpecDen of the lanes in the loop hold a parked leaf
-        // (3/8 measured best with the leaf loop below: three quarters was 9%
-        // slower on incoherent rays, waiting for all of them slower still)
+        // (a quarter measured best with the leaf loop below and quantised
+        // nodes: 1/8, 3/16, 3/8 and 1/2 were 2-5% slower on incoherent rays,
+        // three quarters 9% slower with float nodes)
         while (can_step()) {
             if (INST && in_blas && cur == kReturn) {
                 // back to the world-space ray, keeping the window its hits shrank
