@@ -329,8 +329,15 @@ ILM_HD bool tri_watertight(const RayState& r, f3 v0, f3 v1, f3 v2, float& t_out,
 // bounce, stream), so renders are reproducible and need no shared state.
 ILM_HD void mulhilo64(uint64_t a, uint64_t b, uint64_t& hi, uint64_t& lo) {
 #if defined(__HIP_DEVICE_COMPILE__)
-    lo = a * b;
-    hi = __umul64hi(a, b);
+    // the 128-bit product from four 32 x 32 -> 64 multiply-adds (the carries
+    // of the middle terms folded into them), where a * b and __umul64hi
+    // separately take six multiplies
+    const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32), b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
+    const uint64_t p00 = (uint64_t)a0 * b0;
+    const uint64_t t = (uint64_t)a0 * b1 + (p00 >> 32);          // < 2^64
+    const uint64_t u = (uint64_t)a1 * b0 + (uint32_t)t;          // < 2^64
+    lo = ((uint64_t)(uint32_t)u << 32) | (uint32_t)p00;
+    hi = (uint64_t)a1 * b1 + (t >> 32) + (u >> 32);
 #else
     unsigned __int128 p = (unsigned __int128)a * b;
     hi = (uint64_t)(p >> 64);
