@@ -36,9 +36,15 @@ CFLAGS = [
 ]
 
 
+# Per-source flags. The cache MLP's kernels keep their MFMA accumulators in
+# VGPRs instead of AGPRs: their occupancy is set by LDS, so the registers are
+# there, and results in AGPRs must be copied out before every ReLU and store.
+EXTRA_FLAGS = {"nrc.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+
+
 def _compile(src: str) -> str:
     obj = os.path.join(OUT, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC, *CFLAGS, "-c", os.path.join(SRC, src), "-o", obj]
+    cmd = [HIPCC, *CFLAGS, *EXTRA_FLAGS.get(src, []), "-c", os.path.join(SRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {src}\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

@@ -38,6 +38,24 @@ namespace nrc {
 typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
 typedef __attribute__((ext_vector_type(4))) __bf16 bf16x4;
 typedef __attribute__((ext_vector_type(4))) float f32x4;
+typedef __attribute__((ext_vector_type(2))) __bf16 bf16x2;
+typedef __attribute__((ext_vector_type(2))) float f32x2;
+typedef __attribute__((ext_vector_type(2))) short i16x2;
+typedef __attribute__((ext_vector_type(4))) unsigned u32x4;
+
+// Two floats rounded to bf16 (nearest even) by one v_cvt_pk_bf16_f32; wider
+// conversions compile to one conversion per value and a pack per pair.
+__device__ __forceinline__ unsigned bf16_pair(float x, float y) {
+    return __builtin_bit_cast(unsigned, __builtin_convertvector((f32x2){x, y}, bf16x2));
+}
+// The same pair with ReLU applied after rounding, on the bits: a negative
+// value (sign bit set, -0 included) is a negative 16-bit integer, so a signed
+// max with 0 makes it +0 and keeps every other value. For finite inputs this
+// equals rounding max(x, 0).
+__device__ __forceinline__ unsigned relu_pair(float x, float y) {
+    const i16x2 h = __builtin_bit_cast(i16x2, bf16_pair(x, y));
+    return __builtin_bit_cast(unsigned, __builtin_elementwise_max(h, (i16x2)(short)0));
+}
 
 constexpr int kIn = 96, kHid = 64, kOut = 16;
 constexpr int kQuery = 16;                       // floats per query record
@@ -103,32 +121,52 @@ __device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
-// The 24 features lane group g holds for one sample, as three 8-feature
+// The 24 features lane group g (0..3) holds for one sample, as three 8-feature
 // chunks: chunk c holds features 32c + 8g .. 32c + 8g + 7. A chunk below 72
-// is four (sin, cos) pairs of consecutive frequencies of one coordinate; one
-// sincospi gives the first pair and the double-angle identities
-// sin 2a = 2 sin a cos a, cos 2a = cos^2 a - sin^2 a the next three
-// (three doublings add a few float ulps, far below bf16 rounding).
-__device__ __forceinline__ void lane_features(const float* q, const float* pn, int g, bf16x8 x[3]) {
-    for (int c = 0; c < 3; ++c) {
-        const int fb = 32 * c + 8 * g;
-        if (fb < 72) {
-            const int d = fb / 24, fr = (fb % 24) >> 1;
-            float sn, cs;
-            sincospif(pn[d] * (float)(1 << fr), &sn, &cs);
-            for (int k = 0; k < 4; ++k) {
-                x[c][2 * k] = (__bf16)sn;
-                x[c][2 * k + 1] = (__bf16)cs;
-                float s2 = 2.0f * sn * cs, c2 = cs * cs - sn * sn;
-                sn = s2;
-                cs = c2;
-            }
-        } else {
-            for (int e = 0; e < 8; ++e) {
-                const int f = fb + e;
-                x[c][e] = (__bf16)(f < 83 ? q[3 + (f - 72)] : 0.0f);  // dir(2), normal(2), roughness, kd(3), ks(3)
-            }
-        }
+// is four (sin, cos) pairs of consecutive frequencies of one coordinate; the
+// first pair is evaluated directly and the double-angle identities
+// sin 2a = 2 sin a cos a, cos 2a = cos^2 a - sin^2 a give the next three
+// (three doublings multiply its error by at most 8, far below bf16 rounding).
+// The first pair: sin(pi x) and cos(pi x) for x = p 2^fr (0 <= p <= 1,
+// fr <= 8) are those of x / 2 revolutions, reduced exactly to [0, 1) and
+// evaluated by v_sin_f32 and v_cos_f32, whose argument is in revolutions
+// (measured on gfx950 over that range: at most 1.25e-7 from the double
+// result, against 5e-8 for sincospif, at a tenth of its instructions).
+__device__ __forceinline__ void freq_chunk(float p, int fr, bf16x8& x) {
+    const float r = __builtin_amdgcn_fractf(0.5f * p * (float)(1 << fr));
+    float sn = __builtin_amdgcn_sinf(r), cs = __builtin_amdgcn_cosf(r);
+    u32x4 u;
+    for (int k = 0; k < 4; ++k) {
+        u[k] = bf16_pair(sn, cs);
+        const float s2 = 2.0f * sn * cs, c2 = cs * cs - sn * sn;
+        sn = s2;
+        cs = c2;
+    }
+    x = __builtin_bit_cast(bf16x8, u);
+}
+
+// Chunks 0 and 1 (features 0..63) are frequency chunks for every g; chunk 2 is
+// one for g = 0 (features 64..71: coordinate 2 from frequency 8) and holds the
+// query's attributes for g = 1 (features 72..79: direction, normal, roughness,
+// kd) and g = 2 (80..82: ks, then zeros), zeros for g = 3. Every index into q
+// is a constant: an index that depends on the lane would compile to a compare
+// and a select over all 16 entries of q for each feature.
+// Chunk 0 (features 8g..) is coordinate 0 from frequency 4g for g < 3 and
+// coordinate 1 from frequency 0 for g = 3; chunk 1 (features 32 + 8g..) is
+// coordinate 1 from frequency 4 + 4g for g < 2 and coordinate 2 from frequency
+// 4(g - 2) for g >= 2. The normalised coordinates p0, p1, p2 come as scalars:
+// picking one from an array by a lane-dependent index puts the array in
+// scratch memory.
+__device__ __forceinline__ void lane_features(const float* q, float p0, float p1, float p2, int g, bf16x8 x[3]) {
+    freq_chunk(g == 3 ? p1 : p0, g == 3 ? 0 : 4 * g, x[0]);
+    freq_chunk(g < 2 ? p1 : p2, g < 2 ? 4 + 4 * g : 4 * (g - 2), x[1]);
+    if (g == 0) {
+        freq_chunk(p2, 8, x[2]);
+    } else {
+        float v[8];
+        for (int e = 0; e < 8; ++e) v[e] = g == 1 ? q[3 + e] : (g == 2 && e < 3) ? q[11 + e] : 0.0f;
+        const u32x4 u = {bf16_pair(v[0], v[1]), bf16_pair(v[2], v[3]), bf16_pair(v[4], v[5]), bf16_pair(v[6], v[7])};
+        x[2] = __builtin_bit_cast(bf16x8, u);
     }
 }
 
@@ -142,13 +180,10 @@ __device__ __forceinline__ void load_query(const float* queries, int64_t s, cons
     for (int k = 0; k < 3; ++k) pn[k] = fminf(fmaxf((q[k] - bd.lo[k]) * bd.inv[k], 0.f), 1.f);
 }
 
+// ReLU of two D tiles' values as one B chunk of bf16.
 __device__ __forceinline__ bf16x8 relu_pack(f32x4 a, f32x4 b) {
-    bf16x8 r;
-    r[0] = (__bf16)fmaxf(a[0], 0.f); r[1] = (__bf16)fmaxf(a[1], 0.f);
-    r[2] = (__bf16)fmaxf(a[2], 0.f); r[3] = (__bf16)fmaxf(a[3], 0.f);
-    r[4] = (__bf16)fmaxf(b[0], 0.f); r[5] = (__bf16)fmaxf(b[1], 0.f);
-    r[6] = (__bf16)fmaxf(b[2], 0.f); r[7] = (__bf16)fmaxf(b[3], 0.f);
-    return r;
+    const u32x4 u = {relu_pair(a[0], a[1]), relu_pair(a[2], a[3]), relu_pair(b[0], b[1]), relu_pair(b[2], b[3])};
+    return __builtin_bit_cast(bf16x8, u);
 }
 
 // Makes `off` depend on `dep` in a way the compiler cannot see through. Weight
@@ -173,17 +208,22 @@ struct InferArgs {
     float* out;  // 4 floats per query (rgb, 0)
 };
 
-constexpr int kTiles = 2;  // sample tiles of 16 per wave iteration in inference
+// Inference: two sample tiles of 16 per wave iteration and eight waves per
+// workgroup sharing one copy of the weights in LDS (80 VGPRs: six waves per
+// SIMD). Four tiles with four waves (134 VGPRs, three waves per SIMD) took 8%
+// longer, three tiles with eight waves 5%.
+constexpr int kTiles = 2;
+constexpr int kInferWaves = 8;
 
-__global__ void __launch_bounds__(256) k_infer(InferArgs a) {
+__global__ void __launch_bounds__(64 * kInferWaves) k_infer(InferArgs a) {
     __shared__ __bf16 w[kFwdElems];
     lds_copy(w, a.fwd, kFwdElems);
     __syncthreads();
     const int lane = (int)__lane_id(), g = lane >> 4, m = lane & 15;
     const int wave = threadIdx.x >> 6;
     const int64_t n = a.n_dev ? (int64_t)min((unsigned long long)a.n, *a.n_dev) : a.n;
-    const int64_t step = (int64_t)gridDim.x * 4 * 16 * kTiles;
-    for (int64_t base = ((int64_t)blockIdx.x * 4 + wave) * 16 * kTiles; base < n; base += step) {
+    const int64_t step = (int64_t)gridDim.x * kInferWaves * 16 * kTiles;
+    for (int64_t base = ((int64_t)blockIdx.x * kInferWaves + wave) * 16 * kTiles; base < n; base += step) {
         // An opaque per-iteration copy of the lane's fragment offset keeps the
         // compiler from hoisting all 46 weight fragments out of the loop into
         // registers (184 VGPRs, one wave per SIMD); they are re-read from LDS.
@@ -194,7 +234,7 @@ __global__ void __launch_bounds__(256) k_infer(InferArgs a) {
             int64_t smp = min(base + 16 * s + m, n - 1);
             float q[kQuery], pn[3];
             load_query(a.queries, smp, a.bd, q, pn);
-            lane_features(q, pn, g, x[s]);
+            lane_features(q, pn[0], pn[1], pn[2], g, x[s]);
         }
         f32x4 acc[4][kTiles];
         // layer 0: 96 -> 64
@@ -278,7 +318,7 @@ __global__ void __launch_bounds__(kFbBlock) k_train_fb(TrainArgs a) {
         float q[kQuery], pn[3];
         load_query(a.queries, sl, a.bd, q, pn);
         bf16x8 x[3];
-        lane_features(q, pn, g, x);
+        lane_features(q, pn[0], pn[1], pn[2], g, x);
         // Stores address a uniform column base plus one 32-bit lane offset
         // (the lane's sample, shifted by its group's 4 or 8 features), so
         // that they need no 64-bit address per value.
@@ -299,6 +339,7 @@ __global__ void __launch_bounds__(kFbBlock) k_train_fb(TrainArgs a) {
             for (int c = 0; c < 3; ++c)
                 acc[t] = mfma32(*reinterpret_cast<const bf16x8*>(wf + kF0 + (t * 3 + c) * 512 + lo8), x[c], acc[t]);
         }
+#pragma unroll
         for (int l = 1; l <= 5; ++l) {
             if (l > 1) {
                 for (int t = 0; t < 4; ++t) {
@@ -343,14 +384,16 @@ __global__ void __launch_bounds__(kFbBlock) k_train_fb(TrainArgs a) {
         order_after(lo4, (float)g6[0]);
         for (int t = 0; t < 4; ++t)
             d[t] = mfma16(*reinterpret_cast<const bf16x4*>(wb + kB5 + t * 256 + lo4), g6, f32x4{0.f, 0.f, 0.f, 0.f});
+#pragma unroll
         for (int l = 5; l >= 1; --l) {
             // G_l = dH_l * relu'(H_l), back in B-chunk form
             bf16x8 gl[2];
-            for (int c = 0; c < 2; ++c)
-                for (int e = 0; e < 8; ++e) {
-                    float v = d[2 * c + (e >> 2)][e & 3];
-                    gl[c][e] = (__bf16)(((relu_on[l] >> (8 * c + e)) & 1u) ? v : 0.f);
-                }
+            for (int c = 0; c < 2; ++c) {
+                float v[8];
+                for (int e = 0; e < 8; ++e) v[e] = ((relu_on[l] >> (8 * c + e)) & 1u) ? d[2 * c + (e >> 2)][e & 3] : 0.f;
+                const u32x4 u = {bf16_pair(v[0], v[1]), bf16_pair(v[2], v[3]), bf16_pair(v[4], v[5]), bf16_pair(v[6], v[7])};
+                gl[c] = __builtin_bit_cast(bf16x8, u);
+            }
             if (valid)
                 for (int c = 0; c < 2; ++c)
                     for (int e = 0; e < 8; ++e) (a.G + goff(l, ld) + (int64_t)perm(c, 0, e) * ld)[off4] = gl[c][e];
@@ -390,7 +433,7 @@ __device__ __forceinline__ void strip_of(int strip, int& layer, int& to, int& nt
     layer = 5; to = 0; ntiles = 4;
 }
 
-__global__ void __launch_bounds__(256) k_dw(DwArgs a) {
+__global__ void __launch_bounds__(256, 3) k_dw(DwArgs a) {
     const int lane = (int)__lane_id(), g = lane >> 4, m = lane & 15;
     const int wave = threadIdx.x >> 6;
     const int strip = (int)blockIdx.y * 4 + wave;
@@ -406,7 +449,28 @@ __global__ void __launch_bounds__(256) k_dw(DwArgs a) {
     f32x4 acc[6];
 #pragma unroll
     for (int t = 0; t < 6; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int64_t s = s0; s < s1; s += 32) {
+    // Three steps of 32 samples at a time: their loads are all issued before
+    // the first product, so a wave waits for memory once per 96 samples (four
+    // steps need more than the 168 registers of three waves per SIMD).
+    constexpr int kSteps = 3;
+    int64_t s = s0;
+    for (; s + 32 * kSteps <= s1; s += 32 * kSteps) {
+        bf16x8 A[kSteps], B[kSteps][6];
+#pragma unroll
+        for (int u = 0; u < kSteps; ++u) {
+            const int64_t k0 = s + 32 * u + 8 * g;
+            A[u] = *reinterpret_cast<const bf16x8*>(gA + k0);
+#pragma unroll
+            for (int t = 0; t < 6; ++t)
+                if (t < ntiles) B[u][t] = *reinterpret_cast<const bf16x8*>(hB + (int64_t)(16 * t) * ld + k0);
+        }
+#pragma unroll
+        for (int u = 0; u < kSteps; ++u)
+#pragma unroll
+            for (int t = 0; t < 6; ++t)
+                if (t < ntiles) acc[t] = mfma32(A[u], B[u][t], acc[t]);
+    }
+    for (; s < s1; s += 32) {
         const int64_t k0 = s + 8 * g;
         const bool full = k0 + 8 <= s1;
         bf16x8 A;
@@ -546,9 +610,15 @@ int nrc_infer_queue(ilm_nrc* r, const float* queries, const unsigned long long* 
     a.n_dev = n_dev;
     a.bd = r->bd;
     a.out = out;
-    int64_t per_block = 4 * 16 * nrc::kTiles;
-    int64_t blocks = std::min<int64_t>((capacity + per_block - 1) / per_block, (int64_t)r->ctx->cu_count * 4);
-    hipLaunchKernelGGL(nrc::k_infer, dim3((unsigned)blocks), dim3(256), 0, stream, a);
+    // One round of resident workgroups (the 47 KB of weights each holds in LDS
+    // allow three per CU): a larger grid runs a second, partial round after
+    // the first.
+    int per_cu = 0;
+    ILM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)nrc::k_infer, 64 * nrc::kInferWaves, 0));
+    per_cu = std::max(per_cu, 1);
+    int64_t per_block = nrc::kInferWaves * 16 * nrc::kTiles;
+    int64_t blocks = std::min<int64_t>((capacity + per_block - 1) / per_block, (int64_t)r->ctx->cu_count * per_cu);
+    hipLaunchKernelGGL(nrc::k_infer, dim3((unsigned)blocks), dim3(64 * nrc::kInferWaves), 0, stream, a);
     ILM_HIP(hipGetLastError());
     return ILM_OK;
 }
@@ -743,7 +813,8 @@ int ilm_nrc_train_step(ilm_nrc* r, const float* queries, const float* targets, i
     t.ld = (n + 31) / 32 * 32;
     t.loss = loss;
     t.grad_scale = 1.f / (3.f * (float)n);
-    int64_t fb_blocks = std::min<int64_t>((n + 63) / 64, (int64_t)r->ctx->cu_count * 2);
+    const int64_t fb_per_block = kFbBlock / 64 * 16;  // 16 samples per wave
+    int64_t fb_blocks = std::min<int64_t>((n + fb_per_block - 1) / fb_per_block, (int64_t)r->ctx->cu_count * 2);
     hipLaunchKernelGGL(k_train_fb, dim3((unsigned)fb_blocks), dim3(kFbBlock), 0, st, t);
     DwArgs d;
     d.H = r->H;
