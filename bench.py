@@ -87,7 +87,7 @@ def node_format(ilm, args):
 
 def trace_kernel_name(nodes):
     """The closest-hit traversal kernel that traces float or quantised BVH4 nodes."""
-    return f"k_trace4<false, false, {1 if nodes == 'quantized' else 0}, false>"
+    return f"k_trace4<false, false, {1 if nodes == 'quantized' else 0}, false, false>"
 
 
 def traffic_from_profiles(nodes):
@@ -999,7 +999,7 @@ def main():
             "traffic": traffic,
             "traffic_source": traffic_note,
             "traffic_frac": round(traffic / (inc["kernel_ms"] * 1e-3) / (HBM_PEAK_GBS * 1e9), 4) if traffic else None,
-            "kernel": f"ilm::k_trace4<false, false, {1 if args.nodes == 'quantized' else {4: 0, 2: 2, 8: 3}[args.width]}, false>",
+            "kernel": f"ilm::k_trace4<false, false, {1 if args.nodes == 'quantized' else {4: 0, 2: 2, 8: 3}[args.width]}, false, false>",
             "algorithmic_bytes_per_launch": int(inc["hbm_bytes"]),
             "algorithmic_bytes": "32 B read per ray + 16 B written per hit + tree and triangle arrays once "
                                  f"({tree_bytes} B)",

@@ -186,9 +186,13 @@ __device__ __forceinline__ void node_keys(const float4* __restrict__ nodes, uint
 template <int W> constexpr int kKeep = kLds - (W - 1);
 
 // Speculation thresholds: the node loop ends once kSpecNum / kSpecDen of its
-// lanes hold a parked leaf; the leaf loop ends once fewer than kLeafNum /
-// kLeafDen of the wave's active lanes hold triangles (and a lane can descend).
-constexpr int kSpecNum = 1, kSpecDen = 4;
+// lanes hold a parked leaf (three eighths in the instance walk, whose mesh
+// descents are short and re-enter the node loop after each instance, and in
+// dual launches, whose shadow segments end early); the leaf loop ends once
+// fewer than kLeafNum / kLeafDen of the wave's active lanes hold triangles (and
+// a lane can descend).
+template <bool INST, bool DUAL> constexpr int kSpecNum = (INST || DUAL) ? 3 : 1;
+template <bool INST, bool DUAL> constexpr int kSpecDen = (INST || DUAL) ? 8 : 4;
 constexpr int kLeafNum = 1, kLeafDen = 4;
 static_assert((kLds & (kLds - 1)) == 0, "the LDS ring is indexed by position & (kLds - 1)");
 
@@ -353,7 +357,9 @@ __global__ void __launch_bounds__(kBlock, kTraceWaves<KIND>) k_trace4(TraceArgs 
         // until kSpecNum / kSpecDen of the lanes in the loop hold a parked leaf
         // (a quarter measured best with the leaf loop below and quantised
         // nodes: 1/8, 3/16, 3/8 and 1/2 were 2-5% slower on incoherent rays,
-        // three quarters 9% slower with float nodes)
+        // three quarters 9% slower with float nodes; the instance walk was 14%
+        // faster at 3/8 than at 1/4, the path tracer's dual launches 1.5%,
+        // and as fast as at 1/2)
         while (can_step()) {
             if (INST && in_blas && cur == kReturn) {
                 // back to the world-space ray, keeping the window its hits shrank
@@ -424,7 +430,7 @@ __global__ void __launch_bounds__(kBlock, kTraceWaves<KIND>) k_trace4(TraceArgs 
                 stash = cur;
                 cur = pop();
             }
-            if (__popcll(__ballot(stash != kNoNode)) * kSpecDen >= __popcll(__ballot(true)) * kSpecNum) break;
+            if (__popcll(__ballot(stash != kNoNode)) * kSpecDen<INST, DUAL> >= __popcll(__ballot(true)) * kSpecNum<INST, DUAL>) break;
         }
         // parked leaves, one triangle per lane per iteration: a lane moves on
         // to the next leaf on its stack as soon as its current one is done, so
