@@ -325,11 +325,11 @@ def large_scene_leg(ctx, ilm, workloads, args):
     W, H = 3840, 2160
     n = W * H
     out = {"workload": f"{len(idx)} tris, {W}x{H} camera rays and their cosine bounces", "triangles": int(len(idx)),
-           "rays_per_set": n, "generate_s": round(gen_s, 1)}
+           "rays_per_set": n, "generate_s": round(gen_s, 1), "nodes": args.nodes}
     light = np.array([-0.5, 4.0, -0.5, 1.0, 0.0, 0.0, 0.0, 0.0, 1.0], np.float32)
     for name, builder in (("lbvh", ilm.ILM_BUILDER_LBVH), ("sah", ilm.ILM_BUILDER_SAH)):
         m = ilm.Mesh(ctx, pos, idx)
-        info = m.build(builder=builder)
+        info = m.build(builder=builder, node_format=node_format(ilm, args))
         rs = workloads.intersection_rays(ctx, m, W, H, cam=workloads.large_camera(W / H), light=light)
         hits = ctx.buffer(n * HIT_BYTES)
         steps = max(3, args.steps // 4)
@@ -402,7 +402,9 @@ def large_render_leg(ctx, ilm, workloads, args):
     a 48-triangle room with three panel lights; 30% glossy), eight 3840 x 2160
     cameras, depth 12, rendered as one batch (one scene, eight views): plain
     frames and neural-cache frames (2-bounce prefix, 10-bounce training
-    suffix), 1 spp per view per frame (a 64-spp frame is 64 of them). The
+    suffix), 1 spp per view per frame (a 64-spp frame is 64 of them), with
+    --large-batch paths per wavefront batch (default 2^26: the frame's 66 M
+    paths at once) and the --nodes tree format. The
     plain frames' per-view means check the cache frames' (after `skip`
     training frames). HBM footprint = device memory in use after the frames
     minus before the scene was created."""
@@ -414,6 +416,7 @@ def large_render_leg(ctx, ilm, workloads, args):
     info = m.build(builder=ilm.ILM_BUILDER_LBVH)
     sc = ilm.Scene(ctx, m, nrm, tm, tl, mats, lights)
     b = ilm.Batch(ctx, [sc])
+    b.configure(node_format=node_format(ilm, args), batch_paths=args.large_batch)
     sc.close()
     m.close()
     del pos, nrm, idx, tm, tl
@@ -451,7 +454,8 @@ def large_render_leg(ctx, ilm, workloads, args):
 
     out = {"workload": f"{info['triangles'] if 'triangles' in info else 11085808} tris, {nv} cameras "
                        f"{W}x{H}, depth {depth}, 1 spp per view per frame, one batch",
-           "generate_s": round(gen_s, 1), "lbvh_build_ms": round(info["build_ms"], 1),
+           "generate_s": round(gen_s, 1), "lbvh_build_ms": round(info["build_ms"], 1), "nodes": args.nodes,
+           "batch_paths": args.large_batch or 2**22,
            "hbm_footprint_gib": round((free0 - free1) / 2**30, 2), "hbm_total_gib": round(total / 2**30, 1),
            "plain": summary(plain), "neural_cache": summary(cache[skip:]),
            "cache_training_frames": skip, "cache_train_steps_per_frame": args.large_train_steps,
@@ -777,6 +781,9 @@ def main():
     ap.add_argument("--large-frames", type=int, default=2, help="timed frames per renderer (large render leg)")
     ap.add_argument("--large-skip", type=int, default=8, help="cache training frames before timing (large render)")
     ap.add_argument("--large-train-steps", type=int, default=8, help="Adam steps per cache frame (large render)")
+    ap.add_argument("--large-batch", type=int, default=1 << 26,
+                    help="paths per wavefront batch of the large render (0: the library default, 2^22); the "
+                         "default holds a frame's 66 M paths in one batch (30.6 GiB in all)")
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--batch-spp", type=int, default=2)
     ap.add_argument("--refill", type=int, default=0,

@@ -1563,7 +1563,8 @@ int ilm_batch_autotune(ilm_batch* b, const ilm_batch_view* bv, int nviews, const
     if (rc) return rc;
     // trees: float BVH4, quantised BVH4, float BVH2, float BVH8
     const int fmts[4][2] = {{ILM_NODES_FLOAT, 4}, {ILM_NODES_QUANTIZED, 4}, {ILM_NODES_FLOAT, 2}, {ILM_NODES_FLOAT, 8}};
-    const int64_t sizes[3] = {int64_t(1) << 20, int64_t(1) << 22, int64_t(1) << 24};
+    const int64_t sizes[4] = {int64_t(1) << 20, int64_t(1) << 22, int64_t(1) << 24, int64_t(1) << 26};
+    static_assert(4 * 4 <= 16, "the tuning table holds 16 candidates");
     ilm_batch_tuning t{};
     t.best_ms = 1e30;
     int k = 0;

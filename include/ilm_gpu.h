@@ -590,8 +590,8 @@ typedef struct {
 
 /* Times frames of the given views (at most 2 spp) for each tree (float
  * BVH4, quantised BVH4, float BVH2, float BVH8) and batch size (2^20, 2^22,
- * 2^24 paths), keeps the fastest for later frames and reports the table
- * (out may be NULL). */
+ * 2^24, 2^26 paths: the wavefront state of 2^26 paths takes ~27 GiB), keeps
+ * the fastest for later frames and reports the table (out may be NULL). */
 int ilm_batch_autotune(ilm_batch* batch, const ilm_batch_view* views, int nviews, const ilm_render_params* params,
                        ilm_batch_tuning* out);
 

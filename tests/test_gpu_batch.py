@@ -101,7 +101,8 @@ def test_batch_views_match_single_scenes(gpu_ctx, ilm):
     # sizes and keeps the fastest
     t = b.autotune(views)
     print(f"[batch autotune] {t}", flush=True)
-    assert len(t["table"]) == 12 and t["best_ms"] == min(ms for *_, ms in t["table"])
+    assert len(t["table"]) == 16 and t["best_ms"] == min(ms for *_, ms in t["table"])
+    assert {bp for _, _, bp, _ in t["table"]} == {1 << 20, 1 << 22, 1 << 24, 1 << 26}
     assert {(f, wd) for f, wd, _, _ in t["table"]} == {(ilm.ILM_NODES_FLOAT, 4), (ilm.ILM_NODES_QUANTIZED, 4),
                                                        (ilm.ILM_NODES_FLOAT, 2), (ilm.ILM_NODES_FLOAT, 8)}
     acc.zero()
