@@ -198,8 +198,11 @@ static_assert((kLds & (kLds - 1)) == 0, "the LDS ring is indexed by position & (
 
 // Waves per SIMD the traversal kernels are compiled for: seven (72 VGPRs)
 // measured 4% faster than the six that the unconstrained register count
-// allows; BVH8 nodes need ~120 VGPRs and keep the four those allow.
-template <int K> constexpr int kTraceWaves = K == kFloat8 ? 4 : 7;
+// allows; BVH8 nodes need ~120 VGPRs and keep the four those allow. The
+// statistics variant keeps six (80 VGPRs), where it needs no scratch memory.
+// The two-level walk stays at seven although it spills 44 bytes per lane
+// there: at six, without the spill, it traced 3% fewer incoherent rays.
+template <int K, bool WIDE> constexpr int kTraceWaves = K == kFloat8 ? 4 : WIDE ? 6 : 7;
 
 // Stack marker of the two-level traversal: back to the world-space ray.
 constexpr uint32_t kReturn = 0x7FFFFFFEu;
@@ -212,7 +215,7 @@ constexpr uint32_t kReturn = 0x7FFFFFFEu;
 // one stops at kReturn until the leaf's triangles are tested with the
 // object-space ray.
 template <bool ANY, bool STATS, int KIND, bool DUAL, bool INST = false>
-__global__ void __launch_bounds__(kBlock, kTraceWaves<KIND>) k_trace4(TraceArgs a) {
+__global__ void __launch_bounds__(kBlock, (kTraceWaves<KIND, STATS>)) k_trace4(TraceArgs a) {
     static_assert(!INST || (KIND == kFloat4 && !DUAL && !STATS), "instance tables hold float BVH4 trees");
     constexpr int W = Kind<KIND>::W;
     // per lane: a kLds-entry stack ring (lane-strided, conflict-free) and W
@@ -252,6 +255,7 @@ __global__ void __launch_bounds__(kBlock, kTraceWaves<KIND>) k_trace4(TraceArgs 
     int32_t best_prim = -1;
     int32_t n_nodes = 0, n_tris = 0;
     bool in_blas = false;              // INST: the lane is inside an instance's mesh tree
+    bool world_stale = false;          // INST: r still holds an instance's object-space set-up
     int32_t inst_id = -1, best_inst = -1;
 
     auto pop = [&]() -> uint32_t {
@@ -346,6 +350,7 @@ __global__ void __launch_bounds__(kBlock, kTraceWaves<KIND>) k_trace4(TraceArgs 
                     best_t = best_u = best_v = 0.f;
                     n_nodes = n_tris = 0;
                     in_blas = false;
+                    world_stale = false;
                     inst_id = best_inst = -1;
                     active = true;
                 }
@@ -362,13 +367,11 @@ __global__ void __launch_bounds__(kBlock, kTraceWaves<KIND>) k_trace4(TraceArgs 
         // and as fast as at 1/2)
         while (can_step()) {
             if (INST && in_blas && cur == kReturn) {
-                // back to the world-space ray, keeping the window its hits shrank
-                const float4* rp = reinterpret_cast<const float4*>(a.rays + ray_id);
-                const float4 o4 = rp[0], d4 = rp[1];
-                const float thi = r.thi;
-                ray_setup(r, mk(o4.x, o4.y, o4.z), mk(d4.x, d4.y, d4.z), d4.w);
-                r.thi = fminf(thi, r.thi);
+                // back in the top-level tree; the world-space set-up waits for
+                // its next node (the next instance entry, often the next entry
+                // on the stack, sets up its own ray from the stored one)
                 in_blas = false;
+                world_stale = true;
                 cur = pop();
                 continue;
             }
@@ -399,6 +402,15 @@ __global__ void __launch_bounds__(kBlock, kTraceWaves<KIND>) k_trace4(TraceArgs 
                 inst_id = meta.y;
                 cur = (uint32_t)meta.x;
                 continue;
+            }
+            if (INST && !in_blas && world_stale) {
+                // the world-space ray, keeping the window its hits shrank
+                const float4* rp = reinterpret_cast<const float4*>(a.rays + ray_id);
+                const float4 o4 = rp[0], d4 = rp[1];
+                const float thi = r.thi;
+                ray_setup(r, mk(o4.x, o4.y, o4.z), mk(d4.x, d4.y, d4.z), d4.w);
+                r.thi = fminf(thi, r.thi);
+                world_stale = false;
             }
             if (STATS) ++n_nodes;
             uint32_t key[W];
