@@ -537,6 +537,43 @@ def _corrected(img, ref, noise2):
     return float(np.sqrt(max(e2 - noise2, 0.0))), float(np.sqrt(e2))
 
 
+class _TargetSearch:
+    """Progressive frames checked against the error target at growing
+    intervals: the host reads the image only at those checks, so the frames
+    in between run back to back (a read after every frame idles the GPU
+    while the host computes the error, and the next frame starts slower).
+    Those frames are kept as device snapshots of the accumulated image, so a
+    check that meets the target is traced back to the first frame that met
+    it, whose cumulative time is the renderer's time to the target."""
+
+    def __init__(self, ctx, buf, nbytes, err):
+        self.ctx, self.buf, self.nbytes, self.err = ctx, buf, nbytes, err
+        self.snaps, self.pending = [], []
+
+    def keep(self, frame, ms, divisor):
+        j = len(self.pending)
+        if j == len(self.snaps):
+            self.snaps.append(self.ctx.buffer(self.nbytes))
+        self.ctx.copy(self.snaps[j], self.buf, self.nbytes)
+        self.pending.append((frame, ms, divisor, j))
+
+    def first_hit(self, target):
+        """(frame, ms, error) of the first kept frame at the target, or None."""
+        for frame, ms, divisor, j in self.pending:
+            e = self.err(self.snaps[j], divisor)
+            if e <= target:
+                return frame, ms, e
+        return None
+
+    def clear(self):
+        self.pending = []
+
+    def free(self):
+        for b in self.snaps:
+            b.free()
+        self.snaps = []
+
+
 def cache_convergence(ctx, ilm, workloads, args):
     """Time to relRMSE 0.05 on the radiance-cache workload (displaced level-8
     sphere, 30% glossy, in a closed room with three panel lights), plain
@@ -572,10 +609,15 @@ def cache_convergence(ctx, ilm, workloads, args):
         f"ref noise relRMSE {np.sqrt(noise2):.4f}, {ms_ref / args.cache_ref_spp:.2f} ms/spp")
 
     target = args.cache_target
-    # Plain PT: progressive 1-spp frames. The error is checked after every
-    # frame (on the host, outside the timed renders), so the target is met at
-    # the first frame that reaches it; the trace keeps every frame up to 8
-    # and then every quarter more, plus the frame that reached the target.
+
+    def err(b, divisor):
+        return _corrected(b.download(np.float64, n3).reshape(H, W, 3) / divisor, ref, noise2)[0]
+
+    search = _TargetSearch(ctx, buf, n3 * 8, err)
+    # Plain PT: progressive 1-spp frames, the error checked at every frame up
+    # to 8 and then every quarter more; the first frame at the target is
+    # found among the snapshots since the previous check. The trace holds
+    # the checks and that frame.
     buf.zero()
     plain_ms, spp, trace = 0.0, 0, []
     check = 1
@@ -586,14 +628,19 @@ def cache_convergence(ctx, ilm, workloads, args):
         plain_ms += st["ms"]
         plain_rays += st["camera_rays"] + st["extension_rays"] + st["shadow_rays"]
         spp += 1
-        e, raw = _corrected(buf.download(np.float64, n3).reshape(H, W, 3) / spp, ref, noise2)
-        if spp == check or e <= target:
-            trace.append((spp, round(plain_ms, 2), round(e, 5)))
+        if spp != check:
+            search.keep(spp, plain_ms, spp)
+            continue
+        e = err(buf, spp)
         if e <= target:
-            plain_hit = (spp, plain_ms)
+            first = search.first_hit(target) or (spp, plain_ms, e)
+            trace.append((first[0], round(first[1], 2), round(first[2], 5)))
+            plain_hit = (first[0], first[1])
             break
-        if spp == check:
-            check = spp * 2 if spp < 8 else spp + max(1, spp // 4)
+        trace.append((spp, round(plain_ms, 2), round(e, 5)))
+        search.clear()
+        check = spp * 2 if spp < 8 else spp + max(1, spp // 4)
+    search.clear()
     # 1/sqrt(spp) fit of the last point: spp needed for the target.
     s_last, ms_last, e_last = trace[-1]
     spp_fit = s_last * (e_last / target) ** 2
@@ -623,13 +670,19 @@ def cache_convergence(ctx, ilm, workloads, args):
             frames += 1
             if frames > skip:
                 acc_frames += 1
-                # checked after every frame, like the plain frames
-                e, raw = _corrected(buf.download(np.float64, n3).reshape(H, W, 3) / acc_frames, ref, noise2)
-                if acc_frames in (1, 2, 4) or acc_frames % 4 == 0 or e <= target:
-                    ntrace.append((frames, round(nrc_ms, 2), round(e, 5)))
+                # checked like the plain frames, at 1, 2, 4 and every 4th frame
+                if not (acc_frames in (1, 2, 4) or acc_frames % 4 == 0):
+                    search.keep(frames, nrc_ms, acc_frames)
+                    continue
+                e = err(buf, acc_frames)
                 if e <= target:
-                    nrc_hit = (frames, nrc_ms)
+                    first = search.first_hit(target) or (frames, nrc_ms, e)
+                    ntrace.append((first[0], round(first[1], 2), round(first[2], 5)))
+                    nrc_hit = (first[0], first[1])
                     break
+                ntrace.append((frames, round(nrc_ms, 2), round(e, 5)))
+                search.clear()
+        search.clear()
         out = {"adam_lr": lr, "train_fraction": round(frac, 6), "train_records_per_frame": int(records),
                "train_batch": 65536, "train_steps_per_frame": steps, "skip_frames": skip,
                "ms_per_frame": round(nrc_ms / frames, 3), "frames_to_target": nrc_hit[0] if nrc_hit else None,
@@ -661,15 +714,24 @@ def cache_convergence(ctx, ilm, workloads, args):
             st = ic.render(cam, W, H, 1, buf, seed=30001 + f, spp_offset=f)
             ic_ms += st["ms"]
             f += 1
-            e, _ = _corrected(buf.download(np.float64, n3).reshape(H, W, 3) / f, ref, noise2)
+            if f != check:
+                search.keep(f, ic_ms, f)
+                continue
+            e = err(buf, f)
             best_ic = min(best_ic, e)
-            if f == check or e <= target:
-                itrace.append((f, round(ic_ms, 2), round(e, 5), st["records_total"]))
             if e <= target:
-                ic_hit = (f, ic_ms)
+                first = search.first_hit(target) or (f, ic_ms, e)
+                itrace.append((first[0], round(first[1], 2), round(first[2], 5), st["records_total"]))
+                ic_hit = (first[0], first[1])
                 break
-            if f == check:
-                check = f * 2 if f < 8 else f + max(1, f // 4)
+            itrace.append((f, round(ic_ms, 2), round(e, 5), st["records_total"]))
+            search.clear()
+            check = f * 2 if f < 8 else f + max(1, f // 4)
+        if ic_hit is None:
+            # the frames after the last check, for the best error reached
+            for frame, ms, divisor, j in search.pending:
+                best_ic = min(best_ic, err(search.snaps[j], divisor))
+        search.clear()
         ic_out = {"error_threshold": args.ic_threshold, "ms_per_frame": round(ic_ms / f, 3),
                   "frames_to_target": ic_hit[0] if ic_hit else None,
                   "ms_to_target": round(ic_hit[1], 1) if ic_hit else None,
@@ -714,6 +776,7 @@ def cache_convergence(ctx, ilm, workloads, args):
                           "flop_per_sample": 3 * MLP_FLOP_PER_QUERY,
                           "roofline_frac": round(tr_tflops / MFMA_BF16_PEAK_TFLOPS, 4)}}
     log(f"cache mlp inference: {mlp}")
+    search.free()
     for b in (qd, od, td, buf):
         b.free()
     net.close()
