@@ -186,13 +186,14 @@ __device__ __forceinline__ void node_keys(const float4* __restrict__ nodes, uint
 template <int W> constexpr int kKeep = kLds - (W - 1);
 
 // Speculation thresholds: the node loop ends once kSpecNum / kSpecDen of its
-// lanes hold a parked leaf (three eighths in the instance walk, whose mesh
-// descents are short and re-enter the node loop after each instance, and in
-// dual launches, whose shadow segments end early); the leaf loop ends once
-// fewer than kLeafNum / kLeafDen of the wave's active lanes hold triangles (and
-// a lane can descend).
-template <bool INST, bool DUAL> constexpr int kSpecNum = (INST || DUAL) ? 3 : 1;
-template <bool INST, bool DUAL> constexpr int kSpecDen = (INST || DUAL) ? 8 : 4;
+// lanes hold a parked leaf (a quarter for closest hits; three eighths in the
+// instance walk, whose mesh descents are short and re-enter the node loop
+// after each instance, in dual launches, whose shadow segments end early, and
+// in any-hit launches); the leaf loop ends once fewer than kLeafNum / kLeafDen
+// of the wave's active lanes hold triangles (and a lane can descend).
+template <bool ANY, bool INST, bool DUAL> constexpr bool kSpecWide = ANY || INST || DUAL;
+template <bool ANY, bool INST, bool DUAL> constexpr int kSpecNum = kSpecWide<ANY, INST, DUAL> ? 3 : 1;
+template <bool ANY, bool INST, bool DUAL> constexpr int kSpecDen = kSpecWide<ANY, INST, DUAL> ? 8 : 4;
 constexpr int kLeafNum = 1, kLeafDen = 4;
 static_assert((kLds & (kLds - 1)) == 0, "the LDS ring is indexed by position & (kLds - 1)");
 
@@ -363,8 +364,8 @@ __global__ void __launch_bounds__(kBlock, (kTraceWaves<KIND, STATS>)) k_trace4(T
         // (a quarter measured best with the leaf loop below and quantised
         // nodes: 1/8, 3/16, 3/8 and 1/2 were 2-5% slower on incoherent rays,
         // three quarters 9% slower with float nodes; the instance walk was 14%
-        // faster at 3/8 than at 1/4, the path tracer's dual launches 1.5%,
-        // and as fast as at 1/2)
+        // faster at 3/8 than at 1/4, the path tracer's dual launches 1.5%
+        // (and as fast as at 1/2), shadow rays 2% (1/8 6% slower))
         while (can_step()) {
             if (INST && in_blas && cur == kReturn) {
                 // back in the top-level tree; the world-space set-up waits for
@@ -448,7 +449,7 @@ __global__ void __launch_bounds__(kBlock, (kTraceWaves<KIND, STATS>)) k_trace4(T
                 stash = cur;
                 cur = pop();
             }
-            if (__popcll(__ballot(stash != kNoNode)) * kSpecDen<INST, DUAL> >= __popcll(__ballot(true)) * kSpecNum<INST, DUAL>) break;
+            if (__popcll(__ballot(stash != kNoNode)) * kSpecDen<ANY, INST, DUAL> >= __popcll(__ballot(true)) * kSpecNum<ANY, INST, DUAL>) break;
         }
         // parked leaves, one triangle per lane per iteration: a lane moves on
         // to the next leaf on its stack as soon as its current one is done, so
