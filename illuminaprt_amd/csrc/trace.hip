@@ -23,12 +23,14 @@
 // keys (entry distance bits with the slot in the low bits) are sorted with a
 // min/max comparator network, the child words are read back from LDS by slot,
 // and every push is written at the top and kept only when its child was hit;
-// the nearest child is visited first. Traversal is speculative (Aila and
-// Laine, "Understanding the Efficiency of Ray Traversal on GPUs", HPG 2009):
-// a lane that reaches a leaf parks it and keeps descending until a quarter of
-// the lanes in the node loop hold a leaf; the leaf loop then tests one triangle
-// per lane per iteration, each lane moving on to its next leaf on its own,
-// and hands the wave back to the node loop once few lanes hold triangles.
+// closest-hit rays visit the nearest child first, any-hit launches keep the
+// children in slot order. Traversal is speculative (Aila and Laine,
+// "Understanding the Efficiency of Ray Traversal on GPUs", HPG 2009): a lane
+// that reaches a leaf parks it and keeps descending until a quarter (closest
+// hits) or three eighths (any-hit, dual and two-level launches) of the lanes
+// in the node loop hold a leaf; the leaf loop then tests one triangle per lane
+// per iteration, each lane moving on to its next leaf on its own, and hands
+// the wave back to the node loop once few lanes hold triangles.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
