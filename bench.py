@@ -543,8 +543,10 @@ class _TargetSearch:
     in between run back to back (a read after every frame idles the GPU
     while the host computes the error, and the next frame starts slower).
     Those frames are kept as device snapshots of the accumulated image, so a
-    check that meets the target is traced back to the first frame that met
-    it, whose cumulative time is the renderer's time to the target."""
+    check that meets the target is traced back to the first frame since the
+    previous check that met it, whose cumulative time is the renderer's time
+    to the target (the same frame a check after every frame finds while the
+    error falls from frame to frame)."""
 
     def __init__(self, ctx, buf, nbytes, err):
         self.ctx, self.buf, self.nbytes, self.err = ctx, buf, nbytes, err
