@@ -679,6 +679,11 @@ struct ilm_scene {
     // launches) is unchanged
     bool use_graph = false;
     hipGraphExec_t graph = nullptr;
+    // cache frames: the cache inference at the query vertices runs on a side
+    // stream (default priority) while the training suffix traces on the
+    // context's stream; created on first use
+    hipStream_t side = nullptr;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
     std::vector<int64_t> graph_key;
     // host copies of the shading arrays (merged by batches)
     std::vector<float> h_shade, h_mats, h_lt, h_lrec;
@@ -879,6 +884,9 @@ int ilm_scene_destroy(ilm_scene* s) {
                     (void*)s->d_frame, (void*)s->d_views})
         if (p) ilm_free(s->ctx, p);
     if (s->owns_mesh) ilm_mesh_destroy(s->mesh);
+    if (s->fork_ev) (void)hipEventDestroy(s->fork_ev);
+    if (s->join_ev) (void)hipEventDestroy(s->join_ev);
+    if (s->side) (void)hipStreamDestroy(s->side);
     delete s;
     return ILM_OK;
 }
@@ -1007,6 +1015,12 @@ int render_views(ilm_scene* s, ilm_nrc* nrc, const std::vector<ilm::View>& views
     ILM_HIP(hipMemcpyAsync(s->d_frame, s->h_frame, sizeof(s->h_frame), hipMemcpyHostToDevice, st));
     ILM_HIP(hipMemcpyAsync(s->d_views, views.data(), views.size() * sizeof(ilm::View), hipMemcpyHostToDevice, st));
     const int grid = ctx->cu_count * 8;
+    if (cache && !s->side) {
+        ILM_HIP(hipSetDevice(ctx->device));
+        ILM_HIP(hipStreamCreateWithFlags(&s->side, hipStreamNonBlocking));
+        ILM_HIP(hipEventCreateWithFlags(&s->fork_ev, hipEventDisableTiming));
+        ILM_HIP(hipEventCreateWithFlags(&s->join_ev, hipEventDisableTiming));
+    }
     // The frame's launch sequence (no host synchronisation inside, counters
     // cleared by a kernel rather than a memset node, so it can be captured).
     auto launch_frame = [&]() -> int {
@@ -1025,11 +1039,27 @@ int render_views(ilm_scene* s, ilm_nrc* nrc, const std::vector<ilm::View>& views
             // the context's threshold
             int r2 = ilm::trace_queue(s->mesh, a.rays[0], a.counts + 0, cap, a.hits, nullptr, st, true, 56);
             if (r2) return r2;
+            bool forked = false;  // the cache inference of this batch runs on the side stream
             for (int b = 0; b < max_bounces; ++b) {
                 if (cache)
                     hipLaunchKernelGGL(ilm::k_shade<true>, dim3(grid), dim3(ilm::kPtBlock), 0, st, a, b, cur);
                 else
                     hipLaunchKernelGGL(ilm::k_shade<false>, dim3(grid), dim3(ilm::kPtBlock), 0, st, a, b, cur);
+                if (cache && b == a.nrc_prefix) {
+                    // the batch's queries are complete: infer and resolve them
+                    // beside the training suffix, whose launches carry few rays.
+                    // Query paths end at the query vertex, so the suffix writes
+                    // no radiance of theirs; the batch's end waits for the side
+                    // stream before the training targets and the accumulation.
+                    ILM_HIP(hipEventRecord(s->fork_ev, st));
+                    ILM_HIP(hipStreamWaitEvent(s->side, s->fork_ev, 0));
+                    r2 = ilm::nrc_infer_queue(nrc, a.queries, a.counts + 5, cap, reinterpret_cast<float*>(a.qout),
+                                              s->side);
+                    if (r2) return r2;
+                    hipLaunchKernelGGL(ilm::k_nrc_resolve, dim3(grid), dim3(256), 0, s->side, a);
+                    ILM_HIP(hipEventRecord(s->join_ev, s->side));
+                    forked = true;
+                }
                 // this bounce's shadow segments and the next bounce's rays come
                 // from the same shading pass: one launch traces both
                 const bool next = b + 1 < max_bounces;
@@ -1047,10 +1077,16 @@ int render_views(ilm_scene* s, ilm_nrc* nrc, const std::vector<ilm::View>& views
                 cur ^= 1;
             }
             if (cache) {
-                // batched cache inference at the query vertices, then training targets
-                int r2 = ilm::nrc_infer_queue(nrc, a.queries, a.counts + 5, cap, reinterpret_cast<float*>(a.qout), st);
-                if (r2) return r2;
-                hipLaunchKernelGGL(ilm::k_nrc_resolve, dim3(grid), dim3(256), 0, st, a);
+                // batched cache inference at the query vertices (unless the side
+                // stream did it), then training targets
+                if (forked) {
+                    ILM_HIP(hipStreamWaitEvent(st, s->join_ev, 0));
+                } else {
+                    int r3 = ilm::nrc_infer_queue(nrc, a.queries, a.counts + 5, cap, reinterpret_cast<float*>(a.qout),
+                                                  st);
+                    if (r3) return r3;
+                    hipLaunchKernelGGL(ilm::k_nrc_resolve, dim3(grid), dim3(256), 0, st, a);
+                }
                 hipLaunchKernelGGL(ilm::k_nrc_collect, dim3((unsigned)((a.npaths + 255) / 256)), dim3(256), 0, st, a);
                 hipLaunchKernelGGL(ilm::k_nrc_reset_queries, dim3(1), dim3(1), 0, st, a.counts);
             }
