@@ -1007,6 +1007,13 @@ int render_views(ilm_scene* s, ilm_nrc* nrc, const std::vector<ilm::View>& views
     a.emit0 = mode ? (mode->emit0 ? 1 : 0) : 1;
     hipStream_t st = ctx->stream;
     int ev0 = 0, ev1 = 0;
+    if (cache && !s->side) {
+        // the side stream of cache frames, created before the frame's timing starts
+        ILM_HIP(hipSetDevice(ctx->device));
+        ILM_HIP(hipStreamCreateWithFlags(&s->side, hipStreamNonBlocking));
+        ILM_HIP(hipEventCreateWithFlags(&s->fork_ev, hipEventDisableTiming));
+        ILM_HIP(hipEventCreateWithFlags(&s->join_ev, hipEventDisableTiming));
+    }
     rc = ilm_timer_mark(ctx, &ev0);
     if (rc) return rc;
     // per-frame inputs: the only ones that may change between replays
@@ -1015,12 +1022,6 @@ int render_views(ilm_scene* s, ilm_nrc* nrc, const std::vector<ilm::View>& views
     ILM_HIP(hipMemcpyAsync(s->d_frame, s->h_frame, sizeof(s->h_frame), hipMemcpyHostToDevice, st));
     ILM_HIP(hipMemcpyAsync(s->d_views, views.data(), views.size() * sizeof(ilm::View), hipMemcpyHostToDevice, st));
     const int grid = ctx->cu_count * 8;
-    if (cache && !s->side) {
-        ILM_HIP(hipSetDevice(ctx->device));
-        ILM_HIP(hipStreamCreateWithFlags(&s->side, hipStreamNonBlocking));
-        ILM_HIP(hipEventCreateWithFlags(&s->fork_ev, hipEventDisableTiming));
-        ILM_HIP(hipEventCreateWithFlags(&s->join_ev, hipEventDisableTiming));
-    }
     // The frame's launch sequence (no host synchronisation inside, counters
     // cleared by a kernel rather than a memset node, so it can be captured).
     auto launch_frame = [&]() -> int {
