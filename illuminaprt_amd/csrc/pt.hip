@@ -132,7 +132,7 @@ struct View {
 
 struct PtArgs {
     // scene
-    const float4* shade;     // 6 float4 per triangle: (v0, mat), (v1, light), (v2, 0), n0, n1, n2
+    const float4* shade;     // 4 float4 per triangle (64 bytes): (Ng, mat), (n0, light), (n1, 0), (n2, 0)
     const float* mats;
     const float4* ltris;     // emitter triangles, 3 float4 each
     const float4* lights;    // (Le rgb, area)
@@ -319,16 +319,16 @@ __global__ void __launch_bounds__(kPtBlock) k_shade(PtArgs a, int bounce, int cu
         uint32_t sample = 0;
         int vi = 0;
         if (live) {
-            const float4* sr = a.shade + 6 * (int64_t)h.prim;
-            float4 A = sr[0], B = sr[1], C = sr[2], N0 = sr[3], N1 = sr[4], N2 = sr[5];
-            mat_id = __float_as_int(A.w);
-            light_id = __float_as_int(B.w);
+            const float4* sr = a.shade + 4 * (int64_t)h.prim;
+            float4 G = sr[0], N0 = sr[1], N1 = sr[2], N2 = sr[3];
+            mat_id = __float_as_int(G.w);
+            light_id = __float_as_int(N0.w);
             f3 rd = mk(ray.dx, ray.dy, ray.dz);
             P = mk(ray.ox, ray.oy, ray.oz) + h.t * rd;  // GeometricPrimitive.cpp:151
             float alpha = 1.f - h.u - h.v;
             Ns = normalize(mk(N0.x * alpha + N1.x * h.u + N2.x * h.v, N0.y * alpha + N1.y * h.u + N2.y * h.v,
                               N0.z * alpha + N1.z * h.u + N2.z * h.v));
-            Ng = normalize(cross(mk(B.x - A.x, B.y - A.y, B.z - A.z), mk(C.x - A.x, C.y - A.y, C.z - A.z)));
+            Ng = mk(G.x, G.y, G.z);
             wOut = -normalize(rd);
             float4 b4 = a.beta[p];
             beta = mk(b4.x, b4.y, b4.z);
@@ -801,24 +801,31 @@ int ilm_scene_create(ilm_ctx* ctx, ilm_mesh* mesh, const float* nrm, const int32
     if (!mesh->built) return fail(ILM_ERR_STATE, "build the mesh's BVH first");
     if (nmat <= 0) return fail(ILM_ERR_INVALID, "a scene needs at least one material");
     const int64_t nt = mesh->nt;
-    std::vector<float> shade(24 * nt);
+    // Shading record of a triangle, 64 bytes (half a cache line, one gather
+    // per hit): the geometric normal normalize((v1 - v0) x (v2 - v0))
+    // (IndexedTriangle.cpp:225-243), computed here once in float with
+    // separate products and sums, then the vertex normals; the material
+    // and light ids ride in the w words.
+    std::vector<float> shade(16 * nt);
     std::vector<std::vector<int64_t>> ltri(nlight);
     for (int64_t i = 0; i < nt; ++i) {
         int m = tri_mat[i], l = tri_light[i];
         if (m >= nmat || l >= nlight) return fail(ILM_ERR_INVALID, "material or light index out of range");
-        float* o = shade.data() + 24 * i;
+        float* o = shade.data() + 16 * i;
+        ilm::f3 p[3];
         for (int v = 0; v < 3; ++v) {
             int64_t vi = mesh->idx[3 * i + v];
-            for (int k = 0; k < 3; ++k) {
-                o[4 * v + k] = mesh->pos[3 * vi + k];
-                o[12 + 4 * v + k] = nrm[3 * vi + k];
-            }
-            o[12 + 4 * v + 3] = 0.f;
+            p[v] = ilm::mk(mesh->pos[3 * vi], mesh->pos[3 * vi + 1], mesh->pos[3 * vi + 2]);
+            for (int k = 0; k < 3; ++k) o[4 + 4 * v + k] = nrm[3 * vi + k];
+            o[4 + 4 * v + 3] = 0.f;
         }
+        const ilm::f3 ng = ilm::normalize(ilm::cross(p[1] - p[0], p[2] - p[0]));
+        o[0] = ng.x;
+        o[1] = ng.y;
+        o[2] = ng.z;
         int32_t mm = m < 0 ? -1 : m, ll = l < 0 ? -1 : l;
         memcpy(&o[3], &mm, 4);
         memcpy(&o[7], &ll, 4);
-        o[11] = 0.f;
         if (l >= 0) ltri[l].push_back(i);
     }
     std::vector<float> lt, lrec(4 * std::max(1, nlight));
@@ -879,6 +886,9 @@ int ilm_scene_create(ilm_ctx* ctx, ilm_mesh* mesh, const float* nrm, const int32
 
 int ilm_scene_destroy(ilm_scene* s) {
     if (!s) return ILM_OK;
+    // a frame that failed after its fork may have left cache work on the side
+    // stream, which reads the buffers freed below
+    if (s->side) (void)hipStreamSynchronize(s->side);
     scene_free_state(s);
     for (void* p : {(void*)s->d_shade, (void*)s->d_mats, (void*)s->d_ltris, (void*)s->d_lights, (void*)s->d_lrange,
                     (void*)s->d_frame, (void*)s->d_views})
@@ -1417,8 +1427,8 @@ int ilm_batch_create(ilm_ctx* ctx, ilm_scene* const* scenes, int n, ilm_batch** 
         }
         // shading: rebase material and light ids
         for (int64_t t = 0; t < m->nt; ++t) {
-            float rec[24];
-            memcpy(rec, s->h_shade.data() + 24 * t, sizeof rec);
+            float rec[16];
+            memcpy(rec, s->h_shade.data() + 16 * t, sizeof rec);
             int32_t mm, ll;
             memcpy(&mm, &rec[3], 4);
             memcpy(&ll, &rec[7], 4);
@@ -1426,7 +1436,7 @@ int ilm_batch_create(ilm_ctx* ctx, ilm_scene* const* scenes, int n, ilm_batch** 
             if (ll >= 0) ll += lb;
             memcpy(&rec[3], &mm, 4);
             memcpy(&rec[7], &ll, 4);
-            ms->h_shade.insert(ms->h_shade.end(), rec, rec + 24);
+            ms->h_shade.insert(ms->h_shade.end(), rec, rec + 16);
         }
         ms->h_mats.insert(ms->h_mats.end(), s->h_mats.begin(), s->h_mats.end());
         for (int l = 0; l < s->nlights; ++l) {
