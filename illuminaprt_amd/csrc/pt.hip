@@ -569,13 +569,30 @@ __global__ void k_nrc_reset_queries(unsigned long long* counts) {
 }
 
 // Start of a frame: no training samples yet (weights 0 mark unused slots).
+// The queries of the slots are cleared too: a slot that no training path
+// fills this frame still enters the Adam step with weight 0, and a
+// non-finite query there (uninitialised memory) would make its zero-weight
+// terms NaN instead of 0.
 __global__ void k_nrc_clear(PtArgs a) {
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k == 0) a.counts[6] = 0ull;
-    if (k < a.train_cap) a.tt[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (k < a.train_cap) {
+        a.tt[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+        float4* q = reinterpret_cast<float4*>(a.tq + 16 * k);
+        for (int j = 0; j < 4; ++j) q[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
 }
 
-__global__ void k_shadow_accum(PtArgs a) {
+__global__ void k_shadow_accum(PtArgs a, int cur) {
+    // The bounce's bookkeeping (k_end_bounce's, which scenes without lights
+    // launch instead): this kernel reads none of these counters, and the
+    // next shading pass, which sets the shadow queue's length, starts after it.
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        a.counts[3] += a.counts[cur];
+        a.counts[4] += a.counts[7];
+        a.counts[cur] = 0ull;
+        a.counts[7] = 0ull;
+    }
     // one thread per path vertex: its max_vl shadow slots are adjacent, and a
     // path has one vertex per bounce, so its radiance needs no atomics
     const int64_t count = (int64_t)(a.counts[2] / (unsigned long long)a.max_vl);
@@ -1083,8 +1100,10 @@ int render_views(ilm_scene* s, ilm_nrc* nrc, const std::vector<ilm::View>& views
                     r2 = ilm::trace_queue(s->mesh, a.rays[cur ^ 1], a.counts + (cur ^ 1), cap, a.hits, nullptr, st,
                                           true);
                 if (r2) return r2;
-                if (s->nlights > 0) hipLaunchKernelGGL(ilm::k_shadow_accum, dim3(grid), dim3(256), 0, st, a);
-                hipLaunchKernelGGL(ilm::k_end_bounce, dim3(1), dim3(1), 0, st, a.counts, cur);
+                if (s->nlights > 0)
+                    hipLaunchKernelGGL(ilm::k_shadow_accum, dim3(grid), dim3(256), 0, st, a, cur);
+                else
+                    hipLaunchKernelGGL(ilm::k_end_bounce, dim3(1), dim3(1), 0, st, a.counts, cur);
                 cur ^= 1;
             }
             if (cache) {

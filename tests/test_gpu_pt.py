@@ -128,6 +128,41 @@ def test_room_batches_and_determinism(gpu_ctx, ilm):
     assert np.array_equal(a, b)
 
 
+def test_cache_frames_batches_and_determinism(gpu_ctx, ilm):
+    """Cache frames split into many wavefront batches (each batch's cache
+    inference runs on the scene's second stream beside its training suffix)
+    give the image, the training-record count and the trained weights of
+    one-batch frames: samples, training paths and record slots are keyed by
+    the global path, not by the batch."""
+    from illuminaprt_amd import workloads as w
+    pos, nrm, idx, tm, tl, mats, lights = w.box_room()
+    cam = w.room_camera()
+    W = H = 64
+    m = ilm.Mesh(gpu_ctx, pos, idx)
+    m.build()
+    out = {}
+    for bp in (0, 1000):
+        sc = ilm.Scene(gpu_ctx, m, nrm, tm, tl, mats, lights)
+        net = ilm.Nrc(gpu_ctx, (0, 0, 0), (1, 1, 1), seed=3)
+        acc = gpu_ctx.buffer(W * H * 3 * 8).zero()
+        recs = []
+        for f in range(3):
+            st = sc.render_nrc(net, cam, W, H, 2, acc, seed=40 + f, spp_offset=2 * f, train_batch=256,
+                               train_steps=2, batch_paths=bp)
+            recs.append(st["train_records"])
+        out[bp] = (acc.download(np.float64, W * H * 3), net.weights(), recs)
+        acc.free()
+        net.close()
+        sc.close()
+    m.close()
+    assert all(r > 0 for r in out[0][2]) and out[0][2] == out[1000][2]
+    # the frames fill 287-294 of their 512 record slots: the unused slots
+    # train with weight 0 and must leave the weights finite
+    assert np.isfinite(out[0][1]).all() and np.isfinite(out[1000][1]).all()
+    assert np.array_equal(out[0][0], out[1000][0])
+    assert np.array_equal(out[0][1], out[1000][1])
+
+
 def test_graph_replays_match_eager(gpu_ctx, ilm):
     """A captured frame replayed with new seeds equals the eager frame with the
     same seed, on every replay, with eager work and synchronisation between
