@@ -25,6 +25,7 @@
 namespace ilm {
 
 constexpr int kPtBlock = 256;
+constexpr int kCounts = 16;  // the integrator's device counters (PtArgs::counts), cleared per frame
 
 struct Material {
     f3 kd, ks;
@@ -162,7 +163,9 @@ struct PtArgs {
     uint8_t* socc;
     // [0], [1] ray queues, [2] shadow queue, [3] extension total, [4] shadow total,
     // [5] cache queries of this batch, [6] training samples of this frame,
-    // [7] shadow segments written this bounce (the shadow queue's used slots)
+    // [7] shadow segments written this bounce (the shadow queue's used slots),
+    // [8] training paths at the cache's query vertex in this batch (the query
+    // bounce's shadow queue holds their slots only)
     unsigned long long* counts;
     double* accum;
     // radiance cache (nrc_prefix < 0: plain path tracing)
@@ -233,6 +236,7 @@ __global__ void k_generate(PtArgs a) {
         a.counts[0] = (unsigned long long)a.npaths;
         a.counts[1] = 0ull;
         a.counts[2] = 0ull;
+        a.counts[8] = 0ull;
     }
     if (p >= a.npaths) return;
     int vi;
@@ -286,16 +290,22 @@ __global__ void __launch_bounds__(kPtBlock) k_shade(PtArgs a, int bounce, int cu
     const int nxt = cur ^ 1;
     const int64_t count = (int64_t)a.counts[cur];
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    // the cache's query bounce: only training paths go on, so the shadow
+    // slots go to them alone, in append order (counter 8); its queue length
+    // is set after this pass (k_query_shadow_count)
+    const bool qbounce = CACHE && a.nrc_prefix >= 0 && bounce == a.nrc_prefix;
     // the shadow queue has max_vl fixed slots per queue entry (slot j: the
     // view's light j); unused slots hold an empty segment, so the queue needs
     // no append atomics. Its length is known up front.
-    if (blockIdx.x == 0 && threadIdx.x == 0) a.counts[2] = (unsigned long long)count * (unsigned long long)a.max_vl;
+    if (!qbounce && blockIdx.x == 0 && threadIdx.x == 0)
+        a.counts[2] = (unsigned long long)count * (unsigned long long)a.max_vl;
     uint32_t nshadow = 0;  // shadow segments this lane wrote (for the frame's ray count)
     // LDS of the block appends (0 continuing paths, 1 cache queries, 2
-    // training records), double-buffered by iteration parity: a wave may
-    // start the next iteration while another still reads this one's
-    __shared__ uint32_t s_cnt[3][2][kPtBlock / 64];
-    __shared__ unsigned long long s_base[3][2];
+    // training records, 3 the query bounce's shadow entries), double-buffered
+    // by iteration parity: a wave may start the next iteration while another
+    // still reads this one's
+    __shared__ uint32_t s_cnt[4][2][kPtBlock / 64];
+    __shared__ unsigned long long s_base[4][2];
     // iterate in whole-block steps so that every thread of the block reaches
     // the queue appends (and their barriers) together
     int par = 0;
@@ -383,7 +393,8 @@ __global__ void __launch_bounds__(kPtBlock) k_shade(PtArgs a, int bounce, int cu
             qrec[13] = mat.type == 1 ? mat.ks.z : 0.f;
             qrec[14] = 0.f; qrec[15] = 0.f;
         }
-        if (CACHE && a.nrc_prefix >= 0 && bounce == a.nrc_prefix) {  // uniform: the appends hold barriers
+        int64_t sentry = k;  // the vertex's shadow-queue entry (slots sentry * max_vl + j)
+        if (qbounce) {  // uniform: the appends hold barriers
             int64_t qs = block_append(&a.counts[5], to_cache, s_cnt[1][par], &s_base[1][par]);
             if (to_cache) {
                 float4* d = reinterpret_cast<float4*>(a.queries + 16 * qs);
@@ -393,6 +404,7 @@ __global__ void __launch_bounds__(kPtBlock) k_shade(PtArgs a, int bounce, int cu
             }
             // counts the frame's training records; the record's slot is the path's own
             (void)block_append(&a.counts[6], new_train, s_cnt[2][par], &s_base[2][par]);
+            sentry = block_append(&a.counts[8], new_train, s_cnt[3][par], &s_base[3][par]);
             if (new_train) {
                 const int64_t ts = training_slot(a, a.bpath0 + p, train_offset(a));
                 if (ts < a.train_cap) {
@@ -477,8 +489,8 @@ __global__ void __launch_bounds__(kPtBlock) k_shade(PtArgs a, int bounce, int cu
                     }
                 }
             }
-            if (k < count) {
-                const int64_t slot = k * a.max_vl + j;
+            if (sentry >= 0 && k < count) {
+                const int64_t slot = sentry * a.max_vl + j;
                 if (want) {
                     a.srays[slot] = sray;
                     a.scontrib[slot] = make_float4(contrib.x, contrib.y, contrib.z, __int_as_float(p));
@@ -564,6 +576,11 @@ __global__ void k_nrc_collect(PtArgs a) {
     if (s >= 0) a.tt[s] = make_float4(a.T[3 * p], a.T[3 * p + 1], a.T[3 * p + 2], 1.f);
 }
 
+// The query bounce's shadow queue: max_vl slots per training path there.
+__global__ void k_query_shadow_count(unsigned long long* counts, int max_vl) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) counts[2] = counts[8] * (unsigned long long)max_vl;
+}
+
 __global__ void k_nrc_reset_queries(unsigned long long* counts) {
     if (threadIdx.x == 0 && blockIdx.x == 0) counts[5] = 0ull;
 }
@@ -632,7 +649,7 @@ __global__ void k_shadow_accum(PtArgs a, int cur) {
 // Bookkeeping between bounces (a one-thread kernel, so the bounce loop needs
 // no host synchronisation and no memset nodes).
 __global__ void k_clear_counts(unsigned long long* counts) {
-    if (threadIdx.x < 8) counts[threadIdx.x] = 0ull;
+    if (threadIdx.x < kCounts) counts[threadIdx.x] = 0ull;
 }
 
 __global__ void k_end_bounce(unsigned long long* counts, int cur) {
@@ -791,7 +808,7 @@ int ensure_state(ilm_scene* s, int64_t cap, int nl, bool cache = false, int64_t 
     rc |= scene_alloc(s, cap * nl * sizeof(ilm_ray), (void**)&a.srays);
     rc |= scene_alloc(s, cap * nl * sizeof(float4), (void**)&a.scontrib);
     rc |= scene_alloc(s, cap * nl, (void**)&a.socc);
-    rc |= scene_alloc(s, 8 * sizeof(unsigned long long), (void**)&a.counts);
+    rc |= scene_alloc(s, ilm::kCounts * sizeof(unsigned long long), (void**)&a.counts);
     if (rc) {
         scene_free_state(s);
         return ILM_ERR_NOMEM;
@@ -1074,6 +1091,7 @@ int render_views(ilm_scene* s, ilm_nrc* nrc, const std::vector<ilm::View>& views
                 else
                     hipLaunchKernelGGL(ilm::k_shade<false>, dim3(grid), dim3(ilm::kPtBlock), 0, st, a, b, cur);
                 if (cache && b == a.nrc_prefix) {
+                    hipLaunchKernelGGL(ilm::k_query_shadow_count, dim3(1), dim3(1), 0, st, a.counts, max_vl);
                     // the batch's queries are complete: infer and resolve them
                     // beside the training suffix, whose launches carry few rays.
                     // Query paths end at the query vertex, so the suffix writes
