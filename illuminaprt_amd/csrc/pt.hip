@@ -25,6 +25,10 @@
 namespace ilm {
 
 constexpr int kPtBlock = 256;
+// Rounds of resident workgroups per shading pass: one (each thread walks its
+// share of the queue) measured fastest; 2 and 4 rounds, and 8 workgroups per
+// CU whatever fits, were slower.
+constexpr int kShadeRounds = 1;
 constexpr int kCounts = 16;  // the integrator's device counters (PtArgs::counts), cleared per frame
 
 struct Material {
@@ -1066,6 +1070,13 @@ int render_views(ilm_scene* s, ilm_nrc* nrc, const std::vector<ilm::View>& views
     ILM_HIP(hipMemcpyAsync(s->d_frame, s->h_frame, sizeof(s->h_frame), hipMemcpyHostToDevice, st));
     ILM_HIP(hipMemcpyAsync(s->d_views, views.data(), views.size() * sizeof(ilm::View), hipMemcpyHostToDevice, st));
     const int grid = ctx->cu_count * 8;
+    // The shading pass's grid is whole rounds of resident workgroups (its
+    // register use allows two or three per CU): a partial last round would
+    // leave most of the chip idle at the end of each pass.
+    int shade_per_cu = 0;
+    ILM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &shade_per_cu, cache ? (const void*)ilm::k_shade<true> : (const void*)ilm::k_shade<false>, ilm::kPtBlock, 0));
+    const int shade_grid = ctx->cu_count * std::max(shade_per_cu, 1) * ilm::kShadeRounds;
     // The frame's launch sequence (no host synchronisation inside, counters
     // cleared by a kernel rather than a memset node, so it can be captured).
     auto launch_frame = [&]() -> int {
@@ -1087,9 +1098,9 @@ int render_views(ilm_scene* s, ilm_nrc* nrc, const std::vector<ilm::View>& views
             bool forked = false;  // the cache inference of this batch runs on the side stream
             for (int b = 0; b < max_bounces; ++b) {
                 if (cache)
-                    hipLaunchKernelGGL(ilm::k_shade<true>, dim3(grid), dim3(ilm::kPtBlock), 0, st, a, b, cur);
+                    hipLaunchKernelGGL(ilm::k_shade<true>, dim3(shade_grid), dim3(ilm::kPtBlock), 0, st, a, b, cur);
                 else
-                    hipLaunchKernelGGL(ilm::k_shade<false>, dim3(grid), dim3(ilm::kPtBlock), 0, st, a, b, cur);
+                    hipLaunchKernelGGL(ilm::k_shade<false>, dim3(shade_grid), dim3(ilm::kPtBlock), 0, st, a, b, cur);
                 if (cache && b == a.nrc_prefix) {
                     hipLaunchKernelGGL(ilm::k_query_shadow_count, dim3(1), dim3(1), 0, st, a.counts, max_vl);
                     // the batch's queries are complete: infer and resolve them
