@@ -29,7 +29,8 @@ constexpr int kPtBlock = 256;
 // share of the queue) measured fastest; 2 and 4 rounds, and 8 workgroups per
 // CU whatever fits, were slower.
 constexpr int kShadeRounds = 1;
-constexpr int kCounts = 16;  // the integrator's device counters (PtArgs::counts), cleared per frame
+constexpr int kCounts = 16;
+constexpr int kAccumSlots = 4;  // shadow slots per vertex summed from one batch of loads  // the integrator's device counters (PtArgs::counts), cleared per frame
 
 struct Material {
     f3 kd, ks;
@@ -621,7 +622,38 @@ __global__ void k_shadow_accum(PtArgs a, int cur) {
     for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < count; k += stride) {
         int64_t p = -1;
         float lx = 0.f, ly = 0.f, lz = 0.f, tx = 0.f, ty = 0.f, tz = 0.f;
-        for (int j = 0; j < a.max_vl; ++j) {
+        if (a.max_vl <= kAccumSlots) {
+            // the vertex's slots are loaded together, before any is tested, so
+            // their shared cache lines are fetched once; then summed in slot
+            // order as in the loop below
+            uint8_t occ[kAccumSlots];
+            float4 c[kAccumSlots], t[kAccumSlots] = {};
+#pragma unroll
+            for (int j = 0; j < kAccumSlots; ++j) {
+                if (j < a.max_vl) {
+                    const int64_t slot = k * a.max_vl + j;
+                    occ[j] = a.socc[slot];
+                    c[j] = a.scontrib[slot];
+                    if (a.nrc_prefix >= 0) t[j] = a.scontrib2[slot];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < kAccumSlots; ++j) {
+                if (j >= a.max_vl || occ[j]) continue;
+                const int32_t pj = __float_as_int(c[j].w);
+                if (pj < 0) continue;  // an unused slot
+                p = pj;
+                lx += c[j].x;
+                ly += c[j].y;
+                lz += c[j].z;
+                if (a.nrc_prefix >= 0) {
+                    tx += t[j].x;
+                    ty += t[j].y;
+                    tz += t[j].z;
+                }
+            }
+        }
+        for (int j = a.max_vl <= kAccumSlots ? a.max_vl : 0; j < a.max_vl; ++j) {
             const int64_t slot = k * a.max_vl + j;
             if (a.socc[slot]) continue;
             const float4 c = a.scontrib[slot];
